@@ -1,0 +1,171 @@
+"""Benchmark: fault x pattern evaluations per second on one MI355X.
+
+A step is one run() of the fault-simulation path over the resident workload:
+good machine over every round, then the fault sweeps and detection (with fault
+dropping where the configuration asks for it). Circuit parsing and the upload
+to HBM happen before the timed region.
+
+Prints one JSON line (rank 0). See DESIGN.md for the byte model behind
+`roofline` and the sample behind `cpu_baseline`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: float) -> dict:
+    """Time the reference simulator (oracle/_ref/fsim_ref, SEQUENTIAL, 1 core)
+    on a bounded sample: a seeded subset of faults over the first rounds."""
+    import numpy as np
+
+    import oracle
+
+    P = preset["num_pis"]
+    lines = flst.split(b"\n")
+    F = int(lines[0])
+    N = preset["num_pis"] + preset["num_pos"] + preset["num_gates"]
+    rounds_total = (preset["num_patterns"] + 63) // 64
+    # cost of one fault-round is N node evaluations; aim at ~budget_s seconds
+    # assuming >= 2e8 node evaluations per second on one core
+    fr_budget = max(4, int(budget_s * 2e8 / N))
+    n_rounds = max(1, min(rounds_total, 8, fr_budget // 4))
+    n_faults = max(1, min(F, fr_budget // n_rounds))
+    rng = np.random.default_rng(4242)
+    pick = np.sort(rng.choice(F, size=n_faults, replace=False))
+    sub_flst = (f"{n_faults}\n".encode() + b"\n".join(lines[1 + i] for i in pick) + b"\n")
+    ptn_lines = ptn.split(b"\n")
+    n_pat = min(preset["num_patterns"], 64 * n_rounds)
+    sub_ptn = f"{n_pat}\n".encode() + b"\n".join(ptn_lines[1:1 + n_rounds]) + b"\n"
+    sample = (f"{n_faults} seeded faults x {n_rounds} rounds ({n_pat} patterns) of the full circuit "
+              f"({N} nodes)")
+    if oracle.ref_available():
+        _, secs = oracle.run_reference(ckt, sub_flst, sub_ptn, mode=0, threads=1, want_output=False)
+        kind = "reference"
+    else:
+        o = oracle.Oracle(ckt, sub_flst, sub_ptn)
+        t0 = time.perf_counter()
+        o.run_literal(threads=1)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    return {"value": n_faults * n_pat / secs, "unit": "fault*pattern evals/s", "cores": 1, "kind": kind,
+            "sample": sample, "seconds": round(secs, 3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fault set would shard "
+              "across GPUs by fault, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import lsim_amd
+    from lsim_amd import workloads
+
+    preset = workloads.describe(args.config)
+    t0 = time.perf_counter()
+    ckt, flst, ptn = workloads.texts(args.config)
+    log(f"[bench] generated {args.config} in {time.perf_counter() - t0:.1f}s")
+    sim = lsim_amd.Simulator(device=0)
+    sim.set_option("drop", 1 if preset["drop"] else 0)
+    sim.set_option("report_last_round", 0)
+    t0 = time.perf_counter()
+    sim.read_text(ckt, flst, ptn)
+    log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")
+    mode = lsim_amd.Simulator.Mode.PLAIN if args.mode == "plain" else lsim_amd.Simulator.Mode.PARTITIONED
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        sim.run(mode)
+        log(f"[bench] warmup {i} {time.perf_counter() - t0:.3f}s")
+    times = []
+    agg = None
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        sim.run(mode)  # returns after the device work of the step has completed
+        times.append(time.perf_counter() - t0)
+        st = sim.last_run_stats()
+        if agg is None:
+            agg = {k: 0 for k in st}
+        for k, v in st.items():
+            agg[k] += v
+        log(f"[bench] step {i} {times[-1]:.3f}s sweep {st['t_sweep_ms']:.1f}ms detect {st['t_detect_ms']:.1f}ms "
+            f"good {st['t_good_ms']:.1f}ms launches {st['sweep_launches']} stems {st['stems']}")
+    elapsed = time.perf_counter() - t_all
+    F = sim.num_faults()
+    n = sim.num_patterns()
+    ms = 1000.0 * elapsed / args.steps
+    value = F * n / (elapsed / args.steps)
+    launches = max(1, agg["sweep_launches"])
+    t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
+    bytes_launch = agg["sweep_bytes"] / launches
+    achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
+    detected = int(sim.detected().sum())
+    out = {
+        "metric": "fault x pattern evals/sec",
+        "value": value,
+        "unit": "fault*pattern evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: seeded DAG {preset['num_gates']} gates / {preset['num_pis']} PI / "
+                        f"{preset['num_pos']} PO / {preset['num_levels']} levels, {F} SA0/SA1 faults, "
+                        f"{n} patterns, fault dropping {'on' if preset['drop'] else 'off'}",
+            "engine": args.mode,
+            "faults": F,
+            "patterns": n,
+            "detected": detected,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_sweep_persistent" if args.mode == "partitioned" else "k_sweep_level",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": t_launch_s * 1000.0,
+        },
+        "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(ckt, flst, ptn, preset, args.cpu_budget)
+        except Exception as e:  # reported, never fatal for the GPU figure
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,167 @@
+/*
+ * lsim: MI355X (gfx950) stuck-at fault simulator - C ABI.
+ *
+ * Drop-in for the reference simulation path fsim::Simulator
+ * (fsim/simulation.hpp:39-188): the same .ckt/.flst/.ptn inputs, the same
+ * read / run / accessor sequence and the same detected-fault report
+ * (found_fault_to_pattern, fsim/simulation.hpp:87-89). Every entry point
+ * returns 0 on success or a negative LSIM_E_* code; lsim_last_error() gives
+ * the message. No C++ exception crosses this boundary. Output buffers are
+ * caller-allocated and their element counts are checked.
+ *
+ * One handle is one simulator; calls on one handle must not overlap.
+ */
+#ifndef LSIM_H
+#define LSIM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define LSIM_API __attribute__((visibility("default")))
+#else
+#define LSIM_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSIM_OK 0
+#define LSIM_E_ARG (-1)    /* bad argument or buffer size */
+#define LSIM_E_INPUT (-2)  /* malformed or unsupported circuit/fault/pattern text */
+#define LSIM_E_IO (-3)     /* a file could not be opened */
+#define LSIM_E_STATE (-4)  /* call out of order (e.g. run before read) */
+#define LSIM_E_DEVICE (-5) /* HIP runtime error or no usable GPU */
+#define LSIM_E_NOMEM (-6)  /* device memory exhausted */
+
+typedef struct lsim_sim lsim_sim;
+
+/* Engines. All run on the GPU.
+ *  PLAIN:        per-level kernel launches (good machine and fault sweeps).
+ *  PARTITIONED:  one persistent launch per sweep; each workgroup owns a tile
+ *                and walks the levels with workgroup barriers.
+ * The reference's CPU modes (fsim/simulation.hpp:45) map onto them:
+ * SEQUENTIAL(0) and PARALLEL_OMP(2) -> PLAIN, PARALLEL_TF(1) -> PARTITIONED. */
+#define LSIM_MODE_PLAIN 0
+#define LSIM_MODE_PARTITIONED 1
+#define LSIM_MODE_REF_SEQUENTIAL 0
+#define LSIM_MODE_REF_PARALLEL_TF 1
+#define LSIM_MODE_REF_PARALLEL_OMP 2
+
+/* Create a simulator bound to HIP device `device`.
+ * Replaces constructing fsim::Simulator (fsim/simulation.hpp:39). */
+LSIM_API int lsim_create(int device, lsim_sim **out);
+/* Replaces fsim::Simulator's destructor. */
+LSIM_API void lsim_destroy(lsim_sim *s);
+/* Message for the last failed call on `s` (or of lsim_create when s is NULL). */
+LSIM_API const char *lsim_last_error(const lsim_sim *s);
+
+/* Replaces Simulator::read(const std::string&, const std::string&, const
+ * std::string&) (fsim/simulation.cpp:12-35): LSIM_E_IO when a file cannot be
+ * opened, with the reference's message text. */
+LSIM_API int lsim_read_files(lsim_sim *s, const char *ckt_path, const char *flst_path, const char *ptn_path);
+/* Replaces Simulator::read(std::istream&, std::istream&, std::istream&)
+ * (fsim/simulation.cpp:37-63) for in-memory text. */
+LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len,
+                      const char *ptn, size_t ptn_len);
+
+/* Options (no reference counterpart; the reference has compile-time flags).
+ *  "drop"               0/1  fault dropping: a detected fault is not simulated
+ *                            in later rounds (default 0, as the reference).
+ *  "report_last_round"  0/1  with dropping on, also evaluate every fault in the
+ *                            last round so found_fault_to_pattern stays exact
+ *                            (default 1).
+ *  "scratch_mb"         MiB of device memory the fault sweeps may use for
+ *                            per-tile state (default 0 = automatic).
+ *  "direct"             0/1  inject every fault directly instead of stem
+ *                            tracing (default 0; forced on for circuits with MUX).
+ *  "block_waves"        waves per workgroup of the persistent sweep (default 16). */
+LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
+
+/* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).
+ * num_threads is accepted for signature parity and ignored. */
+LSIM_API int lsim_run(lsim_sim *s, int mode, size_t num_threads);
+
+/* Counts: Simulator::num_PIs/num_POs/num_gates/num_wires/num_rounds
+ * (fsim/simulation.hpp:75-80), faults().size() (:70) and
+ * num_gates_per_level().size() (:84). */
+LSIM_API int64_t lsim_num_pis(const lsim_sim *s);
+LSIM_API int64_t lsim_num_pos(const lsim_sim *s);
+LSIM_API int64_t lsim_num_gates(const lsim_sim *s);
+LSIM_API int64_t lsim_num_wires(const lsim_sim *s);
+LSIM_API int64_t lsim_num_rounds(const lsim_sim *s);
+LSIM_API int64_t lsim_num_faults(const lsim_sim *s);
+LSIM_API int64_t lsim_num_levels(const lsim_sim *s);
+LSIM_API int64_t lsim_num_patterns(const lsim_sim *s);
+
+/* Simulator::gate_idx_to_tpg_order_idx() (fsim/simulation.hpp:81-83); n = P+O+G. */
+LSIM_API int lsim_gate_idx_to_tpg_order_idx(const lsim_sim *s, uint64_t *out, size_t n);
+/* Simulator::num_gates_per_level() (fsim/simulation.hpp:84-86); n = lsim_num_levels. */
+LSIM_API int lsim_num_gates_per_level(const lsim_sim *s, uint64_t *out, size_t n);
+/* Levels of pi(i), gate(i), po(i) in topological order (ElementBase::level(),
+ * fsim/base_ppg.hpp:49); n = P+G+O. */
+LSIM_API int lsim_levels(const lsim_sim *s, uint64_t *out, size_t n);
+/* Simulator::faults() (fsim/simulation.hpp:70): site position and stuck value
+ * as the reference stores it (0 or all ones); n = number of faults. */
+LSIM_API int lsim_faults(const lsim_sim *s, uint64_t *site, uint64_t *value, size_t n);
+/* Simulator::patterns() (fsim/simulation.hpp:71): rounds x PIs words. */
+LSIM_API int lsim_patterns(const lsim_sim *s, uint64_t *out, size_t n);
+
+/* Simulator::g_pi_results/g_gate_results/g_po_results (fsim/simulation.hpp:58-60):
+ * good-machine values of the last round, masked to its live bits. */
+LSIM_API int lsim_good_results(const lsim_sim *s, uint64_t *pi, size_t n_pi, uint64_t *gate, size_t n_gate, uint64_t *po,
+                      size_t n_po);
+/* Simulator::found_fault_to_pattern() (fsim/simulation.hpp:87-89): 2*F words,
+ * {found in the last round, last round index}. LSIM_E_STATE when dropping was
+ * on and report_last_round off. */
+LSIM_API int lsim_found_fault_to_pattern(const lsim_sim *s, uint64_t *out, size_t n);
+/* No reference counterpart: detected in any round (1/0) per fault. */
+LSIM_API int lsim_detected(const lsim_sim *s, uint8_t *out, size_t n);
+/* No reference counterpart: first detecting round per fault, -1 if never. */
+LSIM_API int lsim_first_detect_round(const lsim_sim *s, int64_t *out, size_t n);
+/* No reference counterpart: d(f, r) for every fault and round, n = F*R bytes,
+ * row-major by fault. LSIM_E_STATE when the run used fault dropping. */
+LSIM_API int lsim_detect_matrix(const lsim_sim *s, uint8_t *out, size_t n);
+
+/* Timing and traffic of the last run (no reference counterpart). */
+typedef struct {
+  double t_total_ms;       /* run() wall time after the circuit is resident */
+  double t_good_ms;        /* good-machine kernels (device time) */
+  double t_sweep_ms;       /* fault sweep kernels (device time) */
+  double t_detect_ms;      /* stem-tracing / detection kernels (device time) */
+  int64_t sweep_launches;  /* number of sweep kernel launches */
+  int64_t sweep_tiles;     /* tiles processed by the sweeps */
+  int64_t sweep_node_steps;/* (tile, node) pairs visited by the sweeps */
+  int64_t sweep_bytes;     /* bytes of node operands touched by the sweeps */
+  int64_t good_bytes;      /* algorithmic bytes of the good-machine kernels */
+  int64_t live_faults;     /* faults whose site can reach a PO */
+  int64_t stems;           /* stem sweeps injected (summed over steps) */
+  int64_t steps;           /* word steps executed */
+  int64_t slots;           /* concurrent sweep tiles (scratch slots) */
+  int64_t live_nodes;
+} lsim_run_stats;
+LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
+
+/* Seeded synthetic workload in the reference formats (no reference
+ * counterpart; the reference ships fsim/circuit_generator.py and
+ * fsim/generate_fault_patterns.py as separate producers). Buffers are
+ * allocated by the library and released with lsim_free_text. */
+typedef struct {
+  uint64_t seed;
+  int64_t num_pis, num_pos, num_gates, num_levels;
+  int32_t fanout_cap;    /* 0 = uniform source weights, no cap */
+  double power_exp;      /* source-weight power law exponent (2.2) */
+  double locality;       /* P(second fanin from the last `window` levels) (0.7) */
+  int32_t window;        /* 3 */
+  double unary_frac;     /* INV/CLKBUF share (0.1) */
+  int64_t num_patterns;
+  double fault_fraction; /* (0,1]: seeded subset of the full SA0/SA1 list */
+} lsim_gen_spec;
+LSIM_API int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char **flst, size_t *flst_len, char **ptn,
+                  size_t *ptn_len);
+LSIM_API void lsim_free_text(char *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

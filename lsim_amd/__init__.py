@@ -1,0 +1,334 @@
+"""lsim_amd: MI355X-native stuck-at fault simulator.
+
+Python view of the C ABI in include/lsim.h. `Simulator` mirrors the reference's
+fsim::Simulator (fsim/simulation.hpp:39-188): read the .ckt/.flst/.ptn inputs,
+run, then read the same accessors (found_fault_to_pattern, good values, order
+map, levels, faults, patterns). All simulation runs in the HIP library
+liblsim.so; there is no CPU execution path, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from typing import Optional, Tuple, Union
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblsim.so")
+
+LSIM_OK = 0
+LSIM_E_ARG = -1
+LSIM_E_INPUT = -2
+LSIM_E_IO = -3
+LSIM_E_STATE = -4
+LSIM_E_DEVICE = -5
+LSIM_E_NOMEM = -6
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class LsimError(RuntimeError):
+    """Error returned by the native library (code is an LSIM_E_* value)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("t_total_ms", ctypes.c_double),
+        ("t_good_ms", ctypes.c_double),
+        ("t_sweep_ms", ctypes.c_double),
+        ("t_detect_ms", ctypes.c_double),
+        ("sweep_launches", ctypes.c_int64),
+        ("sweep_tiles", ctypes.c_int64),
+        ("sweep_node_steps", ctypes.c_int64),
+        ("sweep_bytes", ctypes.c_int64),
+        ("good_bytes", ctypes.c_int64),
+        ("live_faults", ctypes.c_int64),
+        ("stems", ctypes.c_int64),
+        ("steps", ctypes.c_int64),
+        ("slots", ctypes.c_int64),
+        ("live_nodes", ctypes.c_int64),
+    ]
+
+
+class GenSpec(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("num_pis", ctypes.c_int64),
+        ("num_pos", ctypes.c_int64),
+        ("num_gates", ctypes.c_int64),
+        ("num_levels", ctypes.c_int64),
+        ("fanout_cap", ctypes.c_int32),
+        ("power_exp", ctypes.c_double),
+        ("locality", ctypes.c_double),
+        ("window", ctypes.c_int32),
+        ("unary_frac", ctypes.c_double),
+        ("num_patterns", ctypes.c_int64),
+        ("fault_fraction", ctypes.c_double),
+    ]
+
+
+def lib() -> ctypes.CDLL:
+    """Load liblsim.so (built by lsim_amd/build.py); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    sz = ctypes.c_size_t
+    sig = {
+        "lsim_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "lsim_destroy": (None, [P]),
+        "lsim_last_error": (ctypes.c_char_p, [P]),
+        "lsim_read_files": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+        "lsim_read_buffers": (ctypes.c_int, [P, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz]),
+        "lsim_set_option": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int64]),
+        "lsim_run": (ctypes.c_int, [P, ctypes.c_int, sz]),
+        "lsim_num_pis": (ctypes.c_int64, [P]),
+        "lsim_num_pos": (ctypes.c_int64, [P]),
+        "lsim_num_gates": (ctypes.c_int64, [P]),
+        "lsim_num_wires": (ctypes.c_int64, [P]),
+        "lsim_num_rounds": (ctypes.c_int64, [P]),
+        "lsim_num_faults": (ctypes.c_int64, [P]),
+        "lsim_num_levels": (ctypes.c_int64, [P]),
+        "lsim_num_patterns": (ctypes.c_int64, [P]),
+        "lsim_gate_idx_to_tpg_order_idx": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_num_gates_per_level": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_levels": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_faults": (ctypes.c_int, [P, u64p, u64p, sz]),
+        "lsim_patterns": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_good_results": (ctypes.c_int, [P, u64p, sz, u64p, sz, u64p, sz]),
+        "lsim_found_fault_to_pattern": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_detected": (ctypes.c_int, [P, u8p, sz]),
+        "lsim_first_detect_round": (ctypes.c_int, [P, i64p, sz]),
+        "lsim_detect_matrix": (ctypes.c_int, [P, u8p, sz]),
+        "lsim_last_run_stats": (ctypes.c_int, [P, ctypes.POINTER(RunStats)]),
+        "lsim_generate": (
+            ctypes.c_int,
+            [
+                ctypes.POINTER(GenSpec),
+                ctypes.POINTER(ctypes.c_void_p),
+                ctypes.POINTER(sz),
+                ctypes.POINTER(ctypes.c_void_p),
+                ctypes.POINTER(sz),
+                ctypes.POINTER(ctypes.c_void_p),
+                ctypes.POINTER(sz),
+            ],
+        ),
+        "lsim_free_text": (None, [ctypes.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list:
+    """Names of the C ABI entry points declared in include/lsim.h."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "lsim.h")
+    with open(hdr) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"LSIM_API[^;(]*?\b(lsim_\w+)\s*\(", text)))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _as_bytes(x: Union[str, bytes]) -> bytes:
+    return x.encode() if isinstance(x, str) else bytes(x)
+
+
+class Simulator:
+    """Mirror of fsim::Simulator backed by the GPU engine.
+
+    device=-1 gives a host-only handle: read() and the structural accessors
+    work, run() raises LsimError(LSIM_E_DEVICE)."""
+
+    class Mode(enum.IntEnum):
+        # reference names (fsim/simulation.hpp:45) and the engine names
+        SEQUENTIAL = 0
+        PARALLEL_TF = 1
+        PARALLEL_OMP = 2
+        PLAIN = 0
+        PARTITIONED = 1
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.lsim_create(device, ctypes.byref(h))
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_last_error(None).decode())
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.lsim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_last_error(self._h).decode())
+
+    # -- input (fsim/simulation.cpp:12-63)
+    def read(self, ckt_path: str, flst_path: str, ptn_path: str) -> None:
+        self._check(self._L.lsim_read_files(self._h, ckt_path.encode(), flst_path.encode(), ptn_path.encode()))
+
+    def read_text(self, ckt, flst, ptn) -> None:
+        c, f, p = _as_bytes(ckt), _as_bytes(flst), _as_bytes(ptn)
+        self._check(self._L.lsim_read_buffers(self._h, c, len(c), f, len(f), p, len(p)))
+
+    def set_option(self, key: str, value: int) -> None:
+        self._check(self._L.lsim_set_option(self._h, key.encode(), int(value)))
+
+    # -- run (fsim/simulation.cpp:65-92)
+    def run(self, mode: int = 0, num_threads: int = 1) -> None:
+        self._check(self._L.lsim_run(self._h, int(mode), int(num_threads)))
+
+    # -- accessors (fsim/simulation.hpp:58-89)
+    def num_PIs(self) -> int:
+        return self._L.lsim_num_pis(self._h)
+
+    def num_POs(self) -> int:
+        return self._L.lsim_num_pos(self._h)
+
+    def num_gates(self) -> int:
+        return self._L.lsim_num_gates(self._h)
+
+    def num_wires(self) -> int:
+        return self._L.lsim_num_wires(self._h)
+
+    def num_rounds(self) -> int:
+        return self._L.lsim_num_rounds(self._h)
+
+    def num_faults(self) -> int:
+        return self._L.lsim_num_faults(self._h)
+
+    def num_patterns(self) -> int:
+        return self._L.lsim_num_patterns(self._h)
+
+    def _u64(self, fn, n: int) -> np.ndarray:
+        out = np.zeros(n, dtype=np.uint64)
+        self._check(fn(self._h, _ptr(out, ctypes.c_uint64), n))
+        return out
+
+    def gate_idx_to_tpg_order_idx(self) -> np.ndarray:
+        n = self.num_PIs() + self.num_POs() + self.num_gates()
+        return self._u64(self._L.lsim_gate_idx_to_tpg_order_idx, n)
+
+    def num_gates_per_level(self) -> np.ndarray:
+        return self._u64(self._L.lsim_num_gates_per_level, self._L.lsim_num_levels(self._h))
+
+    def levels(self) -> np.ndarray:
+        """Levels of pi(i), gate(i), po(i) concatenated (topological order)."""
+        n = self.num_PIs() + self.num_POs() + self.num_gates()
+        return self._u64(self._L.lsim_levels, n)
+
+    def faults(self) -> Tuple[np.ndarray, np.ndarray]:
+        n = self.num_faults()
+        site = np.zeros(n, dtype=np.uint64)
+        val = np.zeros(n, dtype=np.uint64)
+        self._check(self._L.lsim_faults(self._h, _ptr(site, ctypes.c_uint64), _ptr(val, ctypes.c_uint64), n))
+        return site, val
+
+    def patterns(self) -> np.ndarray:
+        R, P = self.num_rounds(), self.num_PIs()
+        return self._u64(self._L.lsim_patterns, R * P).reshape(R, P)
+
+    def good_results(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        P, G, O = self.num_PIs(), self.num_gates(), self.num_POs()
+        pi, gate, po = (np.zeros(n, dtype=np.uint64) for n in (P, G, O))
+        u = ctypes.c_uint64
+        self._check(self._L.lsim_good_results(self._h, _ptr(pi, u), P, _ptr(gate, u), G, _ptr(po, u), O))
+        return pi, gate, po
+
+    def g_pi_results(self) -> np.ndarray:
+        return self.good_results()[0]
+
+    def g_gate_results(self) -> np.ndarray:
+        return self.good_results()[1]
+
+    def g_po_results(self) -> np.ndarray:
+        return self.good_results()[2]
+
+    def found_fault_to_pattern(self) -> np.ndarray:
+        return self._u64(self._L.lsim_found_fault_to_pattern, 2 * self.num_faults())
+
+    def detected(self) -> np.ndarray:
+        out = np.zeros(self.num_faults(), dtype=np.uint8)
+        self._check(self._L.lsim_detected(self._h, _ptr(out, ctypes.c_uint8), out.size))
+        return out
+
+    def first_detect_round(self) -> np.ndarray:
+        out = np.zeros(self.num_faults(), dtype=np.int64)
+        self._check(self._L.lsim_first_detect_round(self._h, _ptr(out, ctypes.c_int64), out.size))
+        return out
+
+    def detect_matrix(self) -> np.ndarray:
+        F, R = self.num_faults(), self.num_rounds()
+        out = np.zeros(F * R, dtype=np.uint8)
+        self._check(self._L.lsim_detect_matrix(self._h, _ptr(out, ctypes.c_uint8), out.size))
+        return out.reshape(F, R)
+
+    def last_run_stats(self) -> dict:
+        st = RunStats()
+        self._check(self._L.lsim_last_run_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in RunStats._fields_}
+
+
+def generate(
+    seed: int,
+    num_pis: int,
+    num_pos: int,
+    num_gates: int,
+    num_levels: int,
+    num_patterns: int,
+    fanout_cap: int = 0,
+    power_exp: float = 2.2,
+    locality: float = 0.7,
+    window: int = 3,
+    unary_frac: float = 0.1,
+    fault_fraction: float = 1.0,
+) -> Tuple[bytes, bytes, bytes]:
+    """Seeded synthetic (.ckt, .flst, .ptn) texts, produced by the library."""
+    L = lib()
+    spec = GenSpec(seed, num_pis, num_pos, num_gates, num_levels, fanout_cap, power_exp, locality, window,
+                   unary_frac, num_patterns, fault_fraction)
+    bufs = [ctypes.c_void_p() for _ in range(3)]
+    lens = [ctypes.c_size_t() for _ in range(3)]
+    rc = L.lsim_generate(ctypes.byref(spec), ctypes.byref(bufs[0]), ctypes.byref(lens[0]), ctypes.byref(bufs[1]),
+                         ctypes.byref(lens[1]), ctypes.byref(bufs[2]), ctypes.byref(lens[2]))
+    if rc != LSIM_OK:
+        raise LsimError(rc, L.lsim_last_error(None).decode() or "generator rejected the spec")
+    out = []
+    for b, n in zip(bufs, lens):
+        out.append(ctypes.string_at(b.value, n.value))
+        L.lsim_free_text(b)
+    return out[0], out[1], out[2]

@@ -1,0 +1,123 @@
+"""GPU parity: the HIP engine against the reference's vectors and the CPU oracle.
+
+Bit-exact on every detection bit d(f, r), on the reference's literal report
+found_fault_to_pattern (fsim/simulation.cpp:142-144) and on the good-machine
+values of the last round (fsim/simulation.cpp:163-178).
+"""
+import numpy as np
+import pytest
+
+import lsim_amd
+import oracle
+from lsim_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+MODES = [lsim_amd.Simulator.Mode.PLAIN, lsim_amd.Simulator.Mode.PARTITIONED]
+MODE_IDS = ["plain", "partitioned"]
+
+
+def _sim(ckt, flst, ptn, **opts):
+    s = lsim_amd.Simulator(device=0)
+    for k, v in opts.items():
+        s.set_option(k, v)
+    s.read_text(ckt, flst, ptn)
+    return s
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_reference_unit_vectors(unit_vectors, mode):
+    """unittests/test_basics.cpp:528-605: good values and fault table."""
+    for c in unit_vectors:
+        s = _sim(c["ckt"], c["flst"], c["ptn"])
+        s.run(mode, 4)
+        pi, gate, po = s.good_results()
+        assert list(pi) == c["pi"], c["name"]
+        assert list(gate) == c["gate"], c["name"]
+        assert list(po) == c["po"], c["name"]
+        assert list(s.found_fault_to_pattern()) == c["table"], c["name"]
+
+
+@pytest.mark.parametrize("direct", [0, 1], ids=["stem_tracing", "direct"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_reference_made_vectors(ref_vectors, mode, direct):
+    """Per-round detection and report against vectors written by the reference."""
+    for c in ref_vectors:
+        s = _sim(c["ckt"], c["flst"], c["ptn"], direct=direct)
+        s.run(mode)
+        F, R = s.num_faults(), s.num_rounds()
+        det = np.array(c["det"], dtype=np.uint8).reshape(F, R)
+        assert np.array_equal(s.detect_matrix(), det), c["name"]
+        assert list(s.found_fault_to_pattern()) == c["table"], c["name"]
+        pi, gate, po = s.good_results()
+        assert list(gate) == c["g_gate"] and list(po) == c["g_po"] and list(pi) == c["g_pi"], c["name"]
+        first = s.first_detect_round()
+        want = np.where(det.any(1), det.argmax(1), -1) if R else -np.ones(F, dtype=np.int64)
+        assert np.array_equal(first, want), c["name"]
+
+
+@pytest.fixture(scope="module")
+def c1():
+    ckt, flst, ptn = workloads.texts("c1")
+    ref = oracle.Oracle(ckt, flst, ptn).run_literal(threads=0)
+    return ckt, flst, ptn, ref
+
+
+@pytest.mark.parametrize("direct", [0, 1], ids=["stem_tracing", "direct"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_c1_full_against_oracle(c1, mode, direct):
+    """5k gates, 10k faults, 16 rounds: every d(f, r) against the literal restatement."""
+    ckt, flst, ptn, ref = c1
+    s = _sim(ckt, flst, ptn, direct=direct)
+    s.run(mode)
+    assert np.array_equal(s.detect_matrix(), ref["det"])
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    pi, gate, po = s.good_results()
+    assert np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_c1_fault_dropping(c1, mode):
+    """With dropping, the first detecting round and the detected set equal the full matrix's."""
+    ckt, flst, ptn, ref = c1
+    s = _sim(ckt, flst, ptn, drop=1, report_last_round=1)
+    s.run(mode)
+    det = ref["det"]
+    want = np.where(det.any(1), det.argmax(1), -1)
+    assert np.array_equal(s.first_detect_round(), want)
+    assert np.array_equal(s.detected(), det.any(1).astype(np.uint8))
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    with pytest.raises(lsim_amd.LsimError):
+        s.detect_matrix()
+
+
+@pytest.fixture(scope="module")
+def c2():
+    ckt, flst, ptn = workloads.texts("c2")
+    return ckt, flst, ptn
+
+
+def test_c2_sampled_against_oracle(c2):
+    """100k gates, 200k faults, 128 rounds: all 128 rounds of a seeded fault
+    sample against the event-driven restatement, and the engines against each
+    other on every fault."""
+    ckt, flst, ptn = c2
+    runs = {}
+    for name, mode, direct in (("plain", MODES[0], 0), ("partitioned", MODES[1], 0), ("direct", MODES[1], 1)):
+        s = _sim(ckt, flst, ptn, direct=direct)
+        s.run(mode)
+        runs[name] = s.detect_matrix()
+        if name == "plain":
+            table = s.found_fault_to_pattern()
+            good = s.good_results()
+        s.close()
+    assert np.array_equal(runs["plain"], runs["partitioned"])
+    assert np.array_equal(runs["plain"], runs["direct"])
+    o = oracle.Oracle(ckt, flst, ptn)
+    rng = np.random.default_rng(2002)
+    sample = np.sort(rng.choice(o.F, size=160, replace=False))
+    ev = o.run_event(threads=0, faults=sample)
+    assert np.array_equal(runs["plain"][sample], ev[sample])
+    R = o.R
+    assert np.array_equal(table[0::2][sample], ev[sample, R - 1])
+    assert np.all(table[1::2] == R - 1)

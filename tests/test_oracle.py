@@ -1,0 +1,171 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+* Against the reference's own unit-test answers (unittests/test_basics.cpp).
+* Against vectors written by the reference simulator compiled from its
+  sources (tests/golden/ref_vectors.json), including per-round detection.
+* Against an independent per-pattern boolean simulation in pure Python
+  (no 64-bit word packing) on small circuits: catches a wrong mask, a
+  dropped round, a swapped fanin or a wrong gate truth table.
+* The event-driven restatement against the literal one.
+* Live against the compiled reference where it is present.
+"""
+import numpy as np
+import pytest
+
+import lsim_amd
+import oracle
+
+
+def test_unit_vectors(unit_vectors):
+    for c in unit_vectors:
+        o = oracle.Oracle(c["ckt"], c["flst"], c["ptn"])
+        P, O, G, W, R = c["num_vars"]
+        assert (o.P, o.O, o.G, o.count(oracle.NUM_WIRES), o.R) == (P, O, G, W, R), c["name"]
+        assert list(o.order_map()) == c["tpg"], c["name"]
+        assert list(o.gates_per_level()) == c["per_level"], c["name"]
+        assert list(o.levels()) == c["pi_lvl"] + c["gate_lvl"] + c["po_lvl"], c["name"]
+        site, val = o.faults()
+        assert o.F == c["fault_size"]
+        tpg = c["tpg"]
+        for i in range(o.F):
+            assert site[i] == tpg[c["faults"][2 * i]]
+            assert val[i] == (2**64 - 1 if c["faults"][2 * i + 1] == 1 else 0)
+        assert list(o.patterns()[0]) == c["ptn_val"], c["name"]
+        r = o.run_literal(threads=1)
+        assert list(r["g_pi"]) == c["pi"] and list(r["g_gate"]) == c["gate"] and list(r["g_po"]) == c["po"]
+        assert list(r["table"]) == c["table"], c["name"]
+
+
+def test_reference_made_vectors(ref_vectors):
+    for c in ref_vectors:
+        o = oracle.Oracle(c["ckt"], c["flst"], c["ptn"])
+        r = o.run_literal(threads=2)
+        det = np.array(c["det"], dtype=np.uint8).reshape(o.F, o.R)
+        assert np.array_equal(r["det"], det), c["name"]
+        assert list(r["table"]) == c["table"], c["name"]
+        assert list(o.order_map()) == c["order_map"], c["name"]
+        assert list(o.levels()) == c["levels"], c["name"]
+        assert list(o.gates_per_level()) == c["gates_per_level"], c["name"]
+        assert list(r["g_pi"]) == c["g_pi"] and list(r["g_gate"]) == c["g_gate"] and list(r["g_po"]) == c["g_po"]
+        assert np.array_equal(o.run_event(threads=2), det), c["name"]
+
+
+def _boolean_detect(o: oracle.Oracle, ckt: str) -> np.ndarray:
+    """d(f, r) from single-pattern boolean simulation of the faulty circuit,
+    written from the gate truth tables with no word-level operations."""
+    toks = [int(x) for x in ckt.split()]
+    P, O, G, W = toks[:4]
+    N = P + O + G
+    pos = o.order_map()
+    order = np.empty(N, dtype=np.int64)
+    order[pos] = np.arange(N)
+    fanin = {}
+    i = 4
+    for _ in range(W):
+        src, _pin, k = toks[i:i + 3]
+        i += 3
+        for _ in range(k):
+            dst, pin = toks[i:i + 2]
+            i += 2
+            fanin.setdefault(dst, {})[3 if pin == 5 else pin] = src
+    types = toks[i:i + G]
+    typ = {P + O + g: t for g, t in enumerate(types)}
+    for j in range(P, P + O):
+        typ[j] = 10
+    pats = o.patterns()
+    site, val = o.faults()
+    F, R = o.F, o.R
+    n = o.count(oracle.NUM_PATTERNS)
+    det = np.zeros((F, R), dtype=np.uint8)
+
+    def simulate(p_bits, fault_node, fault_bit):
+        v = {}
+        for p in range(N):
+            nid = int(order[p])
+            if nid == fault_node:
+                v[nid] = fault_bit
+                continue
+            if nid < P:
+                v[nid] = p_bits[nid]
+                continue
+            ins = [v[fanin[nid][k]] for k in sorted(fanin[nid])]
+            t = typ[nid]
+            if t == 0:
+                out = 1 - ins[0]
+            elif t in (1, 4):
+                out = int(all(ins))
+                out = out if t == 1 else 1 - out
+            elif t in (2, 5):
+                out = int(any(ins))
+                out = out if t == 2 else 1 - out
+            elif t in (3, 6):
+                out = sum(ins) % 2
+                out = out if t == 3 else 1 - out
+            else:  # CLKBUF, PO
+                out = ins[0]
+            v[nid] = out
+        return [v[P + k] for k in range(O)]
+
+    for r in range(R):
+        live = 64 if (n // (64 * (r + 1))) else n % 64
+        for b in range(live):
+            bits = [int(pats[r][k] >> np.uint64(b)) & 1 for k in range(P)]
+            good = simulate(bits, -1, 0)
+            for f in range(F):
+                node = int(order[int(site[f])])
+                if simulate(bits, node, 1 if val[f] else 0) != good:
+                    det[f, r] = 1
+    return det
+
+
+@pytest.mark.parametrize("name", ["two_input_multi_round", "partial_last_round", "wide_gates",
+                                  "duplicate_fanins", "one_round_exact"])
+def test_against_boolean_brute_force(ref_vectors, name):
+    """MUX-free cases: per-pattern boolean simulation fixes d(f, r) without
+    the word packing, the mask arithmetic or the reference's code."""
+    c = next(c for c in ref_vectors if c["name"] == name)
+    o = oracle.Oracle(c["ckt"], c["flst"], c["ptn"])
+    assert np.array_equal(o.run_literal(threads=1)["det"], _boolean_detect(o, c["ckt"]))
+
+
+def test_mux_word_level_select():
+    """fsim/simulation.cpp:1056: (s & b) | (a & !s) with !s of the whole word.
+    One MUX: a=PI0, b=PI1, s=PI2 -> PO. With s != 0 the output is s & b; with
+    s == 0 it is a & 1 (only pattern 0 of a survives)."""
+    ckt = "3 1 1 4 0 0 1 4 1 1 0 1 4 2 2 0 1 4 5 4 0 1 3 1 7"
+    flst = "0"
+    for a, b, s_, want in ((0b1011, 0b0110, 0b0100, 0b0100), (0b1011, 0b0110, 0, 0b1), (0b1010, 5, 0, 0)):
+        ptn = f"4 {a} {b} {s_}"
+        r = oracle.Oracle(ckt, flst, ptn).run_literal(threads=1)
+        assert int(r["g_po"][0]) == want
+
+
+def test_event_matches_literal_on_generated():
+    ckt, flst, ptn = lsim_amd.generate(seed=99, num_pis=24, num_pos=12, num_gates=1500, num_levels=30,
+                                       num_patterns=64 * 5 + 21, fanout_cap=30)
+    o = oracle.Oracle(ckt, flst, ptn)
+    assert np.array_equal(o.run_event(threads=4), o.run_literal(threads=4)["det"])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference binary not built here")
+def test_live_against_compiled_reference():
+    ckt, flst, ptn = lsim_amd.generate(seed=5, num_pis=20, num_pos=10, num_gates=800, num_levels=25,
+                                       num_patterns=3 * 64 + 7, fanout_cap=40)
+    ref, _ = oracle.run_reference(ckt, flst, ptn, mode=0, threads=1)
+    o = oracle.Oracle(ckt, flst, ptn)
+    r = o.run_literal(threads=2)
+    assert np.array_equal(ref["table"], r["table"])
+    assert np.array_equal(ref["order_map"], o.order_map())
+    assert np.array_equal(ref["levels"], o.levels())
+    assert np.array_equal(ref["g_gate"], r["g_gate"]) and np.array_equal(ref["g_po"], r["g_po"])
+
+
+def test_subset_run_matches_full():
+    ckt, flst, ptn = lsim_amd.generate(seed=3, num_pis=16, num_pos=8, num_gates=600, num_levels=20,
+                                       num_patterns=200)
+    o = oracle.Oracle(ckt, flst, ptn)
+    full = o.run_literal(threads=2)["det"]
+    sel = [3, 10, 11, 500, 1199]
+    part = o.run_literal(threads=2, faults=sel)["det"]
+    ev = o.run_event(threads=2, faults=sel)
+    assert np.array_equal(part[sel], full[sel]) and np.array_equal(ev[sel], full[sel])
