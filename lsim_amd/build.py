@@ -45,6 +45,17 @@ def _compile(src: str) -> str:
     return out
 
 
+def build_facade_check() -> str:
+    """The C++ facade test driver (tests/cpp/facade_check.cpp), linked to liblsim.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "facade_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), "-o", out, src,
+                    "-L" + HERE, "-llsim", "-Wl,-rpath,$ORIGIN/../../../lsim_amd"], check=True)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -53,6 +64,7 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_facade_check()
     if verbose:
         print("built", LIB)
     return LIB
