@@ -9,23 +9,34 @@
 //    (lane = round word), one launch per level.
 //  * Faults whose site cannot reach a PO are never detected and are not
 //    simulated.
-//  * Stem tracing (circuits without MUX): inside a fanout-free region a
-//    flip of node x reaches the region's head h exactly where every gate on
-//    the path is sensitised, so d(x/v) = (good_x ^ v) & sens(x->h) & D_h,
-//    with D_h the patterns where flipping h changes a PO. D_h of a PO is all
-//    ones; D_h of a stem comes from an explicit forward sweep of the flip.
-//    Gates are bitwise per pattern, so this is exact. The reference's MUX
-//    uses the word-level `!s` (fsim/simulation.cpp:1056), which couples the
-//    64 patterns of a word; circuits with a MUX inject every fault directly.
-//  * Sweeps: a tile is up to 64 lanes, lane = (injection, word). A wave
-//    evaluates one node for all lanes; per node the tile keeps a 64-bit mask
-//    of lanes whose value differs from the good machine and a 64-word row of
-//    faulty values. A node is skipped when no fanin differs for any lane.
-//    Tiles sweep the live nodes level by level from the first injection.
+//  * Stem tracing (circuits without MUX): inside a fanout-free region a flip
+//    of node x reaches the region's head h exactly on the patterns where
+//    every gate on the path is sensitised, so
+//        d(x/v) = L_x & D_h,   L_x = (good_x ^ v) & sens(x->h) & live bits,
+//    with D_h the patterns on which flipping h changes a PO: all ones for a
+//    PO head, an explicit forward sweep for a stem head. The sweep of stem h
+//    only needs the bits U_h = OR of L over the step's faults in its region,
+//    so it flips h on U_h alone and (stem, word) pairs with U_h = 0 are not
+//    swept at all. Gates are bitwise per pattern, so this is exact. The
+//    reference's MUX uses the word-level `!s` (fsim/simulation.cpp:1056),
+//    which couples the 64 patterns of a word; circuits with a MUX inject
+//    every fault directly (full word), skipping words where the site's good
+//    value already equals the stuck value.
+//  * Sweeps: a tile holds up to 64 lanes, each lane one (injection site,
+//    word, injected bits). A wave evaluates one node for all lanes of a tile;
+//    per node the tile keeps a 64-bit mask of the lanes whose value differs
+//    from the good machine and a 64-word row of faulty values.
+//      - plain form: one launch per level, every live node of the level
+//        visited by every tile of the batch;
+//      - event form: one persistent launch; a workgroup owns a tile at a
+//        time and visits only nodes with a differing fanin, compacted per
+//        level into an LDS worklist, ending at the deepest pending level.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -52,13 +63,30 @@ namespace {
 
 enum : uint32_t { K_SA0 = 0, K_SA1 = 1, K_FLIP = 2 };
 
+// One sweep lane: inject at `site` in word `word`; for K_FLIP the good value
+// is complemented on the bits of `mask`, for K_SA0/K_SA1 the whole word is
+// forced. The OR over POs of (faulty ^ good) goes to out[out].
+struct Lane {
+  unsigned long long mask;
+  int32_t site;
+  int32_t word;
+  int32_t out;
+  uint32_t kind;
+};
+
 struct Tile {
-  int32_t inj_begin;   // first injection of the tile
-  int32_t K;           // injections in the tile
-  int32_t Wt;          // words per injection; lane = k * Wt + j
-  int32_t word0;       // first word
-  int32_t start_level; // level of the first (lowest) injection site
-  int32_t out_begin;   // out[out_begin + lane]
+  int32_t lane0;        // first lane record of the tile
+  int32_t nlanes;       // 1..64
+  int32_t start_level;  // level of the lowest injection site
+  int32_t end_level;    // level of the highest injection site
+};
+
+// Per-tile record of a node that differs from the good machine in some lane:
+// the lane mask, valid only when `epoch` equals the tile's epoch.
+struct MaskRec {
+  unsigned long long mask;
+  uint32_t epoch;
+  uint32_t pad;
 };
 
 __device__ __forceinline__ uint64_t eval_gate(uint32_t type, uint32_t nin, uint64_t v0, uint64_t v1, uint64_t v2,
@@ -147,42 +175,59 @@ struct SweepArgs {
   int max_level;
   const Tile *__restrict__ tiles;
   int ntiles;
-  const int32_t *__restrict__ inj_site;
-  const uint8_t *__restrict__ inj_kind;
+  const Lane *__restrict__ lanes;
   uint64_t *rows;    // per slot: [NL][64] faulty words
-  uint64_t *masks;   // per slot: [NL] lane masks
+  uint64_t *masks;   // per slot: [NL] lane masks (plain form)
   int64_t rows_stride, masks_stride;
-  uint64_t *out;     // [injection][word] OR over POs of (faulty ^ good)
-  int *queue;        // persistent form: next tile
-  unsigned long long *bytes;  // operand bytes touched (see sweep_node)
+  uint64_t *out;     // lane results
+  int *queue;        // event form: next tile
+  unsigned long long *bytes;  // operand bytes touched
+  MaskRec *mrec;     // per slot: [NL] (event form)
+  uint32_t *pend;    // per slot: [NL], == epoch when the node must be evaluated
+  int64_t rec_stride, pend_stride;
+  const int32_t *__restrict__ fo_off;  // live fanouts, ascending position
+  const int2 *__restrict__ fo;         // {position, level}
+  uint32_t epoch_base;
 };
 
 struct LaneSetup {
-  int site;
-  uint32_t kind;
+  unsigned long long mask;
   int64_t w;
+  int site;
+  int out;
+  uint32_t kind;
   int active;
 };
 
 __device__ __forceinline__ LaneSetup lane_setup(const SweepArgs &a, const Tile &T, int lane) {
   LaneSetup s;
-  s.active = T.K * T.Wt;
-  const bool on = lane < s.active;
-  const int k = on ? lane / T.Wt : 0;
-  const int j = on ? lane - k * T.Wt : 0;
-  s.site = on ? a.inj_site[T.inj_begin + k] : -1;
-  s.kind = on ? a.inj_kind[T.inj_begin + k] : 0u;
-  s.w = int64_t(T.word0) + j;
+  s.active = T.nlanes;
+  if (lane < T.nlanes) {
+    const Lane L = a.lanes[T.lane0 + lane];
+    s.mask = L.mask;
+    s.w = L.word;
+    s.site = L.site;
+    s.out = L.out;
+    s.kind = L.kind;
+  } else {
+    const Lane L = a.lanes[T.lane0];  // idle lane: a valid word, never injected
+    s.mask = 0;
+    s.w = L.word;
+    s.site = -1;
+    s.out = -1;
+    s.kind = K_SA0;
+  }
   return s;
 }
 
-// Evaluates live node u for the 64 lanes of one tile; `base` is the first
-// live position of the tile's start level: positions below it lie outside
-// every injection's fanout and hold good values for all lanes.
-// `bytes` accumulates (wave-uniformly) the operand bytes the node visit
-// touches: node record and fanin masks always; for an evaluated node also one
-// good or faulty word per fanin and the node's good word per active lane, and
-// the faulty words written.
+__device__ __forceinline__ uint64_t inject(const LaneSetup &s, uint64_t gu) {
+  return s.kind == K_FLIP ? (gu ^ s.mask) : (s.kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+}
+
+// Plain form, one (tile, node) pair per wave. `base` is the first live
+// position of the tile's start level: positions below it lie outside every
+// injection's fanout and hold good values in all lanes. Every node at or after
+// `base` is visited and writes its lane mask, so masks need no clearing.
 __device__ __forceinline__ void sweep_node(const SweepArgs &a, int u, int base, int lane, const LaneSetup &s,
                                            uint64_t *rows, uint64_t *masks, uint64_t &acc, uint64_t &bytes) {
   u = __builtin_amdgcn_readfirstlane(u);
@@ -214,7 +259,7 @@ __device__ __forceinline__ void sweep_node(const SweepArgs &a, int u, int base, 
     if (nin > 3) v3 = ((m3 >> lane) & 1) ? rows[int64_t(f.w) * 64 + lane] : a.good[int64_t(f.w) * RW + s.w];
     r = eval_gate(type, nin, v0, v1, v2, v3);
   }
-  if (inj) r = s.kind == K_FLIP ? ~gu : (s.kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+  if (inj) r = inject(s, gu);
   const uint64_t d = r ^ gu;
   const uint64_t nm = __ballot(d != 0);
   if (d) rows[int64_t(u) * 64 + lane] = r;
@@ -240,39 +285,126 @@ __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
   uint64_t *masks = a.masks + int64_t(tb) * a.masks_stride;
   uint64_t acc = 0, bytes = 0;
   sweep_node(a, u, a.live_lvl[T.start_level], lane, s, rows, masks, acc, bytes);
-  if (acc && lane < s.active) atomicOr((unsigned long long *)&a.out[T.out_begin + lane], (unsigned long long)acc);
+  if (acc && lane < s.active) atomicOr((unsigned long long *)&a.out[s.out], (unsigned long long)acc);
   if (lane == 0) atomicAdd(a.bytes, (unsigned long long)bytes);
 }
 
-// Persistent form: a workgroup pulls tiles from a queue and sweeps the levels
-// of each with a workgroup barrier between levels. Every workgroup leaves the
-// loop once the queue index passes the last tile.
-__global__ void __launch_bounds__(1024) k_sweep_persistent(SweepArgs a) {
+// Event form: evaluation of pending node u for the 64 lanes of one tile.
+// Fanin lane masks come from MaskRec entries stamped with this tile's epoch
+// (anything else means "no lane differs"); when some lane differs, the node's
+// live fanouts are marked pending and the deepest pending level is raised.
+__device__ __forceinline__ void event_node(const SweepArgs &a, int u, int lane, const LaneSetup &s, uint32_t epoch,
+                                           uint64_t *rows, MaskRec *mrec, uint32_t *pend, uint64_t &acc,
+                                           uint64_t &bytes, int *s_maxl) {
+  u = __builtin_amdgcn_readfirstlane(u);
+  const uint32_t inf = a.info[u];
+  const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
+  const int4 f = a.fin[u];
+  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  if (type != G_PI) {
+    const MaskRec r0 = mrec[f.x];
+    m0 = r0.epoch == epoch ? r0.mask : 0;
+    if (nin > 1) {
+      const MaskRec r = mrec[f.y];
+      m1 = r.epoch == epoch ? r.mask : 0;
+    }
+    if (nin > 2) {
+      const MaskRec r = mrec[f.z];
+      m2 = r.epoch == epoch ? r.mask : 0;
+    }
+    if (nin > 3) {
+      const MaskRec r = mrec[f.w];
+      m3 = r.epoch == epoch ? r.mask : 0;
+    }
+  }
+  const int64_t RW = a.RW;
+  const uint64_t gu = a.good[int64_t(u) * RW + s.w];
+  uint64_t r = gu;
+  if (type != G_PI) {
+    const uint64_t v0 = ((m0 >> lane) & 1) ? rows[int64_t(f.x) * 64 + lane] : a.good[int64_t(f.x) * RW + s.w];
+    uint64_t v1 = 0, v2 = 0, v3 = 0;
+    if (nin > 1) v1 = ((m1 >> lane) & 1) ? rows[int64_t(f.y) * 64 + lane] : a.good[int64_t(f.y) * RW + s.w];
+    if (nin > 2) v2 = ((m2 >> lane) & 1) ? rows[int64_t(f.z) * 64 + lane] : a.good[int64_t(f.z) * RW + s.w];
+    if (nin > 3) v3 = ((m3 >> lane) & 1) ? rows[int64_t(f.w) * 64 + lane] : a.good[int64_t(f.w) * RW + s.w];
+    r = eval_gate(type, nin, v0, v1, v2, v3);
+  }
+  if (s.site == u) r = inject(s, gu);
+  const uint64_t d = r ^ gu;
+  const uint64_t nm = __ballot(d != 0);
+  bytes += 20 + 16 * nin + uint64_t(s.active) * 8 * (type == G_PI ? 1 : nin + 1);
+  if (nm) {
+    if (d) rows[int64_t(u) * 64 + lane] = r;
+    if (lane == 0) {
+      MaskRec rec;
+      rec.mask = nm;
+      rec.epoch = epoch;
+      rec.pad = 0;
+      mrec[u] = rec;
+    }
+    const int e0 = a.fo_off[u], e1 = a.fo_off[u + 1];
+    for (int e = e0 + lane; e < e1; e += 64) pend[a.fo[e].x] = epoch;
+    if (e1 > e0 && lane == 0) atomicMax(s_maxl, a.fo[e1 - 1].y);
+    bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
+  }
+  if (type == G_PO) acc |= d;
+}
+
+// Event form, persistent: a workgroup pulls tiles from a queue. For each tile
+// it walks the levels from the lowest injection; at each level the nodes
+// marked pending for this tile are compacted into an LDS list and spread over
+// the waves; the walk ends after the deepest level any pending node sits on.
+// Pending flags and mask records carry the tile's epoch, so nothing is cleared
+// between tiles. Every workgroup leaves once the queue passes the last tile.
+constexpr int kChunk = 1024;
+__global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
   __shared__ unsigned long long acc_lds[64];
-  __shared__ int s_tile;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  __shared__ int s_tile, s_cnt, s_maxl;
+  __shared__ int s_list[kChunk];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6, nwaves = blockDim.x >> 6;
   uint64_t *rows = a.rows + int64_t(blockIdx.x) * a.rows_stride;
-  uint64_t *masks = a.masks + int64_t(blockIdx.x) * a.masks_stride;
+  MaskRec *mrec = a.mrec + int64_t(blockIdx.x) * a.rec_stride;
+  uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.pend_stride;
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.queue, 1);
-    if (threadIdx.x < 64) acc_lds[threadIdx.x] = 0;
+    if (tid == 0) s_tile = atomicAdd(a.queue, 1);
+    if (tid < 64) acc_lds[tid] = 0;
     __syncthreads();
     const int t = s_tile;
     if (t >= a.ntiles) return;
     const Tile T = a.tiles[t];
     const LaneSetup s = lane_setup(a, T, lane);
-    const int base = a.live_lvl[T.start_level];
+    const uint32_t epoch = a.epoch_base + uint32_t(t) + 1u;
+    if (tid < 64 && s.site >= 0) pend[s.site] = epoch;
+    if (tid == 0) s_maxl = T.end_level;
     uint64_t acc = 0, bytes = 0;
-    for (int l = T.start_level; l <= a.max_level; ++l) {
+    __syncthreads();
+    for (int l = T.start_level; l <= s_maxl; ++l) {
       const int lo = a.live_lvl[l], hi = a.live_lvl[l + 1];
-      for (int u = lo + wave; u < hi; u += nwaves) sweep_node(a, u, base, lane, s, rows, masks, acc, bytes);
-      __syncthreads();
+      for (int c0 = lo; c0 < hi; c0 += blockDim.x) {
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        const int u = c0 + tid;
+        const bool p = u < hi && pend[u] == epoch;
+        const uint64_t b = __ballot(p);
+        bytes += 4 * uint64_t(__popcll(__ballot(u < hi)));
+        if (b) {
+          int base = 0;
+          if (lane == 0) base = atomicAdd(&s_cnt, __popcll(b));
+          base = __shfl(base, 0);
+          if (p) s_list[base + __popcll(b & ((1ull << lane) - 1ull))] = u;
+        }
+        __syncthreads();
+        const int n = s_cnt;
+        for (int i = wave; i < n; i += nwaves)
+          event_node(a, s_list[i], lane, s, epoch, rows, mrec, pend, acc, bytes, &s_maxl);
+        __syncthreads();
+      }
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, (unsigned long long)bytes);
     if (acc) atomicOr(&acc_lds[lane], (unsigned long long)acc);
     __syncthreads();
-    if (threadIdx.x < s.active) a.out[T.out_begin + threadIdx.x] = acc_lds[threadIdx.x];
+    if (tid < s.active) a.out[s.out] = acc_lds[tid];
     __syncthreads();
   }
 }
@@ -285,13 +417,16 @@ struct DetectArgs {
   const uint64_t *__restrict__ good;
   int64_t RW;
   const int32_t *__restrict__ parent;     // per live position, -1 at region heads
-  const int32_t *__restrict__ head_slot;  // per live position: stem's injection index this step, else -1
-  const uint64_t *__restrict__ dout;      // [injection][Wn]
+  const int32_t *__restrict__ head_slot;  // per live position: the stem's unit index in this step, else -1
+  const uint8_t *__restrict__ is_po;      // per live position
+  uint64_t *need;                         // [unit][Wn]: bits a unit's sweep must cover
+  const uint64_t *__restrict__ dout;      // [unit][Wn]: sweep results
   int64_t w0;
   int Wn;
-  const int32_t *__restrict__ flist;      // faults of this step (direct: injection order)
+  const int32_t *__restrict__ flist;      // faults of this step (direct: unit k is fault flist[k])
   int nf;
   const int32_t *__restrict__ site;       // per fault, live position
+  const int32_t *__restrict__ head;       // per fault, region head (live position)
   const uint8_t *__restrict__ sa;
   int64_t num_patterns;
   int32_t *first_round;                   // atomicMin, or null
@@ -325,31 +460,62 @@ __device__ __forceinline__ uint64_t sens(const DetectArgs &a, int g, int x, int6
   return r0 ^ r1;
 }
 
-// Stem tracing: thread = (fault, word).
-__global__ void k_cpt_detect(DetectArgs a) {
+// L = activation & path sensitisation to the region head & live bits.
+__device__ __forceinline__ uint64_t local_term(const DetectArgs &a, int f, int64_t w) {
+  const int x0 = a.site[f];
+  uint64_t D = (a.good[int64_t(x0) * a.RW + w] ^ (a.sa[f] ? ~uint64_t(0) : uint64_t(0))) &
+               round_mask(a.num_patterns, w);
+  for (int x = x0, g = a.parent[x]; g >= 0 && D; x = g, g = a.parent[x]) D &= sens(a, g, x, w);
+  return D;
+}
+
+// Stem tracing, phase 1: thread = (fault, word). A fault whose region ends at
+// a PO is decided here; otherwise its bits are added to its stem's need.
+__global__ void k_local_cpt(DetectArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= int64_t(a.nf) * a.Wn) return;
   const int fi = int(t / a.Wn);
   const int j = int(t - int64_t(fi) * a.Wn);
   const int f = a.flist[fi];
-  const int x0 = a.site[f];
   const int64_t w = a.w0 + j;
-  uint64_t D = ~uint64_t(0);
-  int x = x0;
-  for (int g = a.parent[x]; g >= 0 && D; g = a.parent[x]) {
-    D &= sens(a, g, x, w);
-    x = g;
+  const uint64_t L = local_term(a, f, w);
+  if (!L) return;
+  const int h = a.head[f];
+  if (a.is_po[h]) {
+    record(a, f, w, L);
+  } else {
+    atomicOr((unsigned long long *)&a.need[int64_t(a.head_slot[h]) * a.Wn + j], (unsigned long long)L);
   }
-  if ((a.info[x] & 15u) != G_PO) {
-    const int slot = a.head_slot[x];
-    D &= slot >= 0 ? a.dout[int64_t(slot) * a.Wn + j] : 0;
-  }
-  const uint64_t act = a.good[int64_t(x0) * a.RW + w] ^ (a.sa[f] ? ~uint64_t(0) : uint64_t(0));
-  record(a, f, w, act & D & round_mask(a.num_patterns, w));
 }
 
-// Direct injection: thread = (injection, word); the sweep output is the PO diff.
-__global__ void k_direct_detect(DetectArgs a) {
+// Stem tracing, phase 2: d = L & D_stem.
+__global__ void k_detect_cpt(DetectArgs a) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(a.nf) * a.Wn) return;
+  const int fi = int(t / a.Wn);
+  const int j = int(t - int64_t(fi) * a.Wn);
+  const int f = a.flist[fi];
+  const int h = a.head[f];
+  if (a.is_po[h]) return;
+  const uint64_t D = a.dout[int64_t(a.head_slot[h]) * a.Wn + j];
+  if (!D) return;
+  const int64_t w = a.w0 + j;
+  record(a, f, w, local_term(a, f, w) & D);
+}
+
+// Direct injection, phase 1: a word needs a lane iff the stuck value differs
+// from the site's good value somewhere in the full word.
+__global__ void k_local_direct(DetectArgs a) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(a.nf) * a.Wn) return;
+  const int k = int(t / a.Wn);
+  const int j = int(t - int64_t(k) * a.Wn);
+  const int f = a.flist[k];
+  a.need[t] = a.good[int64_t(a.site[f]) * a.RW + a.w0 + j] ^ (a.sa[f] ? ~uint64_t(0) : uint64_t(0));
+}
+
+// Direct injection, phase 2: the sweep result is the PO difference.
+__global__ void k_detect_direct(DetectArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= int64_t(a.nf) * a.Wn) return;
   const int k = int(t / a.Wn);
@@ -358,9 +524,9 @@ __global__ void k_direct_detect(DetectArgs a) {
   record(a, a.flist[k], w, a.dout[t] & round_mask(a.num_patterns, w));
 }
 
-__global__ void k_set_slots(int32_t *head_slot, const int32_t *inj_site, int n, int clear) {
+__global__ void k_set_slots(int32_t *head_slot, const int32_t *units, int n, int clear) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) head_slot[inj_site[i]] = clear ? -1 : i;
+  if (i < n) head_slot[units[i]] = clear ? -1 : i;
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -403,20 +569,28 @@ struct Engine::Impl {
   std::vector<int32_t> f_site, f_head;  // device positions; head -1 = dead site
   std::vector<uint8_t> f_sa, head_is_po;
   int64_t live_faults = 0;
-  // device
+  // device: circuit
   DevBuf<uint32_t> d_info;
   DevBuf<int4> d_fin;
-  DevBuf<int32_t> d_parent, d_live_lvl, d_site, d_head_slot;
-  DevBuf<uint8_t> d_sa;
+  DevBuf<int32_t> d_parent, d_live_lvl, d_site, d_head, d_head_slot, d_fo_off;
+  DevBuf<uint8_t> d_sa, d_is_po;
+  DevBuf<int2> d_fo;
   DevBuf<uint64_t> d_pi_words, d_good;
-  // per step
-  DevBuf<int32_t> d_inj_site, d_flist, d_first;
-  DevBuf<uint8_t> d_inj_kind, d_last;
+  // device: per step
+  DevBuf<int32_t> d_flist, d_units, d_first;
+  DevBuf<uint8_t> d_last;
+  DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
-  DevBuf<uint64_t> d_out, d_rows, d_masks, d_detbits, d_gather;
+  DevBuf<uint64_t> d_need, d_dout, d_detbits, d_gather;
   DevBuf<int> d_queue;
   DevBuf<unsigned long long> d_bytes;
+  // device: sweep slots
+  DevBuf<uint64_t> d_rows, d_masks;
+  DevBuf<MaskRec> d_mrec;
+  DevBuf<uint32_t> d_pend;
   int64_t slots = 0;
+  uint32_t epoch_base = 0;
+  std::vector<uint8_t> head_mark;
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   ~Impl() {
@@ -424,10 +598,7 @@ struct Engine::Impl {
     if (ev[1]) (void)hipEventDestroy(ev[1]);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  double time_begin() {
-    LSIM_HIP(hipEventRecord(ev[0], stream));
-    return 0;
-  }
+  void time_begin() { LSIM_HIP(hipEventRecord(ev[0], stream)); }
   double time_end() {
     LSIM_HIP(hipEventRecord(ev[1], stream));
     LSIM_HIP(hipEventSynchronize(ev[1]));
@@ -438,6 +609,7 @@ struct Engine::Impl {
 
   void step(const RunOptions &opt, const std::vector<int32_t> &faults, int64_t w0, int Wn, bool direct,
             int32_t *first_dev, uint64_t *detbits_dev, int64_t RB, uint8_t *last_dev, RunResults &res);
+  void sweep(const RunOptions &opt, const std::vector<Lane> &lanes, std::vector<Tile> &tiles, RunResults &res);
   void ensure_slots(const RunOptions &opt, int64_t want);
 };
 
@@ -514,9 +686,10 @@ void Engine::load(const Netlist &nl, const FaultStructure &fs) {
       m.dead_lvl[l + 1] = int32_t(m.dead_lvl[l] + cd[l + 1]);
     }
   }
+  const int64_t nl1 = std::max<int64_t>(m.NL, 1);
   std::vector<uint32_t> info(N);
   std::vector<int4> fin(N);
-  std::vector<int32_t> parent(std::max<int64_t>(m.NL, 1), -1);
+  std::vector<int32_t> parent(nl1, -1);
   for (int64_t p = 0; p < N; ++p) {
     const int32_t u = m.pos2dp[p];
     const int nin = nl.fin_off[p + 1] - nl.fin_off[p];
@@ -527,11 +700,24 @@ void Engine::load(const Netlist &nl, const FaultStructure &fs) {
     fin[u] = make_int4(v[0], v[1], v[2], v[3]);
     if (fs.reach_po[p] && fs.parent[p] >= 0) parent[u] = m.pos2dp[fs.parent[p]];
   }
+  // Live fanouts of live nodes, ascending position, with the fanout's level.
+  std::vector<int32_t> fo_off(nl1 + 1, 0);
+  std::vector<int2> fo;
+  for (int64_t u = 0; u < m.NL; ++u) {
+    const int32_t p = m.dp2pos[u];
+    std::vector<int32_t> outs;
+    for (int32_t e = fs.fo_off[p]; e < fs.fo_off[p + 1]; ++e)
+      if (fs.reach_po[fs.fo[e]]) outs.push_back(m.pos2dp[fs.fo[e]]);
+    std::sort(outs.begin(), outs.end());
+    for (int32_t v : outs) fo.push_back(make_int2(v, m.level_dp[v]));
+    fo_off[u + 1] = int32_t(fo.size());
+  }
+  if (fo.empty()) fo.push_back(make_int2(0, 0));
   // Faults: site, and the head of the site's fanout-free region.
   m.f_site.resize(m.F);
   m.f_head.resize(m.F);
   m.f_sa.resize(m.F);
-  m.head_is_po.assign(std::max<int64_t>(m.NL, 1), 0);
+  m.head_is_po.assign(nl1, 0);
   for (int64_t p = 0; p < N; ++p)
     if (nl.type[p] == G_PO) m.head_is_po[m.pos2dp[p]] = 1;
   m.live_faults = 0;
@@ -547,196 +733,252 @@ void Engine::load(const Netlist &nl, const FaultStructure &fs) {
   for (int64_t r = 0; r < m.R; ++r)
     for (int64_t i = 0; i < m.P; ++i) piw[i * m.R + r] = nl.patterns[r * m.P + i];
 
+  const int64_t F1 = std::max<int64_t>(m.F, 1);
   m.d_info.alloc(N);
   m.d_fin.alloc(N);
-  m.d_parent.alloc(parent.size());
+  m.d_parent.alloc(nl1);
   m.d_live_lvl.alloc(m.live_lvl.size());
-  m.d_site.alloc(std::max<int64_t>(m.F, 1));
-  m.d_sa.alloc(std::max<int64_t>(m.F, 1));
-  m.d_head_slot.alloc(std::max<int64_t>(m.NL, 1));
+  m.d_site.alloc(F1);
+  m.d_head.alloc(F1);
+  m.d_sa.alloc(F1);
+  m.d_is_po.alloc(nl1);
+  m.d_head_slot.alloc(nl1);
+  m.d_fo_off.alloc(fo_off.size());
+  m.d_fo.alloc(fo.size());
   m.d_pi_words.alloc(piw.size());
   m.d_good.alloc(size_t(std::max<int64_t>(N * m.R, 1)));
-  LSIM_HIP(hipMemcpyAsync(m.d_info.p, info.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, m.stream));
-  LSIM_HIP(hipMemcpyAsync(m.d_fin.p, fin.data(), N * sizeof(int4), hipMemcpyHostToDevice, m.stream));
-  LSIM_HIP(hipMemcpyAsync(m.d_parent.p, parent.data(), parent.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                          m.stream));
-  LSIM_HIP(hipMemcpyAsync(m.d_live_lvl.p, m.live_lvl.data(), m.live_lvl.size() * sizeof(int32_t),
-                          hipMemcpyHostToDevice, m.stream));
-  if (m.F) {
-    LSIM_HIP(hipMemcpyAsync(m.d_site.p, m.f_site.data(), m.F * sizeof(int32_t), hipMemcpyHostToDevice, m.stream));
-    LSIM_HIP(hipMemcpyAsync(m.d_sa.p, m.f_sa.data(), m.F * sizeof(uint8_t), hipMemcpyHostToDevice, m.stream));
-  }
-  LSIM_HIP(hipMemcpyAsync(m.d_pi_words.p, piw.data(), piw.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
-                          m.stream));
-  k_fill_i32<<<blocks_for(std::max<int64_t>(m.NL, 1), 256), 256, 0, m.stream>>>(m.d_head_slot.p,
-                                                                              std::max<int64_t>(m.NL, 1), -1);
+  auto h2d = [&](void *dst, const void *src, size_t bytes) {
+    if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m.stream));
+  };
+  h2d(m.d_info.p, info.data(), N * sizeof(uint32_t));
+  h2d(m.d_fin.p, fin.data(), N * sizeof(int4));
+  h2d(m.d_parent.p, parent.data(), parent.size() * sizeof(int32_t));
+  h2d(m.d_live_lvl.p, m.live_lvl.data(), m.live_lvl.size() * sizeof(int32_t));
+  h2d(m.d_site.p, m.f_site.data(), m.F * sizeof(int32_t));
+  h2d(m.d_head.p, m.f_head.data(), m.F * sizeof(int32_t));
+  h2d(m.d_sa.p, m.f_sa.data(), m.F * sizeof(uint8_t));
+  h2d(m.d_is_po.p, m.head_is_po.data(), m.head_is_po.size());
+  h2d(m.d_fo_off.p, fo_off.data(), fo_off.size() * sizeof(int32_t));
+  h2d(m.d_fo.p, fo.data(), fo.size() * sizeof(int2));
+  h2d(m.d_pi_words.p, piw.data(), piw.size() * sizeof(uint64_t));
+  k_fill_i32<<<blocks_for(nl1, 256), 256, 0, m.stream>>>(m.d_head_slot.p, nl1, -1);
   LSIM_HIP(hipGetLastError());
   LSIM_HIP(hipStreamSynchronize(m.stream));
 }
 
 void Engine::Impl::ensure_slots(const RunOptions &opt, int64_t want) {
-  const int64_t per_slot = (NL * 64 + NL) * int64_t(sizeof(uint64_t));
+  // Per slot: faulty rows [NL][64], and the plain form's lane masks or the
+  // event form's mask records and pending flags.
+  const int64_t per_slot = NL * (64 * 8 + 8 + int64_t(sizeof(MaskRec)) + 4);
   int64_t budget;
   if (opt.scratch_mb > 0) {
     budget = opt.scratch_mb << 20;
   } else {
     size_t fr = 0, tot = 0;
     LSIM_HIP(hipMemGetInfo(&fr, &tot));
-    const int64_t held = int64_t((d_rows.n + d_masks.n) * sizeof(uint64_t));
+    const int64_t held = int64_t(d_rows.n * 8 + d_masks.n * 8 + d_mrec.n * sizeof(MaskRec) + d_pend.n * 4);
     budget = std::min<int64_t>(int64_t(fr) / 2 + held, int64_t(64) << 30);
   }
-  int64_t cap = std::max<int64_t>(1, budget / std::max<int64_t>(per_slot, 1));
-  int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 2)));
+  const int64_t cap = std::max<int64_t>(1, budget / std::max<int64_t>(per_slot, 1));
+  const int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 2)));
   if (s > slots || !d_rows.p) {
     d_rows.release();
     d_masks.release();
-    d_rows.alloc(size_t(s) * NL * 64);
-    d_masks.alloc(size_t(s) * NL);
+    d_mrec.release();
+    d_pend.release();
+    const int64_t nl = std::max<int64_t>(NL, 1);
+    d_rows.alloc(size_t(s) * nl * 64);
+    d_masks.alloc(size_t(s) * nl);
+    d_mrec.alloc(size_t(s) * nl);
+    d_pend.alloc(size_t(s) * nl);
+    // Epoch 0 is never used by a tile, so zeroed records read as "no lane differs".
+    LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, size_t(s) * nl * sizeof(MaskRec), stream));
+    LSIM_HIP(hipMemsetAsync(d_pend.p, 0, size_t(s) * nl * sizeof(uint32_t), stream));
     slots = s;
   }
 }
 
+void Engine::Impl::sweep(const RunOptions &opt, const std::vector<Lane> &lanes, std::vector<Tile> &tiles,
+                         RunResults &res) {
+  lsim_run_stats &st = res.stats;
+  const int64_t ntiles = int64_t(tiles.size());
+  if (!ntiles) return;
+  d_lanes.alloc(lanes.size());
+  d_tiles.alloc(tiles.size());
+  LSIM_HIP(hipMemcpyAsync(d_lanes.p, lanes.data(), lanes.size() * sizeof(Lane), hipMemcpyHostToDevice, stream));
+  LSIM_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+  ensure_slots(opt, ntiles);
+  SweepArgs a;
+  a.info = d_info.p;
+  a.fin = d_fin.p;
+  a.good = d_good.p;
+  a.RW = R;
+  a.live_lvl = d_live_lvl.p;
+  a.max_level = L;
+  a.tiles = d_tiles.p;
+  a.ntiles = int(ntiles);
+  a.lanes = d_lanes.p;
+  a.rows = d_rows.p;
+  a.masks = d_masks.p;
+  a.rows_stride = NL * 64;
+  a.masks_stride = NL;
+  a.out = d_dout.p;
+  d_queue.alloc(1);
+  a.queue = d_queue.p;
+  a.bytes = d_bytes.p;
+  a.mrec = d_mrec.p;
+  a.pend = d_pend.p;
+  a.rec_stride = NL;
+  a.pend_stride = NL;
+  a.fo_off = d_fo_off.p;
+  a.fo = d_fo.p;
+  a.epoch_base = 0;
+  st.sweep_tiles += ntiles;
+  time_begin();
+  if (opt.mode == LSIM_MODE_PARTITIONED) {
+    // Tile epochs are unique over the engine's lifetime; before the 32-bit
+    // counter wraps, the records are cleared and numbering restarts.
+    if (uint64_t(epoch_base) + uint64_t(ntiles) + 1 >= (uint64_t(1) << 31)) {
+      LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, d_mrec.n * sizeof(MaskRec), stream));
+      LSIM_HIP(hipMemsetAsync(d_pend.p, 0, d_pend.n * sizeof(uint32_t), stream));
+      epoch_base = 0;
+    }
+    a.epoch_base = epoch_base;
+    epoch_base += uint32_t(ntiles);
+    LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
+    const int bw = std::max(1, std::min(16, opt.block_waves));
+    const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
+    k_sweep_events<<<grid, 64 * bw, 0, stream>>>(a);
+    LSIM_HIP(hipGetLastError());
+    st.sweep_launches++;
+  } else {
+    for (int64_t t0 = 0; t0 < ntiles; t0 += slots) {
+      const int nb = int(std::min<int64_t>(slots, ntiles - t0));
+      int lmin = tiles[t0].start_level;
+      for (int64_t t = t0; t < t0 + nb; ++t) lmin = std::min(lmin, tiles[t].start_level);
+      for (int l = lmin; l <= L; ++l) {
+        const int64_t n = live_lvl[l + 1] - live_lvl[l];
+        if (n <= 0) continue;
+        const int64_t threads = int64_t(nb) * n * 64;
+        k_sweep_level<<<blocks_for(threads, 256), 256, 0, stream>>>(a, l, int(t0), nb);
+        LSIM_HIP(hipGetLastError());
+        st.sweep_launches++;
+      }
+      for (int64_t t = t0; t < t0 + nb; ++t) st.sweep_node_steps += NL - live_lvl[tiles[t].start_level];
+    }
+  }
+  st.t_sweep_ms += time_end();
+}
+
 void Engine::Impl::step(const RunOptions &opt, const std::vector<int32_t> &faults, int64_t w0, int Wn, bool direct,
                         int32_t *first_dev, uint64_t *detbits_dev, int64_t RB, uint8_t *last_dev, RunResults &res) {
-  // Injections of this step, ordered by device position so that tiles start
-  // at increasing levels and neighbouring lanes share most of their sweep.
-  std::vector<int32_t> inj_site, flist;
-  std::vector<uint8_t> inj_kind;
-  flist = faults;
-  std::sort(flist.begin(), flist.end(), [&](int32_t a, int32_t b) {
-    return f_site[a] != f_site[b] ? f_site[a] < f_site[b] : a < b;
-  });
-  if (direct) {
-    inj_site.resize(flist.size());
-    inj_kind.resize(flist.size());
-    for (size_t k = 0; k < flist.size(); ++k) {
-      inj_site[k] = f_site[flist[k]];
-      inj_kind[k] = f_sa[flist[k]] ? K_SA1 : K_SA0;
-    }
-  } else {
-    for (int32_t f : flist) {
-      const int32_t h = f_head[f];
-      if (!head_is_po[h]) inj_site.push_back(h);
-    }
-    std::sort(inj_site.begin(), inj_site.end());
-    inj_site.erase(std::unique(inj_site.begin(), inj_site.end()), inj_site.end());
-    inj_kind.assign(inj_site.size(), K_FLIP);
-  }
-  const int64_t ninj = int64_t(inj_site.size());
-  const int Wt = Wn;
-  const int K = std::max(1, 64 / Wt);
-  std::vector<Tile> tiles;
-  for (int64_t i = 0; i < ninj; i += K) {
-    Tile t;
-    t.inj_begin = int32_t(i);
-    t.K = int32_t(std::min<int64_t>(K, ninj - i));
-    t.Wt = Wt;
-    t.word0 = int32_t(w0);
-    t.start_level = level_dp[inj_site[i]];
-    t.out_begin = int32_t(i * Wn);
-    tiles.push_back(t);
-  }
-  const int64_t nf = int64_t(flist.size());
-  d_flist.alloc(std::max<int64_t>(nf, 1));
-  if (nf)
-    LSIM_HIP(hipMemcpyAsync(d_flist.p, flist.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice, stream));
   lsim_run_stats &st = res.stats;
   st.steps++;
-  if (ninj > 0) {
-    d_inj_site.alloc(ninj);
-    d_inj_kind.alloc(ninj);
-    d_tiles.alloc(tiles.size());
-    d_out.alloc(size_t(ninj) * Wn);
-    LSIM_HIP(hipMemcpyAsync(d_inj_site.p, inj_site.data(), ninj * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    LSIM_HIP(hipMemcpyAsync(d_inj_kind.p, inj_kind.data(), ninj, hipMemcpyHostToDevice, stream));
-    LSIM_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
-    LSIM_HIP(hipMemsetAsync(d_out.p, 0, size_t(ninj) * Wn * sizeof(uint64_t), stream));
-    if (!direct) {
-      k_set_slots<<<blocks_for(ninj, 256), 256, 0, stream>>>(d_head_slot.p, d_inj_site.p, int(ninj), 0);
-      LSIM_HIP(hipGetLastError());
+  const int64_t nf = int64_t(faults.size());
+  if (!nf) return;
+  // Units: stems of the faults' regions (stem tracing) or the faults
+  // themselves in site order (direct injection).
+  std::vector<int32_t> flist, units;
+  if (direct) {
+    std::vector<int32_t> cnt(size_t(NL) + 1, 0);
+    for (int32_t f : faults) cnt[f_site[f] + 1]++;
+    for (int64_t u = 0; u < NL; ++u) cnt[u + 1] += cnt[u];
+    flist.resize(faults.size());
+    for (int32_t f : faults) flist[cnt[f_site[f]]++] = f;
+    units.resize(flist.size());
+    for (size_t k = 0; k < flist.size(); ++k) units[k] = f_site[flist[k]];
+  } else {
+    flist = faults;
+    head_mark.assign(size_t(NL), 0);
+    for (int32_t f : flist) {
+      const int32_t h = f_head[f];
+      if (!head_is_po[h]) head_mark[h] = 1;
     }
-    const int64_t ntiles = int64_t(tiles.size());
-    ensure_slots(opt, ntiles);
-    SweepArgs a;
-    a.info = d_info.p;
-    a.fin = d_fin.p;
-    a.good = d_good.p;
-    a.RW = R;
-    a.live_lvl = d_live_lvl.p;
-    a.max_level = L;
-    a.tiles = d_tiles.p;
-    a.ntiles = int(ntiles);
-    a.inj_site = d_inj_site.p;
-    a.inj_kind = d_inj_kind.p;
-    a.rows = d_rows.p;
-    a.masks = d_masks.p;
-    a.rows_stride = NL * 64;
-    a.masks_stride = NL;
-    a.out = d_out.p;
-    d_queue.alloc(1);
-    a.queue = d_queue.p;
-    a.bytes = d_bytes.p;
-    for (const Tile &t : tiles) st.sweep_node_steps += NL - live_lvl[t.start_level];
-    st.sweep_tiles += ntiles;
-    st.stems += direct ? 0 : ninj;
-    time_begin();
-    if (opt.mode == LSIM_MODE_PARTITIONED) {
-      LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
-      const int bw = std::max(1, std::min(16, opt.block_waves));
-      const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
-      k_sweep_persistent<<<grid, 64 * bw, 0, stream>>>(a);
-      LSIM_HIP(hipGetLastError());
-      st.sweep_launches++;
-    } else {
-      for (int64_t t0 = 0; t0 < ntiles; t0 += slots) {
-        const int nb = int(std::min<int64_t>(slots, ntiles - t0));
-        for (int l = tiles[t0].start_level; l <= L; ++l) {
-          const int64_t n = live_lvl[l + 1] - live_lvl[l];
-          if (n <= 0) continue;
-          const int64_t threads = int64_t(nb) * n * 64;
-          k_sweep_level<<<blocks_for(threads, 256), 256, 0, stream>>>(a, l, int(t0), nb);
-          LSIM_HIP(hipGetLastError());
-          st.sweep_launches++;
-        }
-      }
-    }
-    st.t_sweep_ms += time_end();
+    for (int64_t u = 0; u < NL; ++u)
+      if (head_mark[u]) units.push_back(int32_t(u));
   }
-  if (nf > 0) {
-    DetectArgs d;
-    d.info = d_info.p;
-    d.fin = d_fin.p;
-    d.good = d_good.p;
-    d.RW = R;
-    d.parent = d_parent.p;
-    d.head_slot = d_head_slot.p;
-    d.dout = ninj > 0 ? d_out.p : nullptr;
-    d.w0 = w0;
-    d.Wn = Wn;
-    d.flist = d_flist.p;
-    d.nf = int(nf);
-    d.site = d_site.p;
-    d.sa = d_sa.p;
-    d.num_patterns = num_patterns;
-    d.first_round = first_dev;
-    d.det_bits = detbits_dev;
-    d.RB = RB;
-    d.last_flag = last_dev;
-    d.last_word = R - 1;
-    time_begin();
-    if (direct) {
-      // Injection k is fault flist[k] (same order).
-      k_direct_detect<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-    } else {
-      k_cpt_detect<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-    }
+  const int64_t nu = int64_t(units.size());
+  d_flist.alloc(nf);
+  LSIM_HIP(hipMemcpyAsync(d_flist.p, flist.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  const int64_t nuw = std::max<int64_t>(nu * Wn, 1);
+  d_need.alloc(nuw);
+  d_dout.alloc(nuw);
+  LSIM_HIP(hipMemsetAsync(d_need.p, 0, nuw * sizeof(uint64_t), stream));
+  LSIM_HIP(hipMemsetAsync(d_dout.p, 0, nuw * sizeof(uint64_t), stream));
+  if (!direct && nu) {
+    d_units.alloc(nu);
+    LSIM_HIP(hipMemcpyAsync(d_units.p, units.data(), nu * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    k_set_slots<<<blocks_for(nu, 256), 256, 0, stream>>>(d_head_slot.p, d_units.p, int(nu), 0);
     LSIM_HIP(hipGetLastError());
-    if (!direct && ninj > 0) {
-      k_set_slots<<<blocks_for(ninj, 256), 256, 0, stream>>>(d_head_slot.p, d_inj_site.p, int(ninj), 1);
-      LSIM_HIP(hipGetLastError());
-    }
-    st.t_detect_ms += time_end();
   }
+  DetectArgs d;
+  d.info = d_info.p;
+  d.fin = d_fin.p;
+  d.good = d_good.p;
+  d.RW = R;
+  d.parent = d_parent.p;
+  d.head_slot = d_head_slot.p;
+  d.is_po = d_is_po.p;
+  d.need = d_need.p;
+  d.dout = d_dout.p;
+  d.w0 = w0;
+  d.Wn = Wn;
+  d.flist = d_flist.p;
+  d.nf = int(nf);
+  d.site = d_site.p;
+  d.head = d_head.p;
+  d.sa = d_sa.p;
+  d.num_patterns = num_patterns;
+  d.first_round = first_dev;
+  d.det_bits = detbits_dev;
+  d.RB = RB;
+  d.last_flag = last_dev;
+  d.last_word = R - 1;
+  time_begin();
+  if (direct) k_local_direct<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
+  else k_local_cpt<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
+  LSIM_HIP(hipGetLastError());
+  // The needed bits decide which (unit, word) pairs become sweep lanes.
+  std::vector<uint64_t> need(size_t(nu) * Wn);
+  if (nu)
+    LSIM_HIP(hipMemcpyAsync(need.data(), d_need.p, need.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+  st.t_detect_ms += time_end();
+  std::vector<Lane> lanes;
+  for (int64_t k = 0; k < nu; ++k)
+    for (int j = 0; j < Wn; ++j) {
+      const uint64_t m = need[size_t(k) * Wn + j];
+      if (!m) continue;
+      Lane ln;
+      ln.mask = m;
+      ln.site = units[k];
+      ln.word = int32_t(w0 + j);
+      ln.out = int32_t(k * Wn + j);
+      ln.kind = direct ? (f_sa[flist[k]] ? K_SA1 : K_SA0) : K_FLIP;
+      lanes.push_back(ln);
+    }
+  std::vector<Tile> tiles;
+  for (size_t i = 0; i < lanes.size(); i += 64) {
+    Tile t;
+    t.lane0 = int32_t(i);
+    t.nlanes = int32_t(std::min<size_t>(64, lanes.size() - i));
+    t.start_level = level_dp[lanes[i].site];
+    t.end_level = level_dp[lanes[i + t.nlanes - 1].site];
+    tiles.push_back(t);
+  }
+  st.stems += direct ? 0 : nu;
+  sweep(opt, lanes, tiles, res);
+  if (std::getenv("LSIM_TRACE"))
+    std::fprintf(stderr, "[lsim] step w0=%lld Wn=%d faults=%lld units=%lld lanes=%lld tiles=%lld\n",
+                 (long long)w0, Wn, (long long)nf, (long long)nu, (long long)lanes.size(), (long long)tiles.size());
+  time_begin();
+  if (direct) k_detect_direct<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
+  else k_detect_cpt<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
+  LSIM_HIP(hipGetLastError());
+  if (!direct && nu) {
+    k_set_slots<<<blocks_for(nu, 256), 256, 0, stream>>>(d_head_slot.p, d_units.p, int(nu), 1);
+    LSIM_HIP(hipGetLastError());
+  }
+  st.t_detect_ms += time_end();
 }
 
 void Engine::run(const RunOptions &opt, RunResults &res) {
@@ -784,14 +1026,14 @@ void Engine::run(const RunOptions &opt, RunResults &res) {
   for (int64_t f = 0; f < F; ++f)
     if (m.f_head[f] >= 0) live.push_back(int32_t(f));
   const bool direct = opt.direct || m.has_mux;
-  m.d_first.alloc(std::max<int64_t>(F, 1));
-  k_fill_i32<<<blocks_for(std::max<int64_t>(F, 1), 256), 256, 0, m.stream>>>(m.d_first.p, std::max<int64_t>(F, 1),
-                                                                            INT32_MAX);
+  const int64_t F1 = std::max<int64_t>(F, 1);
+  m.d_first.alloc(F1);
+  k_fill_i32<<<blocks_for(F1, 256), 256, 0, m.stream>>>(m.d_first.p, F1, INT32_MAX);
   LSIM_HIP(hipGetLastError());
   res.first_round.assign(F, INT32_MAX);
+  const int64_t RB = (R + 63) / 64;
   if (R > 0 && !live.empty()) {
     if (!opt.drop) {
-      const int64_t RB = (R + 63) / 64;
       m.d_detbits.alloc(size_t(std::max<int64_t>(F * RB, 1)));
       LSIM_HIP(hipMemsetAsync(m.d_detbits.p, 0, size_t(std::max<int64_t>(F * RB, 1)) * sizeof(uint64_t), m.stream));
       for (int64_t w0 = 0; w0 < R;) {
@@ -811,40 +1053,30 @@ void Engine::run(const RunOptions &opt, RunResults &res) {
         res.last_round[f] = uint8_t((res.det_bits[f * RB + ((R - 1) >> 6)] >> ((R - 1) & 63)) & 1);
       res.has_last = true;
     } else {
+      // Words per step grow geometrically: the faults left after the first
+      // rounds are the hard ones and are dropped less often.
       std::vector<int32_t> alive = live;
       std::vector<int32_t> first(F);
+      int Wt = 1;
       for (int64_t w0 = 0; w0 < R && !alive.empty();) {
-        // Fewer live injections -> more words per tile, so the step still
-        // offers about two tiles per CU.
-        int64_t ninj;
-        if (direct) {
-          ninj = int64_t(alive.size());
-        } else {
-          std::vector<int32_t> heads;
-          for (int32_t f : alive)
-            if (!m.head_is_po[m.f_head[f]]) heads.push_back(m.f_head[f]);
-          std::sort(heads.begin(), heads.end());
-          ninj = int64_t(std::unique(heads.begin(), heads.end()) - heads.begin());
-        }
-        int Wt = 1;
-        while (Wt < 64 && ninj * Wt < int64_t(64) * 2 * m.cus) Wt *= 2;
         const int Wn = int(std::min<int64_t>(Wt, R - w0));
         m.step(opt, alive, w0, Wn, direct, m.d_first.p, nullptr, 0, nullptr, res);
         LSIM_HIP(hipMemcpyAsync(first.data(), m.d_first.p, F * sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
         LSIM_HIP(hipStreamSynchronize(m.stream));
         std::vector<int32_t> next;
+        next.reserve(alive.size());
         for (int32_t f : alive)
           if (first[f] == INT32_MAX) next.push_back(f);
         alive.swap(next);
         w0 += Wn;
+        Wt = std::min(64, Wt * 2);
       }
       if (opt.report_last_round) {
-        m.d_last.alloc(std::max<int64_t>(F, 1));
-        LSIM_HIP(hipMemsetAsync(m.d_last.p, 0, std::max<int64_t>(F, 1), m.stream));
+        m.d_last.alloc(F1);
+        LSIM_HIP(hipMemsetAsync(m.d_last.p, 0, F1, m.stream));
         m.step(opt, live, R - 1, 1, direct, nullptr, nullptr, 0, m.d_last.p, res);
         res.last_round.assign(F, 0);
-        if (F)
-          LSIM_HIP(hipMemcpyAsync(res.last_round.data(), m.d_last.p, F, hipMemcpyDeviceToHost, m.stream));
+        if (F) LSIM_HIP(hipMemcpyAsync(res.last_round.data(), m.d_last.p, F, hipMemcpyDeviceToHost, m.stream));
         res.has_last = true;
       }
     }
@@ -854,8 +1086,8 @@ void Engine::run(const RunOptions &opt, RunResults &res) {
   } else {
     // No rounds or no observable fault: nothing is ever detected.
     res.has_matrix = !opt.drop;
-    res.RB = (R + 63) / 64;
-    res.det_bits.assign(size_t(F * res.RB), 0);
+    res.RB = RB;
+    res.det_bits.assign(size_t(F * RB), 0);
     res.last_round.assign(F, 0);
     res.has_last = !opt.drop || opt.report_last_round;
   }
