@@ -75,7 +75,7 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            per-tile state (default 0 = automatic).
  *  "direct"             0/1  inject every fault directly instead of stem
  *                            tracing (default 0; forced on for circuits with MUX).
- *  "block_waves"        waves per workgroup of the persistent sweep (default 16). */
+ *  "block_waves"        waves per workgroup of the persistent sweep (default 8). */
 LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
 
 /* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).

@@ -72,6 +72,8 @@ struct Lane {
   int32_t word;
   int32_t out;
   uint32_t kind;
+  int32_t level;  // level of `site`
+  int32_t pad;
 };
 
 struct Tile {
@@ -196,6 +198,7 @@ struct LaneSetup {
   int site;
   int out;
   uint32_t kind;
+  int level;
   int active;
 };
 
@@ -209,6 +212,7 @@ __device__ __forceinline__ LaneSetup lane_setup(const SweepArgs &a, const Tile &
     s.site = L.site;
     s.out = L.out;
     s.kind = L.kind;
+    s.level = L.level;
   } else {
     const Lane L = a.lanes[T.lane0];  // idle lane: a valid word, never injected
     s.mask = 0;
@@ -216,6 +220,7 @@ __device__ __forceinline__ LaneSetup lane_setup(const SweepArgs &a, const Tile &
     s.site = -1;
     s.out = -1;
     s.kind = K_SA0;
+    s.level = -1;
   }
   return s;
 }
@@ -289,83 +294,118 @@ __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
   if (lane == 0) atomicAdd(a.bytes, (unsigned long long)bytes);
 }
 
-// Event form: evaluation of pending node u for the 64 lanes of one tile.
+// Event form: evaluation of up to NB pending nodes for the 64 lanes of one
+// tile, their loads interleaved so a wave keeps NB dependent chains in flight.
 // Fanin lane masks come from MaskRec entries stamped with this tile's epoch
 // (anything else means "no lane differs"); when some lane differs, the node's
-// live fanouts are marked pending and the deepest pending level is raised.
-__device__ __forceinline__ void event_node(const SweepArgs &a, int u, int lane, const LaneSetup &s, uint32_t epoch,
-                                           uint64_t *rows, MaskRec *mrec, uint32_t *pend, uint64_t &acc,
-                                           uint64_t &bytes, int *s_maxl) {
-  u = __builtin_amdgcn_readfirstlane(u);
-  const uint32_t inf = a.info[u];
-  const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-  const int4 f = a.fin[u];
-  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-  if (type != G_PI) {
-    const MaskRec r0 = mrec[f.x];
-    m0 = r0.epoch == epoch ? r0.mask : 0;
-    if (nin > 1) {
-      const MaskRec r = mrec[f.y];
-      m1 = r.epoch == epoch ? r.mask : 0;
-    }
-    if (nin > 2) {
-      const MaskRec r = mrec[f.z];
-      m2 = r.epoch == epoch ? r.mask : 0;
-    }
-    if (nin > 3) {
-      const MaskRec r = mrec[f.w];
-      m3 = r.epoch == epoch ? r.mask : 0;
-    }
-  }
+// live fanouts are marked pending, counted per level, and the deepest pending
+// level is raised.
+template <int NB>
+__device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, int cnt, int lane, const LaneSetup &s,
+                                            uint32_t epoch, uint64_t *rows, MaskRec *mrec, uint32_t *pend,
+                                            uint64_t &acc, uint64_t &bytes, int *s_maxl, int *s_lvlcnt) {
+  int u[NB];
+  uint32_t type[NB], nin[NB];
+  int4 f[NB];
+  uint64_t m[NB][4], gu[NB], r[NB];
   const int64_t RW = a.RW;
-  const uint64_t gu = a.good[int64_t(u) * RW + s.w];
-  uint64_t r = gu;
-  if (type != G_PI) {
-    const uint64_t v0 = ((m0 >> lane) & 1) ? rows[int64_t(f.x) * 64 + lane] : a.good[int64_t(f.x) * RW + s.w];
-    uint64_t v1 = 0, v2 = 0, v3 = 0;
-    if (nin > 1) v1 = ((m1 >> lane) & 1) ? rows[int64_t(f.y) * 64 + lane] : a.good[int64_t(f.y) * RW + s.w];
-    if (nin > 2) v2 = ((m2 >> lane) & 1) ? rows[int64_t(f.z) * 64 + lane] : a.good[int64_t(f.z) * RW + s.w];
-    if (nin > 3) v3 = ((m3 >> lane) & 1) ? rows[int64_t(f.w) * 64 + lane] : a.good[int64_t(f.w) * RW + s.w];
-    r = eval_gate(type, nin, v0, v1, v2, v3);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    u[k] = __builtin_amdgcn_readfirstlane(us[k < cnt ? k : 0]);
+    const uint32_t inf = a.info[u[k]];
+    type[k] = inf & 15u;
+    nin[k] = (inf >> 4) & 7u;
+    f[k] = a.fin[u[k]];
   }
-  if (s.site == u) r = inject(s, gu);
-  const uint64_t d = r ^ gu;
-  const uint64_t nm = __ballot(d != 0);
-  bytes += 20 + 16 * nin + uint64_t(s.active) * 8 * (type == G_PI ? 1 : nin + 1);
-  if (nm) {
-    if (d) rows[int64_t(u) * 64 + lane] = r;
-    if (lane == 0) {
-      MaskRec rec;
-      rec.mask = nm;
-      rec.epoch = epoch;
-      rec.pad = 0;
-      mrec[u] = rec;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    m[k][0] = m[k][1] = m[k][2] = m[k][3] = 0;
+    if (k < cnt && type[k] != G_PI) {
+      const MaskRec r0 = mrec[f[k].x];
+      m[k][0] = r0.epoch == epoch ? r0.mask : 0;
+      if (nin[k] > 1) {
+        const MaskRec q = mrec[f[k].y];
+        m[k][1] = q.epoch == epoch ? q.mask : 0;
+      }
+      if (nin[k] > 2) {
+        const MaskRec q = mrec[f[k].z];
+        m[k][2] = q.epoch == epoch ? q.mask : 0;
+      }
+      if (nin[k] > 3) {
+        const MaskRec q = mrec[f[k].w];
+        m[k][3] = q.epoch == epoch ? q.mask : 0;
+      }
     }
-    const int e0 = a.fo_off[u], e1 = a.fo_off[u + 1];
-    for (int e = e0 + lane; e < e1; e += 64) pend[a.fo[e].x] = epoch;
-    if (e1 > e0 && lane == 0) atomicMax(s_maxl, a.fo[e1 - 1].y);
-    bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
+    gu[k] = a.good[int64_t(u[k]) * RW + s.w];
   }
-  if (type == G_PO) acc |= d;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    r[k] = gu[k];
+    if (k < cnt && type[k] != G_PI) {
+      const uint32_t n = nin[k];
+      const uint64_t v0 =
+          ((m[k][0] >> lane) & 1) ? rows[int64_t(f[k].x) * 64 + lane] : a.good[int64_t(f[k].x) * RW + s.w];
+      uint64_t v1 = 0, v2 = 0, v3 = 0;
+      if (n > 1)
+        v1 = ((m[k][1] >> lane) & 1) ? rows[int64_t(f[k].y) * 64 + lane] : a.good[int64_t(f[k].y) * RW + s.w];
+      if (n > 2)
+        v2 = ((m[k][2] >> lane) & 1) ? rows[int64_t(f[k].z) * 64 + lane] : a.good[int64_t(f[k].z) * RW + s.w];
+      if (n > 3)
+        v3 = ((m[k][3] >> lane) & 1) ? rows[int64_t(f[k].w) * 64 + lane] : a.good[int64_t(f[k].w) * RW + s.w];
+      r[k] = eval_gate(type[k], n, v0, v1, v2, v3);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (k >= cnt) break;
+    if (s.site == u[k]) r[k] = inject(s, gu[k]);
+    const uint64_t d = r[k] ^ gu[k];
+    const uint64_t nm = __ballot(d != 0);
+    bytes += 20 + 16 * nin[k] + uint64_t(s.active) * 8 * (type[k] == G_PI ? 1 : nin[k] + 1);
+    if (nm) {
+      if (d) rows[int64_t(u[k]) * 64 + lane] = r[k];
+      if (lane == 0) {
+        MaskRec rec;
+        rec.mask = nm;
+        rec.epoch = epoch;
+        rec.pad = 0;
+        mrec[u[k]] = rec;
+      }
+      const int e0 = a.fo_off[u[k]], e1 = a.fo_off[u[k] + 1];
+      for (int e = e0 + lane; e < e1; e += 64) {
+        const int2 v = a.fo[e];
+        pend[v.x] = epoch;
+        atomicAdd(&s_lvlcnt[v.y], 1);
+      }
+      if (e1 > e0 && lane == 0) atomicMax(s_maxl, a.fo[e1 - 1].y);
+      bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
+    }
+    if (type[k] == G_PO) acc |= d;
+  }
 }
 
 // Event form, persistent: a workgroup pulls tiles from a queue. For each tile
-// it walks the levels from the lowest injection; at each level the nodes
-// marked pending for this tile are compacted into an LDS list and spread over
-// the waves; the walk ends after the deepest level any pending node sits on.
-// Pending flags and mask records carry the tile's epoch, so nothing is cleared
-// between tiles. Every workgroup leaves once the queue passes the last tile.
+// it walks the levels from the lowest injection, skipping levels with no
+// pending node (per-level counts in LDS); at a level with pending nodes those
+// are compacted into an LDS list and spread over the waves, two per wave at a
+// time. The walk ends after the deepest level any pending node sits on.
+// Pending flags and mask records carry the tile's epoch, so nothing in global
+// memory is cleared between tiles; each level's LDS count is zeroed once the
+// level is done. Every workgroup leaves once the queue passes the last tile.
 constexpr int kChunk = 1024;
+constexpr int kNodesPerWave = 2;
 __global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
   __shared__ unsigned long long acc_lds[64];
   __shared__ int s_tile, s_cnt, s_maxl;
   __shared__ int s_list[kChunk];
+  extern __shared__ int s_lvlcnt[];  // max_level + 1 entries
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6, nwaves = blockDim.x >> 6;
   uint64_t *rows = a.rows + int64_t(blockIdx.x) * a.rows_stride;
   MaskRec *mrec = a.mrec + int64_t(blockIdx.x) * a.rec_stride;
   uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.pend_stride;
+  for (int i = tid; i <= a.max_level; i += blockDim.x) s_lvlcnt[i] = 0;
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(a.queue, 1);
     if (tid < 64) acc_lds[tid] = 0;
@@ -375,11 +415,15 @@ __global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
     const Tile T = a.tiles[t];
     const LaneSetup s = lane_setup(a, T, lane);
     const uint32_t epoch = a.epoch_base + uint32_t(t) + 1u;
-    if (tid < 64 && s.site >= 0) pend[s.site] = epoch;
+    if (tid < 64 && s.site >= 0) {
+      pend[s.site] = epoch;
+      atomicAdd(&s_lvlcnt[s.level], 1);
+    }
     if (tid == 0) s_maxl = T.end_level;
     uint64_t acc = 0, bytes = 0;
     __syncthreads();
     for (int l = T.start_level; l <= s_maxl; ++l) {
+      if (s_lvlcnt[l] == 0) continue;
       const int lo = a.live_lvl[l], hi = a.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += blockDim.x) {
         if (tid == 0) s_cnt = 0;
@@ -396,10 +440,12 @@ __global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
         }
         __syncthreads();
         const int n = s_cnt;
-        for (int i = wave; i < n; i += nwaves)
-          event_node(a, s_list[i], lane, s, epoch, rows, mrec, pend, acc, bytes, &s_maxl);
+        for (int i = wave * kNodesPerWave; i < n; i += nwaves * kNodesPerWave)
+          event_nodes<kNodesPerWave>(a, &s_list[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
+                                     acc, bytes, &s_maxl, s_lvlcnt);
         __syncthreads();
       }
+      if (tid == 0) s_lvlcnt[l] = 0;
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, (unsigned long long)bytes);
     if (acc) atomicOr(&acc_lds[lane], (unsigned long long)acc);
@@ -777,10 +823,12 @@ void Engine::Impl::ensure_slots(const RunOptions &opt, int64_t want) {
     size_t fr = 0, tot = 0;
     LSIM_HIP(hipMemGetInfo(&fr, &tot));
     const int64_t held = int64_t(d_rows.n * 8 + d_masks.n * 8 + d_mrec.n * sizeof(MaskRec) + d_pend.n * 4);
-    budget = std::min<int64_t>(int64_t(fr) / 2 + held, int64_t(64) << 30);
+    budget = (int64_t(fr) + held) / 10 * 7;
   }
   const int64_t cap = std::max<int64_t>(1, budget / std::max<int64_t>(per_slot, 1));
-  const int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 2)));
+  // Up to three resident 8-wave workgroups per CU (the event sweep's register
+  // use allows 6 waves per SIMD).
+  const int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 3)));
   if (s > slots || !d_rows.p) {
     d_rows.release();
     d_masks.release();
@@ -848,7 +896,7 @@ void Engine::Impl::sweep(const RunOptions &opt, const std::vector<Lane> &lanes, 
     LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
     const int bw = std::max(1, std::min(16, opt.block_waves));
     const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
-    k_sweep_events<<<grid, 64 * bw, 0, stream>>>(a);
+    k_sweep_events<<<grid, 64 * bw, size_t(L + 1) * sizeof(int), stream>>>(a);
     LSIM_HIP(hipGetLastError());
     st.sweep_launches++;
   } else {
@@ -954,6 +1002,8 @@ void Engine::Impl::step(const RunOptions &opt, const std::vector<int32_t> &fault
       ln.word = int32_t(w0 + j);
       ln.out = int32_t(k * Wn + j);
       ln.kind = direct ? (f_sa[flist[k]] ? K_SA1 : K_SA0) : K_FLIP;
+      ln.level = level_dp[units[k]];
+      ln.pad = 0;
       lanes.push_back(ln);
     }
   std::vector<Tile> tiles;
@@ -966,10 +1016,12 @@ void Engine::Impl::step(const RunOptions &opt, const std::vector<int32_t> &fault
     tiles.push_back(t);
   }
   st.stems += direct ? 0 : nu;
+  const double before = st.t_sweep_ms;
   sweep(opt, lanes, tiles, res);
   if (std::getenv("LSIM_TRACE"))
-    std::fprintf(stderr, "[lsim] step w0=%lld Wn=%d faults=%lld units=%lld lanes=%lld tiles=%lld\n",
-                 (long long)w0, Wn, (long long)nf, (long long)nu, (long long)lanes.size(), (long long)tiles.size());
+    std::fprintf(stderr, "[lsim] step w0=%lld Wn=%d faults=%lld units=%lld lanes=%lld tiles=%lld sweep=%.2fms\n",
+                 (long long)w0, Wn, (long long)nf, (long long)nu, (long long)lanes.size(), (long long)tiles.size(),
+                 st.t_sweep_ms - before);
   time_begin();
   if (direct) k_detect_direct<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
   else k_detect_cpt<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);

@@ -21,7 +21,7 @@ struct RunOptions {
   bool report_last_round = true;
   int64_t scratch_mb = 0;
   bool direct = false;
-  int block_waves = 16;
+  int block_waves = 8;
 };
 
 struct RunResults {
