@@ -28,6 +28,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(config: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `config` from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: float) -> dict:
     """Time the reference simulator (oracle/_ref/fsim_ref, SEQUENTIAL, 1 core)
     on a bounded sample: a seeded subset of faults over the first rounds."""
@@ -77,7 +90,20 @@ def main() -> int:
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.int64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fault set would shard "
               "across GPUs by fault, see DESIGN.md)", file=sys.stderr)
@@ -125,6 +151,7 @@ def main() -> int:
     bytes_launch = agg["sweep_bytes"] / launches
     achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
     detected = int(sim.detected().sum())
+    kernel = "k_sweep_events" if args.mode == "partitioned" else "k_sweep_level"
     out = {
         "metric": "fault x pattern evals/sec",
         "value": value,
@@ -149,12 +176,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_sweep_persistent" if args.mode == "partitioned" else "k_sweep_level",
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(args.config, kernel),
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": t_launch_s * 1000.0,
         },
