@@ -294,16 +294,38 @@ __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
   if (lane == 0) atomicAdd(a.bytes, (unsigned long long)bytes);
 }
 
-// Event form: evaluation of up to NB pending nodes for the 64 lanes of one
-// tile, their loads interleaved so a wave keeps NB dependent chains in flight.
-// Fanin lane masks come from MaskRec entries stamped with this tile's epoch
-// (anything else means "no lane differs"); when some lane differs, the node's
-// live fanouts are marked pending, counted per level, and the deepest pending
-// level is raised.
+// Event form. A level's pending nodes are staged in LDS by the scan that finds
+// them: node record, live-fanout range and the lane masks of its fanins (the
+// fanins sit on lower levels, so their masks are final). A MaskRec stamped
+// with another epoch means "no lane differs".
+struct NodeStage {
+  uint64_t m[4];    // fanin lane masks
+  int4 fin;
+  int u;
+  uint32_t info;
+  int fo0, fo1;     // live fanouts fo[fo0, fo1)
+};
+
+__device__ __forceinline__ uint64_t stamped_mask(const MaskRec *mrec, int f, uint32_t epoch) {
+  const MaskRec r = mrec[f];
+  return r.epoch == epoch ? r.mask : 0;
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Evaluation of up to NB staged nodes for the 64 lanes of one tile, with
+// their fanin loads interleaved. When some lane differs, the node's row and
+// mask record are written and its live fanouts marked pending (counted per
+// level in LDS); the deepest pending level is raised.
 template <int NB>
-__device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, int cnt, int lane, const LaneSetup &s,
-                                            uint32_t epoch, uint64_t *rows, MaskRec *mrec, uint32_t *pend,
-                                            uint64_t &acc, uint64_t &bytes, int *s_maxl, int *s_lvlcnt) {
+__device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage *st, int cnt, int lane,
+                                            const LaneSetup &s, uint32_t epoch, uint64_t *rows, MaskRec *mrec,
+                                            uint32_t *pend, uint64_t &acc, uint64_t &bytes, int *s_maxl,
+                                            int *s_lvlcnt) {
   int u[NB];
   uint32_t type[NB], nin[NB];
   int4 f[NB];
@@ -311,31 +333,17 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, i
   const int64_t RW = a.RW;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
-    u[k] = __builtin_amdgcn_readfirstlane(us[k < cnt ? k : 0]);
-    const uint32_t inf = a.info[u[k]];
+    const NodeStage &n = st[k < cnt ? k : 0];
+    u[k] = __builtin_amdgcn_readfirstlane(n.u);
+    const uint32_t inf = __builtin_amdgcn_readfirstlane(n.info);
     type[k] = inf & 15u;
     nin[k] = (inf >> 4) & 7u;
-    f[k] = a.fin[u[k]];
-  }
+    f[k].x = __builtin_amdgcn_readfirstlane(n.fin.x);
+    f[k].y = __builtin_amdgcn_readfirstlane(n.fin.y);
+    f[k].z = __builtin_amdgcn_readfirstlane(n.fin.z);
+    f[k].w = __builtin_amdgcn_readfirstlane(n.fin.w);
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    m[k][0] = m[k][1] = m[k][2] = m[k][3] = 0;
-    if (k < cnt && type[k] != G_PI) {
-      const MaskRec r0 = mrec[f[k].x];
-      m[k][0] = r0.epoch == epoch ? r0.mask : 0;
-      if (nin[k] > 1) {
-        const MaskRec q = mrec[f[k].y];
-        m[k][1] = q.epoch == epoch ? q.mask : 0;
-      }
-      if (nin[k] > 2) {
-        const MaskRec q = mrec[f[k].z];
-        m[k][2] = q.epoch == epoch ? q.mask : 0;
-      }
-      if (nin[k] > 3) {
-        const MaskRec q = mrec[f[k].w];
-        m[k][3] = q.epoch == epoch ? q.mask : 0;
-      }
-    }
+    for (int q = 0; q < 4; ++q) m[k][q] = uniform64(n.m[q]);
     gu[k] = a.good[int64_t(u[k]) * RW + s.w];
   }
 #pragma unroll
@@ -361,7 +369,7 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, i
     if (s.site == u[k]) r[k] = inject(s, gu[k]);
     const uint64_t d = r[k] ^ gu[k];
     const uint64_t nm = __ballot(d != 0);
-    bytes += 20 + 16 * nin[k] + uint64_t(s.active) * 8 * (type[k] == G_PI ? 1 : nin[k] + 1);
+    bytes += uint64_t(s.active) * 8 * (type[k] == G_PI ? 1 : nin[k] + 1);
     if (nm) {
       if (d) rows[int64_t(u[k]) * 64 + lane] = r[k];
       if (lane == 0) {
@@ -371,7 +379,7 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, i
         rec.pad = 0;
         mrec[u[k]] = rec;
       }
-      const int e0 = a.fo_off[u[k]], e1 = a.fo_off[u[k] + 1];
+      const int e0 = __builtin_amdgcn_readfirstlane(st[k].fo0), e1 = __builtin_amdgcn_readfirstlane(st[k].fo1);
       for (int e = e0 + lane; e < e1; e += 64) {
         const int2 v = a.fo[e];
         pend[v.x] = epoch;
@@ -386,18 +394,20 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const int *us, i
 
 // Event form, persistent: a workgroup pulls tiles from a queue. For each tile
 // it walks the levels from the lowest injection, skipping levels with no
-// pending node (per-level counts in LDS); at a level with pending nodes those
-// are compacted into an LDS list and spread over the waves, two per wave at a
-// time. The walk ends after the deepest level any pending node sits on.
-// Pending flags and mask records carry the tile's epoch, so nothing in global
-// memory is cleared between tiles; each level's LDS count is zeroed once the
-// level is done. Every workgroup leaves once the queue passes the last tile.
-constexpr int kChunk = 1024;
+// pending node (per-level counts in LDS). At a level with pending nodes, one
+// thread per node checks the pending flag; those that find one append the node
+// to the LDS stage together with its record and fanin masks; then the waves
+// evaluate the staged nodes, two per wave at a time. The walk ends after the
+// deepest level any pending node sits on. Pending flags and mask records
+// carry the tile's epoch, so nothing in global memory is cleared between
+// tiles; each level's LDS count is zeroed once the level is done. Every
+// workgroup leaves once the queue passes the last tile.
+constexpr int kChunk = 512;
 constexpr int kNodesPerWave = 2;
-__global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
+__global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
   __shared__ unsigned long long acc_lds[64];
   __shared__ int s_tile, s_cnt, s_maxl;
-  __shared__ int s_list[kChunk];
+  __shared__ NodeStage s_stage[kChunk];
   extern __shared__ int s_lvlcnt[];  // max_level + 1 entries
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -420,7 +430,7 @@ __global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
       atomicAdd(&s_lvlcnt[s.level], 1);
     }
     if (tid == 0) s_maxl = T.end_level;
-    uint64_t acc = 0, bytes = 0;
+    uint64_t acc = 0, bytes = 0, stage_bytes = 0;  // bytes: wave-uniform; stage_bytes: per lane
     __syncthreads();
     for (int l = T.start_level; l <= s_maxl; ++l) {
       if (s_lvlcnt[l] == 0) continue;
@@ -436,17 +446,35 @@ __global__ void __launch_bounds__(1024) k_sweep_events(SweepArgs a) {
           int base = 0;
           if (lane == 0) base = atomicAdd(&s_cnt, __popcll(b));
           base = __shfl(base, 0);
-          if (p) s_list[base + __popcll(b & ((1ull << lane) - 1ull))] = u;
+          if (p) {
+            NodeStage &n = s_stage[base + __popcll(b & ((1ull << lane) - 1ull))];
+            const uint32_t inf = a.info[u];
+            const int4 f = a.fin[u];
+            const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
+            n.u = u;
+            n.info = inf;
+            n.fin = f;
+            n.fo0 = a.fo_off[u];
+            n.fo1 = a.fo_off[u + 1];
+            const bool gate = type != G_PI;
+            n.m[0] = gate ? stamped_mask(mrec, f.x, epoch) : 0;
+            n.m[1] = gate && nin > 1 ? stamped_mask(mrec, f.y, epoch) : 0;
+            n.m[2] = gate && nin > 2 ? stamped_mask(mrec, f.z, epoch) : 0;
+            n.m[3] = gate && nin > 3 ? stamped_mask(mrec, f.w, epoch) : 0;
+            stage_bytes += 28 + (gate ? 16 * nin : 0);
+          }
         }
         __syncthreads();
         const int n = s_cnt;
         for (int i = wave * kNodesPerWave; i < n; i += nwaves * kNodesPerWave)
-          event_nodes<kNodesPerWave>(a, &s_list[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
+          event_nodes<kNodesPerWave>(a, &s_stage[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
                                      acc, bytes, &s_maxl, s_lvlcnt);
         __syncthreads();
       }
       if (tid == 0) s_lvlcnt[l] = 0;
     }
+    for (int o = 32; o > 0; o >>= 1) stage_bytes += __shfl_xor(stage_bytes, o);
+    bytes += stage_bytes;
     if (lane == 0 && bytes) atomicAdd(a.bytes, (unsigned long long)bytes);
     if (acc) atomicOr(&acc_lds[lane], (unsigned long long)acc);
     __syncthreads();
@@ -894,7 +922,7 @@ void Engine::Impl::sweep(const RunOptions &opt, const std::vector<Lane> &lanes, 
     a.epoch_base = epoch_base;
     epoch_base += uint32_t(ntiles);
     LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
-    const int bw = std::max(1, std::min(16, opt.block_waves));
+    const int bw = std::max(1, std::min(kChunk / 64, opt.block_waves));
     const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
     k_sweep_events<<<grid, 64 * bw, size_t(L + 1) * sizeof(int), stream>>>(a);
     LSIM_HIP(hipGetLastError());
