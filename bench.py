@@ -81,12 +81,193 @@ def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: fl
             "sample": sample, "seconds": round(secs, 3)}
 
 
+def _roofline(kernel: str, agg: dict, config: str) -> dict:
+    launches = max(1, agg["sweep_launches"])
+    t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
+    bytes_launch = agg["sweep_bytes"] / launches
+    achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(config, kernel),
+            "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
+
+
+def bench_batch(args) -> int:
+    """256 circuits of uneven size, each with its full fault list and its own
+    pattern block, run together through the batch API (one persistent sweep
+    launch per word step over all jobs). A step adds the 256 jobs (parse and
+    pattern upload, untimed) and times run() to the last result on the host."""
+    import lsim_amd
+    from lsim_amd import workloads
+
+    specs = workloads.c4_specs()
+    t0 = time.perf_counter()
+    batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0)
+    cids, jobs_text = [], []
+    for i, sp in enumerate(specs):
+        cids.append(batch.add_circuit(workloads.circuit_text(sp)))
+        jobs_text.append(workloads.batch_job_texts(sp))
+        if i % 32 == 31:
+            log(f"[bench] c4: {i + 1} circuits resident ({time.perf_counter() - t0:.1f}s)")
+    F_tot = sum(2 * sp["num_gates"] for sp in specs)
+    evals = sum(2 * sp["num_gates"] * sp["num_patterns"] for sp in specs)
+    times, agg, detected = [], None, 0
+    for it in range(args.warmup + args.steps):
+        ids = [batch.add_job(c, f, p, drop=True) for c, (f, p) in zip(cids, jobs_text)]
+        t0 = time.perf_counter()
+        batch.run()
+        dt = time.perf_counter() - t0
+        st = batch.last_run_stats()
+        timed = it >= args.warmup
+        if timed:
+            times.append(dt)
+            if agg is None:
+                agg = {k: 0 for k in st}
+            for k, v in st.items():
+                agg[k] += v
+            detected = sum(int((batch.first_detect_round(j) >= 0).sum()) for j in ids)
+        log(f"[bench] c4 {'step' if timed else 'warmup'} {dt:.3f}s sweep {st['t_sweep_ms']:.1f}ms "
+            f"detect {st['t_detect_ms']:.1f}ms good {st['t_good_ms']:.1f}ms steps {st['steps']} "
+            f"launches {st['sweep_launches']}")
+        for j in ids:
+            batch.release_job(j)
+    elapsed = sum(times)
+    kernel = "k_sweep_events" if args.mode == "partitioned" else "k_sweep_level"
+    out = {
+        "metric": "fault x pattern evals/sec",
+        "value": evals / (elapsed / args.steps),
+        "unit": "fault*pattern evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {
+            "workload": f"c4: {len(specs)} seeded circuits, {sum(sp['num_gates'] for sp in specs)} gates in all "
+                        f"(1k-200k each), {F_tot} SA0/SA1 faults, 2,048-16,384 patterns per circuit, "
+                        "fault dropping on, one batch",
+            "engine": args.mode, "faults": F_tot, "evals_per_step": evals, "detected": detected,
+        },
+        "roofline": _roofline(kernel, agg, "c4"),
+        "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _pct(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else 0.0
+
+
+def bench_serving(args) -> int:
+    """Serving over 64 resident circuits. Phase 1: every job alone (unloaded
+    latency). Phase 2: all jobs at once (saturated job rate). Phase 3: the same
+    jobs as a Poisson stream at `--c5-load` x the saturated rate; reports
+    p50/p99 latency (submit to result), jobs/s and evals/s, and checks that
+    every job's first-detection rounds equal its unloaded run's."""
+    import random
+
+    import numpy as np
+
+    import lsim_amd
+    from lsim_amd import workloads
+
+    circuits = workloads.c5_circuit_specs()
+    t0 = time.perf_counter()
+    srv = lsim_amd.Server(device=0, job_budget_mb=64 * 1024, scratch_mb=16 * 1024)
+    cids = []
+    for i, sp in enumerate(circuits):
+        cids.append(srv.add_circuit(workloads.circuit_text(sp)))
+        if i % 8 == 7:
+            log(f"[bench] c5: {i + 1} circuits resident ({time.perf_counter() - t0:.1f}s)")
+    jobs = workloads.c5_jobs(circuits, args.c5_jobs)
+    texts = [workloads.job_texts(circuits[j["circuit"]], j) for j in jobs]
+    F = [int(f.split(b"\n", 1)[0]) for f, _ in texts]
+    evals = sum(f * j["num_patterns"] for f, j in zip(F, jobs))
+
+    def submit(k):
+        f, p = texts[k]
+        return srv.submit(cids[jobs[k]["circuit"]], f, p, drop=True)
+
+    # Phase 1: unloaded.
+    alone, first = [], []
+    for k in range(len(jobs)):
+        jid = submit(k)
+        srv.wait(jid, 600)
+        ts, ta, tf = srv.job_times(jid)
+        alone.append(tf - ts)
+        first.append(srv.first_detect_round(jid))
+        srv.release_job(jid)
+    log(f"[bench] c5 unloaded: p50 {_pct(alone, 50) * 1e3:.1f}ms p99 {_pct(alone, 99) * 1e3:.1f}ms "
+        f"sum {sum(alone):.2f}s")
+    # Phase 2: saturated.
+    t0 = time.perf_counter()
+    ids = [submit(k) for k in range(len(jobs))]
+    for jid in ids:
+        srv.wait(jid, 600)
+    t_sat = time.perf_counter() - t0
+    exact = all(np.array_equal(srv.first_detect_round(jid), first[k]) for k, jid in enumerate(ids))
+    for jid in ids:
+        srv.release_job(jid)
+    rate_sat = len(jobs) / t_sat
+    log(f"[bench] c5 saturated: {len(jobs)} jobs in {t_sat:.2f}s ({rate_sat:.2f} jobs/s)")
+    # Phase 3: Poisson arrivals at a fixed rate.
+    lam = args.c5_load * rate_sat
+    rng = random.Random(2005)
+    ids = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for k in range(len(jobs)):
+        t_next += rng.expovariate(lam)
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        ids.append(submit(k))
+    for jid in ids:
+        srv.wait(jid, 600)
+    t_run = time.perf_counter() - t_start
+    lat, slow = [], []
+    for k, jid in enumerate(ids):
+        ts, ta, tf = srv.job_times(jid)
+        lat.append(tf - ts)
+        slow.append((tf - ts) / max(alone[k], 1e-9))
+        exact = exact and np.array_equal(srv.first_detect_round(jid), first[k])
+    cnt = srv.counters()
+    srv.close()
+    p99_alone = _pct(alone, 99)
+    out = {
+        "metric": "p99 latency at fixed QPS",
+        "value": _pct(lat, 99) * 1e3,
+        "unit": "ms",
+        "n_gpus": 1, "steps": len(jobs), "warmup": 0,
+        "ms_per_step": 1e3 * t_run / len(jobs),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {
+            "workload": f"c5: {len(circuits)} resident seeded circuits (10k-2M gates, "
+                        f"{sum(sp['num_gates'] for sp in circuits)} in all), {len(jobs)} jobs (1-100% of faults, "
+                        "1,024-32,768 patterns), fault dropping on, Poisson arrivals",
+            "qps": lam, "load": args.c5_load,
+        },
+        "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3,
+                       "unloaded_p50": _pct(alone, 50) * 1e3, "unloaded_p99": p99_alone * 1e3,
+                       "p99_over_unloaded_p99": _pct(lat, 99) / max(p99_alone, 1e-9),
+                       "slowdown_p50": _pct(slow, 50), "slowdown_p99": _pct(slow, 99)},
+        "throughput": {"jobs_per_s": len(jobs) / t_run, "evals_per_s": evals / t_run,
+                       "saturated_jobs_per_s": rate_sat, "saturated_evals_per_s": evals / t_sat},
+        "bit_exact_alone_vs_mix": bool(exact),
+        "server": cnt,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-jobs", type=int, default=96, help="jobs per phase of the serving run")
+    ap.add_argument("--c5-load", type=float, default=0.5, help="arrival rate as a fraction of the saturated rate")
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -109,6 +290,10 @@ def main() -> int:
               "across GPUs by fault, see DESIGN.md)", file=sys.stderr)
         return 2
 
+    if args.config == "c4":
+        return bench_batch(args)
+    if args.config == "c5":
+        return bench_serving(args)
     import lsim_amd
     from lsim_amd import workloads
 
@@ -146,10 +331,6 @@ def main() -> int:
     n = sim.num_patterns()
     ms = 1000.0 * elapsed / args.steps
     value = F * n / (elapsed / args.steps)
-    launches = max(1, agg["sweep_launches"])
-    t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
-    bytes_launch = agg["sweep_bytes"] / launches
-    achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
     detected = int(sim.detected().sum())
     kernel = "k_sweep_events" if args.mode == "partitioned" else "k_sweep_level"
     out = {
@@ -174,17 +355,7 @@ def main() -> int:
             "patterns": n,
             "detected": detected,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kernel,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(args.config, kernel),
-            "bytes_per_launch": bytes_launch,
-            "avg_launch_ms": t_launch_s * 1000.0,
-        },
+        "roofline": _roofline(kernel, agg, args.config),
         "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
     }
     if not args.no_cpu_baseline:

@@ -142,6 +142,58 @@ typedef struct {
 } lsim_run_stats;
 LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
 
+/* ------------------------------------------------------------------------
+ * Batching (no reference counterpart): many circuits of uneven size, each
+ * with its own fault list and pattern block, simulated together; every
+ * iteration sweeps the tiles of all unfinished jobs in one persistent launch.
+ * Calls on one batch must not overlap. */
+typedef struct lsim_batch lsim_batch;
+LSIM_API int lsim_batch_create(int device, lsim_batch **out);
+LSIM_API void lsim_batch_destroy(lsim_batch *b);
+LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
+/* Options as lsim_set_option: "scratch_mb", "block_waves", "mode" (0 plain, 1 persistent). */
+LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
+/* Makes a circuit (.ckt text) resident; returns its id. */
+LSIM_API int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id);
+/* Adds a job (.flst and .ptn text against a resident circuit); returns its id. */
+LSIM_API int lsim_batch_add_job(lsim_batch *b, int64_t circuit_id, const char *flst, size_t flst_len,
+                                const char *ptn, size_t ptn_len, int drop, int64_t *job_id);
+/* Runs every job added since the last run, together, to completion. */
+LSIM_API int lsim_batch_run(lsim_batch *b);
+LSIM_API int64_t lsim_batch_job_faults(const lsim_batch *b, int64_t job_id);
+LSIM_API int64_t lsim_batch_job_rounds(const lsim_batch *b, int64_t job_id);
+LSIM_API int lsim_batch_first_detect_round(const lsim_batch *b, int64_t job_id, int64_t *out, size_t n);
+LSIM_API int lsim_batch_detect_matrix(const lsim_batch *b, int64_t job_id, uint8_t *out, size_t n);
+LSIM_API int lsim_batch_found_fault_to_pattern(const lsim_batch *b, int64_t job_id, uint64_t *out, size_t n);
+LSIM_API int lsim_batch_last_run_stats(const lsim_batch *b, lsim_run_stats *out);
+LSIM_API int lsim_batch_release_job(lsim_batch *b, int64_t job_id);
+
+/* ------------------------------------------------------------------------
+ * Serving (no reference counterpart): circuits stay resident; jobs are
+ * submitted from any thread and admitted in arrival order while their HBM
+ * footprint fits the budget; a worker thread advances all admitted jobs one
+ * word step per iteration (continuous batching), so a job that arrives while
+ * others run joins at the next iteration. */
+typedef struct lsim_server lsim_server;
+/* job_budget_mb: HBM for admitted jobs (good values, fault state, results);
+ * scratch_mb: HBM for sweep tiles (0 = 8 GiB). */
+LSIM_API int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, lsim_server **out);
+LSIM_API void lsim_server_destroy(lsim_server *s);
+LSIM_API const char *lsim_server_last_error(const lsim_server *s);
+LSIM_API int lsim_server_add_circuit(lsim_server *s, const char *ckt, size_t ckt_len, int64_t *circuit_id);
+LSIM_API int lsim_server_submit(lsim_server *s, int64_t circuit_id, const char *flst, size_t flst_len,
+                                const char *ptn, size_t ptn_len, int drop, int64_t *job_id);
+/* Blocks until the job finished (0), failed (its error code) or timeout_s
+ * passed (LSIM_E_STATE); timeout_s < 0 waits forever. */
+LSIM_API int lsim_server_wait(lsim_server *s, int64_t job_id, double timeout_s);
+LSIM_API int lsim_server_first_detect_round(lsim_server *s, int64_t job_id, int64_t *out, size_t n);
+LSIM_API int64_t lsim_server_job_faults(lsim_server *s, int64_t job_id);
+/* times[0..2] = submit, admit, finish in seconds since the server started. */
+LSIM_API int lsim_server_job_times(lsim_server *s, int64_t job_id, double *times);
+LSIM_API int lsim_server_release_job(lsim_server *s, int64_t job_id);
+/* counters[0..3] = iterations, jobs finished, jobs failed, peak admitted jobs. */
+LSIM_API int lsim_server_counters(lsim_server *s, int64_t *counters);
+
 /* Seeded synthetic workload in the reference formats (no reference
  * counterpart; the reference ships fsim/circuit_generator.py and
  * fsim/generate_fault_patterns.py as separate producers). Buffers are
@@ -160,6 +212,13 @@ typedef struct {
 LSIM_API int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char **flst, size_t *flst_len, char **ptn,
                   size_t *ptn_len);
 LSIM_API void lsim_free_text(char *p);
+/* The fault list or the patterns alone (same seeds and rules as lsim_generate):
+ * SA0/SA1 over inner gate ids [P+O, P+O+G) thinned to `fraction`, or
+ * ceil(n/64) rounds of P random words. */
+LSIM_API int lsim_generate_faults(uint64_t seed, int64_t num_pis, int64_t num_pos, int64_t num_gates,
+                                  double fraction, char **flst, size_t *flst_len);
+LSIM_API int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn,
+                                    size_t *ptn_len);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,37 @@ def lib() -> ctypes.CDLL:
             ],
         ),
         "lsim_free_text": (None, [ctypes.c_void_p]),
+        "lsim_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "lsim_batch_destroy": (None, [P]),
+        "lsim_batch_last_error": (ctypes.c_char_p, [P]),
+        "lsim_batch_set_option": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int64]),
+        "lsim_batch_add_circuit": (ctypes.c_int, [P, ctypes.c_char_p, sz, i64p]),
+        "lsim_batch_add_job": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_char_p, sz, ctypes.c_char_p, sz,
+                                              ctypes.c_int, i64p]),
+        "lsim_batch_run": (ctypes.c_int, [P]),
+        "lsim_batch_job_faults": (ctypes.c_int64, [P, ctypes.c_int64]),
+        "lsim_batch_job_rounds": (ctypes.c_int64, [P, ctypes.c_int64]),
+        "lsim_batch_first_detect_round": (ctypes.c_int, [P, ctypes.c_int64, i64p, sz]),
+        "lsim_batch_detect_matrix": (ctypes.c_int, [P, ctypes.c_int64, u8p, sz]),
+        "lsim_batch_found_fault_to_pattern": (ctypes.c_int, [P, ctypes.c_int64, u64p, sz]),
+        "lsim_batch_last_run_stats": (ctypes.c_int, [P, ctypes.POINTER(RunStats)]),
+        "lsim_batch_release_job": (ctypes.c_int, [P, ctypes.c_int64]),
+        "lsim_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(P)]),
+        "lsim_server_destroy": (None, [P]),
+        "lsim_server_last_error": (ctypes.c_char_p, [P]),
+        "lsim_server_add_circuit": (ctypes.c_int, [P, ctypes.c_char_p, sz, i64p]),
+        "lsim_server_submit": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_char_p, sz, ctypes.c_char_p, sz,
+                                              ctypes.c_int, i64p]),
+        "lsim_server_wait": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_double]),
+        "lsim_server_first_detect_round": (ctypes.c_int, [P, ctypes.c_int64, i64p, sz]),
+        "lsim_server_job_faults": (ctypes.c_int64, [P, ctypes.c_int64]),
+        "lsim_server_job_times": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
+        "lsim_server_release_job": (ctypes.c_int, [P, ctypes.c_int64]),
+        "lsim_server_counters": (ctypes.c_int, [P, i64p]),
+        "lsim_generate_faults": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
+        "lsim_generate_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
+                                                  ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -332,3 +363,163 @@ def generate(
         out.append(ctypes.string_at(b.value, n.value))
         L.lsim_free_text(b)
     return out[0], out[1], out[2]
+
+
+def _take_text(rc: int, buf: ctypes.c_void_p, n: ctypes.c_size_t) -> bytes:
+    L = lib()
+    if rc != LSIM_OK:
+        raise LsimError(rc, L.lsim_last_error(None).decode() or "generator rejected the spec")
+    out = ctypes.string_at(buf.value, n.value)
+    L.lsim_free_text(buf)
+    return out
+
+
+def generate_faults(seed: int, num_pis: int, num_pos: int, num_gates: int, fraction: float = 1.0) -> bytes:
+    """Seeded .flst text: SA0/SA1 over the inner gates, thinned to `fraction`."""
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    rc = lib().lsim_generate_faults(seed, num_pis, num_pos, num_gates, fraction, ctypes.byref(b), ctypes.byref(n))
+    return _take_text(rc, b, n)
+
+
+def generate_patterns(seed: int, num_pis: int, num_patterns: int) -> bytes:
+    """Seeded .ptn text with `num_patterns` random patterns."""
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    rc = lib().lsim_generate_patterns(seed, num_pis, num_patterns, ctypes.byref(b), ctypes.byref(n))
+    return _take_text(rc, b, n)
+
+
+class Batch:
+    """Many circuits with their own fault lists and pattern blocks, run
+    together (one persistent sweep launch per word step over all jobs)."""
+
+    def __init__(self, device: int = 0, **options):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.lsim_batch_create(device, ctypes.byref(h))
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_batch_last_error(None).decode())
+        self._h = h
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def _check(self, rc: int) -> None:
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_batch_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.lsim_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, key: str, value: int) -> None:
+        self._check(self._L.lsim_batch_set_option(self._h, key.encode(), int(value)))
+
+    def add_circuit(self, ckt) -> int:
+        c = _as_bytes(ckt)
+        cid = ctypes.c_int64()
+        self._check(self._L.lsim_batch_add_circuit(self._h, c, len(c), ctypes.byref(cid)))
+        return cid.value
+
+    def add_job(self, circuit: int, flst, ptn, drop: bool = False) -> int:
+        f, p = _as_bytes(flst), _as_bytes(ptn)
+        jid = ctypes.c_int64()
+        self._check(self._L.lsim_batch_add_job(self._h, circuit, f, len(f), p, len(p), int(drop), ctypes.byref(jid)))
+        return jid.value
+
+    def run(self) -> None:
+        self._check(self._L.lsim_batch_run(self._h))
+
+    def first_detect_round(self, job: int) -> np.ndarray:
+        n = self._L.lsim_batch_job_faults(self._h, job)
+        out = np.zeros(n, dtype=np.int64)
+        self._check(self._L.lsim_batch_first_detect_round(self._h, job, _ptr(out, ctypes.c_int64), n))
+        return out
+
+    def detect_matrix(self, job: int) -> np.ndarray:
+        F = self._L.lsim_batch_job_faults(self._h, job)
+        R = self._L.lsim_batch_job_rounds(self._h, job)
+        out = np.zeros(F * R, dtype=np.uint8)
+        self._check(self._L.lsim_batch_detect_matrix(self._h, job, _ptr(out, ctypes.c_uint8), out.size))
+        return out.reshape(F, R)
+
+    def found_fault_to_pattern(self, job: int) -> np.ndarray:
+        F = self._L.lsim_batch_job_faults(self._h, job)
+        out = np.zeros(2 * F, dtype=np.uint64)
+        self._check(self._L.lsim_batch_found_fault_to_pattern(self._h, job, _ptr(out, ctypes.c_uint64), out.size))
+        return out
+
+    def last_run_stats(self) -> dict:
+        st = RunStats()
+        self._check(self._L.lsim_batch_last_run_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in RunStats._fields_}
+
+    def release_job(self, job: int) -> None:
+        self._check(self._L.lsim_batch_release_job(self._h, job))
+
+
+class Server:
+    """Resident circuits and a worker thread that admits submitted jobs by HBM
+    footprint and advances all admitted jobs together (continuous batching)."""
+
+    def __init__(self, device: int = 0, job_budget_mb: int = 64 * 1024, scratch_mb: int = 0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.lsim_server_create(device, job_budget_mb, scratch_mb, ctypes.byref(h))
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_server_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != LSIM_OK:
+            raise LsimError(rc, self._L.lsim_server_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.lsim_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_circuit(self, ckt) -> int:
+        c = _as_bytes(ckt)
+        cid = ctypes.c_int64()
+        self._check(self._L.lsim_server_add_circuit(self._h, c, len(c), ctypes.byref(cid)))
+        return cid.value
+
+    def submit(self, circuit: int, flst, ptn, drop: bool = True) -> int:
+        f, p = _as_bytes(flst), _as_bytes(ptn)
+        jid = ctypes.c_int64()
+        self._check(self._L.lsim_server_submit(self._h, circuit, f, len(f), p, len(p), int(drop), ctypes.byref(jid)))
+        return jid.value
+
+    def wait(self, job: int, timeout_s: float = -1.0) -> None:
+        self._check(self._L.lsim_server_wait(self._h, job, float(timeout_s)))
+
+    def first_detect_round(self, job: int) -> np.ndarray:
+        n = self._L.lsim_server_job_faults(self._h, job)
+        out = np.zeros(n, dtype=np.int64)
+        self._check(self._L.lsim_server_first_detect_round(self._h, job, _ptr(out, ctypes.c_int64), n))
+        return out
+
+    def job_times(self, job: int):
+        t = (ctypes.c_double * 3)()
+        self._check(self._L.lsim_server_job_times(self._h, job, t))
+        return t[0], t[1], t[2]
+
+    def release_job(self, job: int) -> None:
+        self._check(self._L.lsim_server_release_job(self._h, job))
+
+    def counters(self) -> dict:
+        c = (ctypes.c_int64 * 4)()
+        self._check(self._L.lsim_server_counters(self._h, c))
+        return {"iterations": c[0], "finished": c[1], "failed": c[2], "peak_admitted": c[3]}

@@ -1,56 +1,50 @@
 // C ABI of the simulator (include/lsim.h). Every entry point catches C++
 // exceptions and turns them into LSIM_E_* codes plus a message.
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <fstream>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <string>
+#include <thread>
 
 #include "engine.hpp"
 #include "generator.hpp"
 #include "lsim.h"
 #include "netlist.hpp"
 
-struct lsim_sim {
-  int device = -1;  // -1: host-only handle (read and accessors, no run)
-  std::unique_ptr<lsim::Engine> engine;
-  lsim::Netlist nl;
-  lsim::FaultStructure fs;
-  bool loaded = false;
-  bool ran = false;
-  lsim::RunOptions opt;
-  lsim::RunResults res;
-  std::string err;
-};
-
 namespace {
 
 thread_local std::string g_create_error;
 
 template <typename Fn>
-int guarded(lsim_sim *s, Fn &&fn) {
+int guarded(std::string *err, Fn &&fn) {
   try {
     fn();
-    if (s) s->err.clear();
+    if (err) err->clear();
     return LSIM_OK;
   } catch (const lsim::InputError &e) {
-    if (s) s->err = e.what();
+    if (err) *err = e.what();
     return LSIM_E_INPUT;
   } catch (const lsim::DeviceError &e) {
-    if (s) s->err = e.what();
+    if (err) *err = e.what();
     return e.code;
   } catch (const std::bad_alloc &) {
-    if (s) s->err = "host out of memory";
+    if (err) *err = "host out of memory";
     return LSIM_E_NOMEM;
   } catch (const std::exception &e) {
-    if (s) s->err = e.what();
+    if (err) *err = e.what();
     return LSIM_E_ARG;
   }
 }
 
-int fail(lsim_sim *s, int code, const std::string &msg) {
-  if (s) s->err = msg;
+int fail(std::string *err, int code, const std::string &msg) {
+  if (err) *err = msg;
   return code;
 }
 
@@ -63,40 +57,95 @@ bool read_file(const char *path, std::string &out) {
   return true;
 }
 
+int check_n(std::string *err, const void *p, size_t n, size_t want) {
+  if (!p && want) return fail(err, LSIM_E_ARG, "null output buffer");
+  if (n != want)
+    return fail(err, LSIM_E_ARG,
+                "output buffer holds " + std::to_string(n) + " elements, expected " + std::to_string(want));
+  return LSIM_OK;
+}
+
+// Report helpers shared by the three front ends.
+void fill_table(const lsim::RunResults &r, int64_t F, int64_t R, uint64_t *out) {
+  // {found in the last round, last round} (fsim/simulation.cpp:142-144); no
+  // rounds leaves the table zero-filled (fsim/simulation.cpp:116).
+  for (int64_t f = 0; f < F; ++f) {
+    out[2 * f] = R > 0 ? r.last_round[f] : 0;
+    out[2 * f + 1] = R > 0 ? uint64_t(R - 1) : 0;
+  }
+}
+
+void fill_matrix(const lsim::RunResults &r, int64_t F, int64_t R, uint8_t *out) {
+  for (int64_t f = 0; f < F; ++f)
+    for (int64_t k = 0; k < R; ++k) out[f * R + k] = uint8_t((r.det_bits[f * r.RB + (k >> 6)] >> (k & 63)) & 1);
+}
+
+struct ResidentCircuit {
+  lsim::Netlist nl;
+  lsim::FaultStructure fs;
+  lsim::CircuitRes *res = nullptr;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------ single simulator
+
+struct lsim_sim {
+  int device = -1;  // -1: host-only handle (read and accessors, no run)
+  std::unique_ptr<lsim::Engine> engine;
+  lsim::Netlist nl;
+  lsim::FaultStructure fs;
+  lsim::CircuitRes *circuit = nullptr;
+  bool loaded = false;
+  bool ran = false;
+  lsim::JobOptions jopt;
+  lsim::EngineOptions eopt;
+  lsim::RunResults res;
+  std::string err;
+};
+
+namespace {
+
 int load_texts(lsim_sim *s, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len, const char *ptn,
                size_t ptn_len) {
-  return guarded(s, [&] {
+  return guarded(&s->err, [&] {
     s->loaded = false;
     s->ran = false;
     s->res = lsim::RunResults();
+    if (s->engine && s->circuit) {
+      s->engine->remove_circuit(s->circuit);
+      s->circuit = nullptr;
+    }
     lsim::read_netlist(s->nl, ckt, ckt_len, flst, flst_len, ptn, ptn_len);
     lsim::build_fault_structure(s->nl, s->fs);
-    if (s->engine) s->engine->load(s->nl, s->fs);
+    if (s->engine) s->circuit = s->engine->add_circuit(s->nl, s->fs);
     s->loaded = true;
   });
 }
 
 int check_loaded(const lsim_sim *s) {
   if (!s) return LSIM_E_ARG;
-  if (!s->loaded) {
-    const_cast<lsim_sim *>(s)->err = "no circuit has been read";
-    return LSIM_E_STATE;
-  }
-  return LSIM_OK;
-}
-
-int check_n(const lsim_sim *s, const void *p, size_t n, size_t want) {
-  if (!p && want) return fail(const_cast<lsim_sim *>(s), LSIM_E_ARG, "null output buffer");
-  if (n != want)
-    return fail(const_cast<lsim_sim *>(s), LSIM_E_ARG,
-                "output buffer holds " + std::to_string(n) + " elements, expected " + std::to_string(want));
+  if (!s->loaded) return fail(&const_cast<lsim_sim *>(s)->err, LSIM_E_STATE, "no circuit has been read");
   return LSIM_OK;
 }
 
 int check_ran(const lsim_sim *s) {
   int rc = check_loaded(s);
   if (rc) return rc;
-  if (!s->ran) return fail(const_cast<lsim_sim *>(s), LSIM_E_STATE, "run() has not completed");
+  if (!s->ran) return fail(&const_cast<lsim_sim *>(s)->err, LSIM_E_STATE, "run() has not completed");
+  return LSIM_OK;
+}
+
+int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::string &k, int64_t value, bool &known) {
+  known = true;
+  if (k == "scratch_mb") {
+    eo.scratch_mb = value < 0 ? 0 : value;
+  } else if (k == "block_waves") {
+    if (value < 1 || value > 8) return fail(err, LSIM_E_ARG, "block_waves must be 1..8");
+    eo.block_waves = int(value);
+  } else {
+    known = false;
+  }
   return LSIM_OK;
 }
 
@@ -111,7 +160,7 @@ int lsim_create(int device, lsim_sim **out) {
   if (!s) return LSIM_E_NOMEM;
   s->device = device;
   if (device >= 0) {
-    int rc = guarded(s.get(), [&] { s->engine.reset(new lsim::Engine(device)); });
+    int rc = guarded(&s->err, [&] { s->engine.reset(new lsim::Engine(device)); });
     if (rc) {
       g_create_error = s->err;
       return rc;
@@ -121,7 +170,14 @@ int lsim_create(int device, lsim_sim **out) {
   return LSIM_OK;
 }
 
-void lsim_destroy(lsim_sim *s) { delete s; }
+void lsim_destroy(lsim_sim *s) {
+  if (!s) return;
+  try {
+    s->engine.reset();
+  } catch (...) {
+  }
+  delete s;
+}
 
 const char *lsim_last_error(const lsim_sim *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
 
@@ -129,9 +185,9 @@ int lsim_read_files(lsim_sim *s, const char *ckt_path, const char *flst_path, co
   if (!s || !ckt_path || !flst_path || !ptn_path) return LSIM_E_ARG;
   std::string c, f, p;
   // Messages as in fsim/simulation.cpp:24-32.
-  if (!read_file(ckt_path, c)) return fail(s, LSIM_E_IO, std::string("cannot open circut file ") + ckt_path);
-  if (!read_file(flst_path, f)) return fail(s, LSIM_E_IO, std::string("cannot open fault file ") + flst_path);
-  if (!read_file(ptn_path, p)) return fail(s, LSIM_E_IO, std::string("cannot open pattern file ") + ptn_path);
+  if (!read_file(ckt_path, c)) return fail(&s->err, LSIM_E_IO, std::string("cannot open circut file ") + ckt_path);
+  if (!read_file(flst_path, f)) return fail(&s->err, LSIM_E_IO, std::string("cannot open fault file ") + flst_path);
+  if (!read_file(ptn_path, p)) return fail(&s->err, LSIM_E_IO, std::string("cannot open pattern file ") + ptn_path);
   return load_texts(s, c.data(), c.size(), f.data(), f.size(), p.data(), p.size());
 }
 
@@ -144,15 +200,13 @@ int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, const char *
 int lsim_set_option(lsim_sim *s, const char *key, int64_t value) {
   if (!s || !key) return LSIM_E_ARG;
   const std::string k(key);
-  if (k == "drop") s->opt.drop = value != 0;
-  else if (k == "report_last_round") s->opt.report_last_round = value != 0;
-  else if (k == "scratch_mb") s->opt.scratch_mb = value < 0 ? 0 : value;
-  else if (k == "direct") s->opt.direct = value != 0;
-  else if (k == "block_waves") {
-    if (value < 1 || value > 16) return fail(s, LSIM_E_ARG, "block_waves must be 1..16");
-    s->opt.block_waves = int(value);
-  } else
-    return fail(s, LSIM_E_ARG, "unknown option " + k);
+  bool known = false;
+  int rc = set_engine_option(s->eopt, &s->err, k, value, known);
+  if (rc || known) return rc;
+  if (k == "drop") s->jopt.drop = value != 0;
+  else if (k == "report_last_round") s->jopt.report_last_round = value != 0;
+  else if (k == "direct") s->jopt.direct = value != 0;
+  else return fail(&s->err, LSIM_E_ARG, "unknown option " + k);
   return LSIM_OK;
 }
 
@@ -160,19 +214,30 @@ int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
   (void)num_threads;
   int rc = check_loaded(s);
   if (rc) return rc;
-  if (!s->engine) return fail(s, LSIM_E_DEVICE, "handle was created without a device (device -1)");
+  if (!s->engine) return fail(&s->err, LSIM_E_DEVICE, "handle was created without a device (device -1)");
   int m;
   switch (mode) {
     case LSIM_MODE_REF_SEQUENTIAL:
     case LSIM_MODE_REF_PARALLEL_OMP: m = LSIM_MODE_PLAIN; break;
     case LSIM_MODE_REF_PARALLEL_TF: m = LSIM_MODE_PARTITIONED; break;
-    default: return fail(s, LSIM_E_ARG, "unknown mode " + std::to_string(mode));
+    default: return fail(&s->err, LSIM_E_ARG, "unknown mode " + std::to_string(mode));
   }
-  return guarded(s, [&] {
+  return guarded(&s->err, [&] {
     s->ran = false;
-    lsim::RunOptions o = s->opt;
-    o.mode = m;
-    s->engine->run(o, s->res);
+    lsim::EngineOptions eo = s->eopt;
+    eo.mode = m;
+    lsim::JobOptions jo = s->jopt;
+    jo.keep_good_last = true;
+    lsim::JobState *job = s->engine->add_job(s->circuit, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns,
+                                             s->nl.R, s->nl.patterns, jo);
+    try {
+      s->engine->run({job}, eo);
+      s->res = std::move(s->engine->results(job));
+    } catch (...) {
+      s->engine->release_job(job);
+      throw;
+    }
+    s->engine->release_job(job);
     s->ran = true;
   });
 }
@@ -188,28 +253,29 @@ int64_t lsim_num_patterns(const lsim_sim *s) { return s && s->loaded ? s->nl.num
 
 int lsim_gate_idx_to_tpg_order_idx(const lsim_sim *s, uint64_t *out, size_t n) {
   int rc = check_loaded(s);
-  if (rc || (rc = check_n(s, out, n, size_t(s->nl.N)))) return rc;
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, size_t(s->nl.N)))) return rc;
   for (size_t i = 0; i < n; ++i) out[i] = uint64_t(s->nl.order_idx[i]);
   return LSIM_OK;
 }
 
 int lsim_num_gates_per_level(const lsim_sim *s, uint64_t *out, size_t n) {
   int rc = check_loaded(s);
-  if (rc || (rc = check_n(s, out, n, s->nl.gates_per_level.size()))) return rc;
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, s->nl.gates_per_level.size()))) return rc;
   for (size_t i = 0; i < n; ++i) out[i] = uint64_t(s->nl.gates_per_level[i]);
   return LSIM_OK;
 }
 
 int lsim_levels(const lsim_sim *s, uint64_t *out, size_t n) {
   int rc = check_loaded(s);
-  if (rc || (rc = check_n(s, out, n, size_t(s->nl.N)))) return rc;
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, size_t(s->nl.N)))) return rc;
   for (size_t i = 0; i < n; ++i) out[i] = uint64_t(s->nl.level[i]);
   return LSIM_OK;
 }
 
 int lsim_faults(const lsim_sim *s, uint64_t *site, uint64_t *value, size_t n) {
   int rc = check_loaded(s);
-  if (rc || (rc = check_n(s, site, n, s->nl.fault_site.size())) || (rc = check_n(s, value, n, n))) return rc;
+  std::string *e = s ? &const_cast<lsim_sim *>(s)->err : nullptr;
+  if (rc || (rc = check_n(e, site, n, s->nl.fault_site.size())) || (rc = check_n(e, value, n, n))) return rc;
   for (size_t i = 0; i < n; ++i) {
     site[i] = uint64_t(s->nl.fault_site[i]);
     value[i] = s->nl.fault_sa[i] ? ~uint64_t(0) : 0;
@@ -219,16 +285,17 @@ int lsim_faults(const lsim_sim *s, uint64_t *site, uint64_t *value, size_t n) {
 
 int lsim_patterns(const lsim_sim *s, uint64_t *out, size_t n) {
   int rc = check_loaded(s);
-  if (rc || (rc = check_n(s, out, n, s->nl.patterns.size()))) return rc;
-  std::memcpy(out, s->nl.patterns.data(), n * sizeof(uint64_t));
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, s->nl.patterns.size()))) return rc;
+  if (n) std::memcpy(out, s->nl.patterns.data(), n * sizeof(uint64_t));
   return LSIM_OK;
 }
 
 int lsim_good_results(const lsim_sim *s, uint64_t *pi, size_t n_pi, uint64_t *gate, size_t n_gate, uint64_t *po,
                       size_t n_po) {
   int rc = check_ran(s);
-  if (rc || (rc = check_n(s, pi, n_pi, size_t(s->nl.P))) || (rc = check_n(s, gate, n_gate, size_t(s->nl.G))) ||
-      (rc = check_n(s, po, n_po, size_t(s->nl.O))))
+  std::string *e = s ? &const_cast<lsim_sim *>(s)->err : nullptr;
+  if (rc || (rc = check_n(e, pi, n_pi, size_t(s->nl.P))) || (rc = check_n(e, gate, n_gate, size_t(s->nl.G))) ||
+      (rc = check_n(e, po, n_po, size_t(s->nl.O))))
     return rc;
   const auto &g = s->res.good_last;
   for (size_t i = 0; i < n_pi; ++i) pi[i] = g[i];
@@ -240,30 +307,24 @@ int lsim_good_results(const lsim_sim *s, uint64_t *pi, size_t n_pi, uint64_t *ga
 int lsim_found_fault_to_pattern(const lsim_sim *s, uint64_t *out, size_t n) {
   int rc = check_ran(s);
   const size_t F = s ? s->nl.fault_site.size() : 0;
-  if (rc || (rc = check_n(s, out, n, 2 * F))) return rc;
+  std::string *e = s ? &const_cast<lsim_sim *>(s)->err : nullptr;
+  if (rc || (rc = check_n(e, out, n, 2 * F))) return rc;
   if (!s->res.has_last)
-    return fail(const_cast<lsim_sim *>(s), LSIM_E_STATE,
-                "the last round was not evaluated for every fault (dropping on, report_last_round off)");
-  // {found in the last round, last round} (fsim/simulation.cpp:142-144);
-  // no rounds leaves the table zero-filled (fsim/simulation.cpp:116).
-  const int64_t R = s->nl.R;
-  for (size_t f = 0; f < F; ++f) {
-    out[2 * f] = R > 0 ? s->res.last_round[f] : 0;
-    out[2 * f + 1] = R > 0 ? uint64_t(R - 1) : 0;
-  }
+    return fail(e, LSIM_E_STATE, "the last round was not evaluated for every fault (dropping on, report_last_round off)");
+  fill_table(s->res, int64_t(F), s->nl.R, out);
   return LSIM_OK;
 }
 
 int lsim_detected(const lsim_sim *s, uint8_t *out, size_t n) {
   int rc = check_ran(s);
-  if (rc || (rc = check_n(s, out, n, s->nl.fault_site.size()))) return rc;
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, s->nl.fault_site.size()))) return rc;
   for (size_t f = 0; f < n; ++f) out[f] = s->res.first_round[f] >= 0;
   return LSIM_OK;
 }
 
 int lsim_first_detect_round(const lsim_sim *s, int64_t *out, size_t n) {
   int rc = check_ran(s);
-  if (rc || (rc = check_n(s, out, n, s->nl.fault_site.size()))) return rc;
+  if (rc || (rc = check_n(&const_cast<lsim_sim *>(s)->err, out, n, s->nl.fault_site.size()))) return rc;
   for (size_t f = 0; f < n; ++f) out[f] = s->res.first_round[f];
   return LSIM_OK;
 }
@@ -272,12 +333,10 @@ int lsim_detect_matrix(const lsim_sim *s, uint8_t *out, size_t n) {
   int rc = check_ran(s);
   const size_t F = s ? s->nl.fault_site.size() : 0;
   const int64_t R = s ? s->nl.R : 0;
-  if (rc || (rc = check_n(s, out, n, F * size_t(R)))) return rc;
-  if (!s->res.has_matrix)
-    return fail(const_cast<lsim_sim *>(s), LSIM_E_STATE, "the run used fault dropping; no per-round matrix");
-  const int64_t RB = s->res.RB;
-  for (size_t f = 0; f < F; ++f)
-    for (int64_t r = 0; r < R; ++r) out[f * R + r] = uint8_t((s->res.det_bits[f * RB + (r >> 6)] >> (r & 63)) & 1);
+  std::string *e = s ? &const_cast<lsim_sim *>(s)->err : nullptr;
+  if (rc || (rc = check_n(e, out, n, F * size_t(R)))) return rc;
+  if (!s->res.has_matrix) return fail(e, LSIM_E_STATE, "the run used fault dropping; no per-round matrix");
+  fill_matrix(s->res, int64_t(F), R, out);
   return LSIM_OK;
 }
 
@@ -289,12 +348,520 @@ int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out) {
   return LSIM_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------- batching
+
+struct BatchJob {
+  int64_t circuit = -1;
+  int64_t F = 0, R = 0;
+  lsim::JobState *js = nullptr;
+  bool finished = false;
+  lsim::RunResults res;
+};
+
+struct lsim_batch {
+  std::unique_ptr<lsim::Engine> engine;
+  lsim::EngineOptions eopt;
+  std::vector<std::unique_ptr<ResidentCircuit>> circuits;
+  std::map<int64_t, BatchJob> jobs;
+  int64_t next_job = 0;
+  lsim_run_stats stats{};
+  std::string err;
+};
+
+extern "C" {
+
+int lsim_batch_create(int device, lsim_batch **out) {
+  if (!out) return LSIM_E_ARG;
+  *out = nullptr;
+  std::unique_ptr<lsim_batch> b(new (std::nothrow) lsim_batch);
+  if (!b) return LSIM_E_NOMEM;
+  int rc = guarded(&b->err, [&] { b->engine.reset(new lsim::Engine(device)); });
+  if (rc) {
+    g_create_error = b->err;
+    return rc;
+  }
+  *out = b.release();
+  return LSIM_OK;
+}
+
+void lsim_batch_destroy(lsim_batch *b) {
+  if (!b) return;
+  try {
+    b->engine.reset();
+  } catch (...) {
+  }
+  delete b;
+}
+
+const char *lsim_batch_last_error(const lsim_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value) {
+  if (!b || !key) return LSIM_E_ARG;
+  const std::string k(key);
+  bool known = false;
+  int rc = set_engine_option(b->eopt, &b->err, k, value, known);
+  if (rc || known) return rc;
+  if (k == "mode") {
+    if (value != LSIM_MODE_PLAIN && value != LSIM_MODE_PARTITIONED) return fail(&b->err, LSIM_E_ARG, "mode must be 0 or 1");
+    b->eopt.mode = int(value);
+    return LSIM_OK;
+  }
+  return fail(&b->err, LSIM_E_ARG, "unknown option " + k);
+}
+
+int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id) {
+  if (!b || (!ckt && ckt_len) || !circuit_id) return LSIM_E_ARG;
+  return guarded(&b->err, [&] {
+    auto rc = std::make_unique<ResidentCircuit>();
+    lsim::read_circuit(rc->nl, ckt, ckt_len);
+    lsim::build_fault_structure(rc->nl, rc->fs);
+    rc->res = b->engine->add_circuit(rc->nl, rc->fs);
+    *circuit_id = int64_t(b->circuits.size());
+    b->circuits.push_back(std::move(rc));
+  });
+}
+
+int lsim_batch_add_job(lsim_batch *b, int64_t circuit_id, const char *flst, size_t flst_len, const char *ptn,
+                       size_t ptn_len, int drop, int64_t *job_id) {
+  if (!b || !job_id || (!flst && flst_len) || (!ptn && ptn_len)) return LSIM_E_ARG;
+  if (circuit_id < 0 || circuit_id >= int64_t(b->circuits.size())) return fail(&b->err, LSIM_E_ARG, "unknown circuit");
+  return guarded(&b->err, [&] {
+    const ResidentCircuit &c = *b->circuits[size_t(circuit_id)];
+    std::vector<int32_t> site;
+    std::vector<uint8_t> sa;
+    int64_t n = 0, R = 0;
+    std::vector<uint64_t> words;
+    lsim::read_faults(c.nl, flst, flst_len, site, sa);
+    lsim::read_patterns(c.nl, ptn, ptn_len, n, R, words);
+    lsim::JobOptions jo;
+    jo.drop = drop != 0;
+    jo.report_last_round = true;
+    BatchJob bj;
+    bj.circuit = circuit_id;
+    bj.F = int64_t(site.size());
+    bj.R = R;
+    bj.js = b->engine->add_job(c.res, std::move(site), std::move(sa), n, R, words, jo);
+    *job_id = b->next_job++;
+    b->jobs[*job_id] = std::move(bj);
+  });
+}
+
+int lsim_batch_run(lsim_batch *b) {
+  if (!b) return LSIM_E_ARG;
+  return guarded(&b->err, [&] {
+    std::vector<lsim::JobState *> js;
+    std::vector<BatchJob *> owners;
+    for (auto &kv : b->jobs)
+      if (!kv.second.finished) {
+        js.push_back(kv.second.js);
+        owners.push_back(&kv.second);
+      }
+    const auto t0 = std::chrono::steady_clock::now();
+    b->engine->run(js, b->eopt);
+    b->stats = lsim_run_stats{};
+    b->stats.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (BatchJob *o : owners) {
+      o->res = std::move(b->engine->results(o->js));
+      b->engine->release_job(o->js);
+      o->js = nullptr;
+      o->finished = true;
+      const lsim_run_stats &s = o->res.stats;
+      // Device phases are shared by the batch: keep the largest.
+      b->stats.t_good_ms = std::max(b->stats.t_good_ms, s.t_good_ms);
+      b->stats.t_sweep_ms = std::max(b->stats.t_sweep_ms, s.t_sweep_ms);
+      b->stats.t_detect_ms = std::max(b->stats.t_detect_ms, s.t_detect_ms);
+      b->stats.live_faults += s.live_faults;
+      b->stats.stems += s.stems;
+      b->stats.steps = std::max(b->stats.steps, s.steps);
+      b->stats.sweep_launches = std::max(b->stats.sweep_launches, s.sweep_launches);
+      b->stats.sweep_tiles = std::max(b->stats.sweep_tiles, s.sweep_tiles);
+      b->stats.sweep_node_steps = std::max(b->stats.sweep_node_steps, s.sweep_node_steps);
+      b->stats.sweep_bytes = std::max(b->stats.sweep_bytes, s.sweep_bytes);
+      b->stats.slots = s.slots;
+      b->stats.good_bytes += s.good_bytes;
+    }
+  });
+}
+
+static const BatchJob *batch_job(const lsim_batch *b, int64_t id, bool need_finished) {
+  if (!b) return nullptr;
+  auto it = b->jobs.find(id);
+  if (it == b->jobs.end()) return nullptr;
+  if (need_finished && !it->second.finished) return nullptr;
+  return &it->second;
+}
+
+int64_t lsim_batch_job_faults(const lsim_batch *b, int64_t job_id) {
+  const BatchJob *j = batch_job(b, job_id, false);
+  return j ? j->F : -1;
+}
+
+int64_t lsim_batch_job_rounds(const lsim_batch *b, int64_t job_id) {
+  const BatchJob *j = batch_job(b, job_id, false);
+  return j ? j->R : -1;
+}
+
+int lsim_batch_first_detect_round(const lsim_batch *b, int64_t job_id, int64_t *out, size_t n) {
+  const BatchJob *j = batch_job(b, job_id, true);
+  if (!j) return LSIM_E_STATE;
+  int rc = check_n(&const_cast<lsim_batch *>(b)->err, out, n, size_t(j->F));
+  if (rc) return rc;
+  for (size_t f = 0; f < n; ++f) out[f] = j->res.first_round[f];
+  return LSIM_OK;
+}
+
+int lsim_batch_detect_matrix(const lsim_batch *b, int64_t job_id, uint8_t *out, size_t n) {
+  const BatchJob *j = batch_job(b, job_id, true);
+  if (!j) return LSIM_E_STATE;
+  std::string *e = &const_cast<lsim_batch *>(b)->err;
+  int rc = check_n(e, out, n, size_t(j->F * j->R));
+  if (rc) return rc;
+  if (!j->res.has_matrix) return fail(e, LSIM_E_STATE, "the job used fault dropping; no per-round matrix");
+  fill_matrix(j->res, j->F, j->R, out);
+  return LSIM_OK;
+}
+
+int lsim_batch_found_fault_to_pattern(const lsim_batch *b, int64_t job_id, uint64_t *out, size_t n) {
+  const BatchJob *j = batch_job(b, job_id, true);
+  if (!j) return LSIM_E_STATE;
+  std::string *e = &const_cast<lsim_batch *>(b)->err;
+  int rc = check_n(e, out, n, size_t(2 * j->F));
+  if (rc) return rc;
+  if (!j->res.has_last) return fail(e, LSIM_E_STATE, "the last round was not evaluated for every fault");
+  fill_table(j->res, j->F, j->R, out);
+  return LSIM_OK;
+}
+
+int lsim_batch_last_run_stats(const lsim_batch *b, lsim_run_stats *out) {
+  if (!b || !out) return LSIM_E_ARG;
+  *out = b->stats;
+  return LSIM_OK;
+}
+
+int lsim_batch_release_job(lsim_batch *b, int64_t job_id) {
+  if (!b) return LSIM_E_ARG;
+  auto it = b->jobs.find(job_id);
+  if (it == b->jobs.end()) return fail(&b->err, LSIM_E_ARG, "unknown job");
+  return guarded(&b->err, [&] {
+    if (it->second.js) b->engine->release_job(it->second.js);
+    b->jobs.erase(it);
+  });
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------------- serving
+
+struct ServerJob {
+  int64_t id = 0;
+  int64_t circuit = -1;
+  std::vector<int32_t> site;
+  std::vector<uint8_t> sa;
+  int64_t n = 0, R = 0;
+  std::vector<uint64_t> words;
+  lsim::JobOptions opt;
+  int64_t bytes = 0;
+  lsim::JobState *js = nullptr;
+  double t_submit = 0, t_admit = 0, t_finish = 0;
+  int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  int code = 0;
+  std::string error;
+  lsim::RunResults res;
+};
+
+struct lsim_server {
+  std::unique_ptr<lsim::Engine> engine;
+  lsim::EngineOptions eopt;
+  int64_t budget = 0, used = 0;
+  std::mutex mu;  // queue, job table, counters
+  std::condition_variable cv_work, cv_done;
+  std::mutex engine_mu;  // every Engine call
+  std::vector<std::unique_ptr<ResidentCircuit>> circuits;
+  std::deque<ServerJob *> queue;
+  std::vector<ServerJob *> running;
+  std::map<int64_t, std::unique_ptr<ServerJob>> jobs;
+  int64_t next_id = 0;
+  int64_t iterations = 0, finished = 0, failed = 0, peak = 0;
+  bool stop = false;
+  std::thread worker;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  std::string err;
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  void loop();
+};
+
+void lsim_server::loop() {
+  for (;;) {
+    std::vector<ServerJob *> admit;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+      if (stop) return;
+      // Admission in arrival order while the footprint fits the budget; a job
+      // that cannot fit even on an idle device fails instead of blocking.
+      while (!queue.empty()) {
+        ServerJob *j = queue.front();
+        if (used + j->bytes <= budget) {
+          used += j->bytes;
+          admit.push_back(j);
+          queue.pop_front();
+        } else if (running.empty() && admit.empty()) {
+          j->state = 3;
+          j->code = LSIM_E_NOMEM;
+          j->error = "job needs " + std::to_string(j->bytes) + " bytes, above the server's job budget";
+          j->t_finish = now();
+          ++failed;
+          queue.pop_front();
+          cv_done.notify_all();
+        } else {
+          break;
+        }
+      }
+    }
+    std::vector<ServerJob *> done_now;
+    std::vector<ServerJob *> failed_now;
+    {
+      std::lock_guard<std::mutex> elk(engine_mu);
+      std::vector<ServerJob *> fresh;
+      for (ServerJob *j : admit) {
+        int rc = guarded(&j->error, [&] {
+          j->js = engine->add_job(circuits[size_t(j->circuit)]->res, std::move(j->site), std::move(j->sa), j->n, j->R,
+                                  j->words, j->opt);
+        });
+        j->words.clear();
+        j->words.shrink_to_fit();
+        if (rc) {
+          j->code = rc;
+          failed_now.push_back(j);
+        } else {
+          j->t_admit = now();
+          fresh.push_back(j);
+        }
+      }
+      std::vector<lsim::JobState *> js;
+      std::vector<ServerJob *> live;
+      for (ServerJob *j : running) live.push_back(j);
+      for (ServerJob *j : fresh) live.push_back(j);
+      for (ServerJob *j : live) js.push_back(j->js);
+      std::string msg;
+      int rc = guarded(&msg, [&] {
+        std::vector<lsim::JobState *> fs;
+        for (ServerJob *j : fresh) fs.push_back(j->js);
+        engine->start(fs);
+        engine->iterate(js, eopt);
+      });
+      std::vector<ServerJob *> still;
+      for (ServerJob *j : live) {
+        if (rc) {
+          // The iteration failed: its jobs fail, the server keeps serving.
+          j->code = rc;
+          j->error = msg;
+          engine->release_job(j->js);
+          j->js = nullptr;
+          failed_now.push_back(j);
+        } else if (engine->done(j->js)) {
+          j->res = std::move(engine->results(j->js));
+          engine->release_job(j->js);
+          j->js = nullptr;
+          done_now.push_back(j);
+        } else {
+          still.push_back(j);
+        }
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      running.swap(still);
+      peak = std::max<int64_t>(peak, int64_t(running.size() + done_now.size()));
+      ++iterations;
+      const double t = now();
+      for (ServerJob *j : done_now) {
+        j->state = 2;
+        j->t_finish = t;
+        used -= j->bytes;
+        ++finished;
+      }
+      for (ServerJob *j : failed_now) {
+        j->state = 3;
+        j->t_finish = t;
+        used -= j->bytes;
+        ++failed;
+      }
+    }
+    if (!done_now.empty() || !failed_now.empty()) cv_done.notify_all();
+  }
+}
+
+extern "C" {
+
+int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, lsim_server **out) {
+  if (!out || job_budget_mb <= 0) return LSIM_E_ARG;
+  *out = nullptr;
+  std::unique_ptr<lsim_server> s(new (std::nothrow) lsim_server);
+  if (!s) return LSIM_E_NOMEM;
+  int rc = guarded(&s->err, [&] {
+    s->engine.reset(new lsim::Engine(device));
+    s->budget = job_budget_mb << 20;
+    s->eopt.mode = LSIM_MODE_PARTITIONED;
+    s->eopt.scratch_mb = scratch_mb > 0 ? scratch_mb : 8192;
+    s->worker = std::thread([p = s.get()] { p->loop(); });
+  });
+  if (rc) {
+    g_create_error = s->err;
+    return rc;
+  }
+  *out = s.release();
+  return LSIM_OK;
+}
+
+void lsim_server_destroy(lsim_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  try {
+    std::lock_guard<std::mutex> elk(s->engine_mu);
+    for (auto &kv : s->jobs)
+      if (kv.second->js) s->engine->release_job(kv.second->js);
+    s->engine.reset();
+  } catch (...) {
+  }
+  delete s;
+}
+
+const char *lsim_server_last_error(const lsim_server *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+int lsim_server_add_circuit(lsim_server *s, const char *ckt, size_t ckt_len, int64_t *circuit_id) {
+  if (!s || (!ckt && ckt_len) || !circuit_id) return LSIM_E_ARG;
+  auto rc_ptr = std::make_unique<ResidentCircuit>();
+  std::string msg;
+  int rc = guarded(&msg, [&] {
+    lsim::read_circuit(rc_ptr->nl, ckt, ckt_len);
+    lsim::build_fault_structure(rc_ptr->nl, rc_ptr->fs);
+    std::lock_guard<std::mutex> elk(s->engine_mu);
+    rc_ptr->res = s->engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
+  });
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc) {
+    s->err = msg;
+    return rc;
+  }
+  *circuit_id = int64_t(s->circuits.size());
+  s->circuits.push_back(std::move(rc_ptr));
+  return LSIM_OK;
+}
+
+int lsim_server_submit(lsim_server *s, int64_t circuit_id, const char *flst, size_t flst_len, const char *ptn,
+                       size_t ptn_len, int drop, int64_t *job_id) {
+  if (!s || !job_id || (!flst && flst_len) || (!ptn && ptn_len)) return LSIM_E_ARG;
+  const ResidentCircuit *c;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (circuit_id < 0 || circuit_id >= int64_t(s->circuits.size())) return fail(&s->err, LSIM_E_ARG, "unknown circuit");
+    c = s->circuits[size_t(circuit_id)].get();
+  }
+  auto j = std::make_unique<ServerJob>();
+  std::string msg;
+  int rc = guarded(&msg, [&] {
+    lsim::read_faults(c->nl, flst, flst_len, j->site, j->sa);
+    lsim::read_patterns(c->nl, ptn, ptn_len, j->n, j->R, j->words);
+  });
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc) {
+    s->err = msg;
+    return rc;
+  }
+  j->circuit = circuit_id;
+  j->opt.drop = drop != 0;
+  j->opt.report_last_round = false;
+  j->bytes = lsim::Engine::job_bytes(c->res, int64_t(j->site.size()), j->R, j->opt.drop);
+  j->t_submit = s->now();
+  j->id = s->next_id++;
+  *job_id = j->id;
+  s->queue.push_back(j.get());
+  s->jobs[j->id] = std::move(j);
+  s->cv_work.notify_all();
+  return LSIM_OK;
+}
+
+int lsim_server_wait(lsim_server *s, int64_t job_id, double timeout_s) {
+  if (!s) return LSIM_E_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(&s->err, LSIM_E_ARG, "unknown job");
+  ServerJob *j = it->second.get();
+  auto ready = [&] { return j->state >= 2; };
+  if (timeout_s < 0) {
+    s->cv_done.wait(lk, ready);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+    return fail(&s->err, LSIM_E_STATE, "job not finished within the timeout");
+  }
+  if (j->state == 3) return fail(&s->err, j->code ? j->code : LSIM_E_DEVICE, j->error);
+  return LSIM_OK;
+}
+
+int lsim_server_first_detect_round(lsim_server *s, int64_t job_id, int64_t *out, size_t n) {
+  if (!s) return LSIM_E_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end() || it->second->state != 2) return fail(&s->err, LSIM_E_STATE, "job not finished");
+  const ServerJob &j = *it->second;
+  int rc = check_n(&s->err, out, n, j.res.first_round.size());
+  if (rc) return rc;
+  for (size_t f = 0; f < n; ++f) out[f] = j.res.first_round[f];
+  return LSIM_OK;
+}
+
+int64_t lsim_server_job_faults(lsim_server *s, int64_t job_id) {
+  if (!s) return -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return -1;
+  return it->second->state == 2 ? int64_t(it->second->res.first_round.size()) : int64_t(it->second->site.size());
+}
+
+int lsim_server_job_times(lsim_server *s, int64_t job_id, double *times) {
+  if (!s || !times) return LSIM_E_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(&s->err, LSIM_E_ARG, "unknown job");
+  times[0] = it->second->t_submit;
+  times[1] = it->second->t_admit;
+  times[2] = it->second->t_finish;
+  return LSIM_OK;
+}
+
+int lsim_server_release_job(lsim_server *s, int64_t job_id) {
+  if (!s) return LSIM_E_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(&s->err, LSIM_E_ARG, "unknown job");
+  if (it->second->state < 2) return fail(&s->err, LSIM_E_STATE, "job still queued or running");
+  s->jobs.erase(it);
+  return LSIM_OK;
+}
+
+int lsim_server_counters(lsim_server *s, int64_t *counters) {
+  if (!s || !counters) return LSIM_E_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  counters[0] = s->iterations;
+  counters[1] = s->finished;
+  counters[2] = s->failed;
+  counters[3] = s->peak;
+  return LSIM_OK;
+}
+
+// ------------------------------------------------------------------ generator
+
 int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char **flst, size_t *flst_len, char **ptn,
                   size_t *ptn_len) {
   if (!spec || !ckt || !ckt_len || !flst || !flst_len || !ptn || !ptn_len) return LSIM_E_ARG;
   *ckt = *flst = *ptn = nullptr;
-  lsim_sim scratch;  // carries the error message of this call
-  int rc = guarded(&scratch, [&] {
+  std::string msg;
+  int rc = guarded(&msg, [&] {
     lsim::GenSpec g;
     g.seed = spec->seed;
     g.num_pis = spec->num_pis;
@@ -323,7 +890,7 @@ int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char *
     dup(p, ptn, ptn_len);
   });
   if (rc) {
-    g_create_error = scratch.err;
+    g_create_error = msg;
     std::free(*ckt);
     std::free(*flst);
     std::free(*ptn);
@@ -333,5 +900,31 @@ int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char *
 }
 
 void lsim_free_text(char *p) { std::free(p); }
+
+static int emit_text(const std::string &x, char **o, size_t *n) {
+  *o = static_cast<char *>(std::malloc(x.size() + 1));
+  if (!*o) return LSIM_E_NOMEM;
+  std::memcpy(*o, x.data(), x.size());
+  (*o)[x.size()] = 0;
+  *n = x.size();
+  return LSIM_OK;
+}
+
+int lsim_generate_faults(uint64_t seed, int64_t num_pis, int64_t num_pos, int64_t num_gates, double fraction,
+                         char **flst, size_t *flst_len) {
+  if (!flst || !flst_len) return LSIM_E_ARG;
+  *flst = nullptr;
+  std::string text;
+  int rc = guarded(&g_create_error, [&] { text = lsim::generate_flst(seed, num_pis, num_pos, num_gates, fraction); });
+  return rc ? rc : emit_text(text, flst, flst_len);
+}
+
+int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn, size_t *ptn_len) {
+  if (!ptn || !ptn_len) return LSIM_E_ARG;
+  *ptn = nullptr;
+  std::string text;
+  int rc = guarded(&g_create_error, [&] { text = lsim::generate_ptn(seed, num_pis, num_patterns); });
+  return rc ? rc : emit_text(text, ptn, ptn_len);
+}
 
 }  // extern "C"

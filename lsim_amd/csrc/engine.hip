@@ -6,7 +6,7 @@
 // The reference re-simulates the whole circuit per (fault, round); here:
 //
 //  * Good machine: one pass over all rounds, node-major layout good[node][R]
-//    (lane = round word), one launch per level.
+//    (lane = round word), one batched launch per level over all started jobs.
 //  * Faults whose site cannot reach a PO are never detected and are not
 //    simulated.
 //  * Stem tracing (circuits without MUX): inside a fanout-free region a flip
@@ -22,15 +22,16 @@
 //    which couples the 64 patterns of a word; circuits with a MUX inject
 //    every fault directly (full word), skipping words where the site's good
 //    value already equals the stuck value.
-//  * Sweeps: a tile holds up to 64 lanes, each lane one (injection site,
-//    word, injected bits). A wave evaluates one node for all lanes of a tile;
-//    per node the tile keeps a 64-bit mask of the lanes whose value differs
-//    from the good machine and a 64-word row of faulty values.
+//  * Sweeps: a tile holds up to 64 lanes of one job, each lane one (injection
+//    site, word, injected bits). A wave evaluates one node for all lanes of a
+//    tile; per node the tile keeps a 64-bit mask of the lanes whose value
+//    differs from the good machine and a 64-word row of faulty values.
 //      - plain form: one launch per level, every live node of the level
-//        visited by every tile of the batch;
-//      - event form: one persistent launch; a workgroup owns a tile at a
-//        time and visits only nodes with a differing fanin, compacted per
-//        level into an LDS worklist, ending at the deepest pending level.
+//        visited by every tile of the batch (one job at a time);
+//      - event form: one persistent launch over the tiles of all active jobs;
+//        a workgroup owns a tile at a time and visits only nodes with a
+//        differing fanin, staged per level in LDS, ending at the deepest
+//        pending level.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +82,7 @@ struct Tile {
   int32_t nlanes;       // 1..64
   int32_t start_level;  // level of the lowest injection site
   int32_t end_level;    // level of the highest injection site
+  int32_t job;          // index into the iteration's job table
 };
 
 // Per-tile record of a node that differs from the good machine in some lane:
@@ -90,6 +92,67 @@ struct MaskRec {
   uint32_t epoch;
   uint32_t pad;
 };
+
+// Everything a kernel needs about one job and its circuit in this iteration.
+struct JobDev {
+  // circuit (device positions: live nodes first, then dead nodes)
+  const uint32_t *info;      // type | fanin count << 4
+  const int4 *fin;           // fanin positions (PI: PI index in .x)
+  const int32_t *live_lvl;   // first live position of each level, L+2 entries
+  const int32_t *dead_lvl;   // first dead position of each level, L+2 entries
+  const int32_t *fo_off;     // live fanouts of live nodes
+  const int2 *fo;            // {position, level}, ascending
+  const int32_t *parent;     // fanout-free-region parent, -1 at heads
+  const uint8_t *is_po;      // per live position
+  int max_level;
+  int NL;
+  // job
+  uint64_t *good;            // [N][RW]
+  const uint64_t *pi_words;  // [P][RW]
+  int64_t RW;
+  int64_t num_patterns;
+  const int32_t *site;       // per fault, live position (dead faults never listed)
+  const int32_t *head;       // per fault, region head
+  const uint8_t *sa;
+  int32_t *head_slot;        // per live position: the stem's unit index in this step, else -1
+  int32_t *first_round;      // atomicMin, or null
+  uint64_t *det_bits;        // [F][RB], or null
+  int64_t RB;
+  uint8_t *last_flag;        // d(f, R-1), or null
+  int64_t last_word;
+  // this step
+  const int32_t *flist;      // faults of the step (direct: in unit order)
+  int64_t nf;
+  int64_t w0;
+  int Wn;
+  int direct;
+  int64_t item0;             // first detection item of the job
+  int64_t word0;             // first need/dout word of the job's units
+};
+
+// The fields a sweep reads, copied out of the job table once per tile so
+// they stay in registers (stores to sweep state cannot alias them).
+struct SweepView {
+  const uint32_t *__restrict__ info;
+  const int4 *__restrict__ fin;
+  const uint64_t *__restrict__ good;
+  const int32_t *__restrict__ live_lvl;
+  const int32_t *__restrict__ fo_off;
+  const int2 *__restrict__ fo;
+  int64_t RW;
+};
+
+__device__ __forceinline__ SweepView sweep_view(const JobDev *jobs, int j) {
+  SweepView v;
+  v.info = jobs[j].info;
+  v.fin = jobs[j].fin;
+  v.good = jobs[j].good;
+  v.live_lvl = jobs[j].live_lvl;
+  v.fo_off = jobs[j].fo_off;
+  v.fo = jobs[j].fo;
+  v.RW = jobs[j].RW;
+  return v;
+}
 
 __device__ __forceinline__ uint64_t eval_gate(uint32_t type, uint32_t nin, uint64_t v0, uint64_t v1, uint64_t v2,
                                               uint64_t v3) {
@@ -130,25 +193,40 @@ __device__ __forceinline__ uint64_t round_mask(int64_t num_patterns, int64_t w) 
   return bits < 64 ? ((uint64_t(1) << bits) - 1) : ~uint64_t(0);
 }
 
+// Index of the job whose item range [off[j], off[j+1]) holds t.
+__device__ __forceinline__ int find_job(const int64_t *off, int njobs, int64_t t) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------- good machine
 
-// Threads cover (node, word) pairs of two device-position ranges (the live
-// and the dead nodes of one level); word is the fastest index so a wave
-// reads a node's consecutive words in one coalesced access.
-__global__ void k_good_level(const uint32_t *__restrict__ info, const int4 *__restrict__ fin,
-                             uint64_t *__restrict__ good, const uint64_t *__restrict__ pi_words, int64_t RW, int a0,
-                             int n0, int a1, int n1) {
+// Level `level` of every job: threads cover (job, node, word); word is the
+// fastest index so a wave reads a node's consecutive words in one coalesced
+// access. off = per-job item prefix for this level.
+__global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const int64_t *__restrict__ off,
+                             int level) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(n0 + n1) * RW) return;
-  const int i = int(t / RW);
-  const int64_t w = t - int64_t(i) * RW;
-  const int u = i < n0 ? a0 + i : a1 + (i - n0);
-  const uint32_t inf = info[u];
+  if (t >= off[njobs]) return;
+  const int j = find_job(off, njobs, t);
+  const JobDev &J = jobs[j];
+  const int64_t RW = J.RW;
+  const int64_t lt = t - off[j];
+  const int i = int(lt / RW);
+  const int64_t w = lt - int64_t(i) * RW;
+  const int a0 = J.live_lvl[level], n0 = J.live_lvl[level + 1] - a0;
+  const int u = i < n0 ? a0 + i : J.dead_lvl[level] + (i - n0);
+  const uint32_t inf = J.info[u];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-  const int4 f = fin[u];
+  const int4 f = J.fin[u];
+  const uint64_t *good = J.good;
   uint64_t r;
   if (type == G_PI) {
-    r = pi_words[int64_t(f.x) * RW + w];
+    r = J.pi_words[int64_t(f.x) * RW + w];
   } else {
     const uint64_t v0 = good[int64_t(f.x) * RW + w];
     const uint64_t v1 = nin > 1 ? good[int64_t(f.y) * RW + w] : 0;
@@ -156,7 +234,7 @@ __global__ void k_good_level(const uint32_t *__restrict__ info, const int4 *__re
     const uint64_t v3 = nin > 3 ? good[int64_t(f.w) * RW + w] : 0;
     r = eval_gate(type, nin, v0, v1, v2, v3);
   }
-  good[int64_t(u) * RW + w] = r;
+  J.good[int64_t(u) * RW + w] = r;
 }
 
 // Masked good values of one round, gathered per device position.
@@ -169,26 +247,20 @@ __global__ void k_gather_round(const uint64_t *__restrict__ good, int64_t RW, in
 // ---------------------------------------------------------------- sweeps
 
 struct SweepArgs {
-  const uint32_t *__restrict__ info;
-  const int4 *__restrict__ fin;
-  const uint64_t *__restrict__ good;
-  int64_t RW;
-  const int32_t *__restrict__ live_lvl;  // first live device position of each level, L+2 entries
-  int max_level;
+  const JobDev *__restrict__ jobs;
   const Tile *__restrict__ tiles;
   int ntiles;
+  int lvl_entries;   // LDS per-level counters (max level over jobs + 1)
   const Lane *__restrict__ lanes;
-  uint64_t *rows;    // per slot: [NL][64] faulty words
-  uint64_t *masks;   // per slot: [NL] lane masks (plain form)
+  uint64_t *rows;    // per slot: [NLmax][64] faulty words
+  uint64_t *masks;   // per slot: [NLmax] lane masks (plain form)
   int64_t rows_stride, masks_stride;
   uint64_t *out;     // lane results
   int *queue;        // event form: next tile
   unsigned long long *bytes;  // operand bytes touched
-  MaskRec *mrec;     // per slot: [NL] (event form)
-  uint32_t *pend;    // per slot: [NL], == epoch when the node must be evaluated
+  MaskRec *mrec;     // per slot: [NLmax] (event form)
+  uint32_t *pend;    // per slot: [NLmax], == epoch when the node must be evaluated
   int64_t rec_stride, pend_stride;
-  const int32_t *__restrict__ fo_off;  // live fanouts, ascending position
-  const int2 *__restrict__ fo;         // {position, level}
   uint32_t epoch_base;
 };
 
@@ -233,12 +305,12 @@ __device__ __forceinline__ uint64_t inject(const LaneSetup &s, uint64_t gu) {
 // position of the tile's start level: positions below it lie outside every
 // injection's fanout and hold good values in all lanes. Every node at or after
 // `base` is visited and writes its lane mask, so masks need no clearing.
-__device__ __forceinline__ void sweep_node(const SweepArgs &a, int u, int base, int lane, const LaneSetup &s,
+__device__ __forceinline__ void sweep_node(const SweepView &J, int u, int base, int lane, const LaneSetup &s,
                                            uint64_t *rows, uint64_t *masks, uint64_t &acc, uint64_t &bytes) {
   u = __builtin_amdgcn_readfirstlane(u);
-  const uint32_t inf = a.info[u];
+  const uint32_t inf = J.info[u];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-  const int4 f = a.fin[u];
+  const int4 f = J.fin[u];
   uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
   if (type != G_PI) {
     if (f.x >= base) m0 = masks[f.x];
@@ -253,15 +325,16 @@ __device__ __forceinline__ void sweep_node(const SweepArgs &a, int u, int base, 
     if (lane == 0) masks[u] = 0;
     return;
   }
-  const int64_t RW = a.RW;
-  const uint64_t gu = a.good[int64_t(u) * RW + s.w];
+  const int64_t RW = J.RW;
+  const uint64_t *good = J.good;
+  const uint64_t gu = good[int64_t(u) * RW + s.w];
   uint64_t r = gu;
   if (type != G_PI) {
-    const uint64_t v0 = ((m0 >> lane) & 1) ? rows[int64_t(f.x) * 64 + lane] : a.good[int64_t(f.x) * RW + s.w];
+    const uint64_t v0 = ((m0 >> lane) & 1) ? rows[int64_t(f.x) * 64 + lane] : good[int64_t(f.x) * RW + s.w];
     uint64_t v1 = 0, v2 = 0, v3 = 0;
-    if (nin > 1) v1 = ((m1 >> lane) & 1) ? rows[int64_t(f.y) * 64 + lane] : a.good[int64_t(f.y) * RW + s.w];
-    if (nin > 2) v2 = ((m2 >> lane) & 1) ? rows[int64_t(f.z) * 64 + lane] : a.good[int64_t(f.z) * RW + s.w];
-    if (nin > 3) v3 = ((m3 >> lane) & 1) ? rows[int64_t(f.w) * 64 + lane] : a.good[int64_t(f.w) * RW + s.w];
+    if (nin > 1) v1 = ((m1 >> lane) & 1) ? rows[int64_t(f.y) * 64 + lane] : good[int64_t(f.y) * RW + s.w];
+    if (nin > 2) v2 = ((m2 >> lane) & 1) ? rows[int64_t(f.z) * 64 + lane] : good[int64_t(f.z) * RW + s.w];
+    if (nin > 3) v3 = ((m3 >> lane) & 1) ? rows[int64_t(f.w) * 64 + lane] : good[int64_t(f.w) * RW + s.w];
     r = eval_gate(type, nin, v0, v1, v2, v3);
   }
   if (inj) r = inject(s, gu);
@@ -273,11 +346,13 @@ __device__ __forceinline__ void sweep_node(const SweepArgs &a, int u, int base, 
   bytes += uint64_t(s.active) * 8 * (type == G_PI ? 1 : nin + 1) + 8 * uint64_t(__popcll(nm));
 }
 
-// Plain form: one launch per level; a wave handles one (tile, node) pair.
+// Plain form: one launch per level; a wave handles one (tile, node) pair. All
+// tiles of the launch belong to one job.
 __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
   const int lane = threadIdx.x & 63;
   const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const int lo = a.live_lvl[level], hi = a.live_lvl[level + 1];
+  const SweepView J = sweep_view(a.jobs, a.tiles[tile0].job);
+  const int lo = J.live_lvl[level], hi = J.live_lvl[level + 1];
   const int n = hi - lo;
   if (n <= 0) return;
   const int tb = int(gw / n);
@@ -289,7 +364,7 @@ __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
   uint64_t *rows = a.rows + int64_t(tb) * a.rows_stride;
   uint64_t *masks = a.masks + int64_t(tb) * a.masks_stride;
   uint64_t acc = 0, bytes = 0;
-  sweep_node(a, u, a.live_lvl[T.start_level], lane, s, rows, masks, acc, bytes);
+  sweep_node(J, u, J.live_lvl[T.start_level], lane, s, rows, masks, acc, bytes);
   if (acc && lane < s.active) atomicOr((unsigned long long *)&a.out[s.out], (unsigned long long)acc);
   if (lane == 0) atomicAdd(a.bytes, (unsigned long long)bytes);
 }
@@ -299,11 +374,11 @@ __global__ void k_sweep_level(SweepArgs a, int level, int tile0, int nbatch) {
 // fanins sit on lower levels, so their masks are final). A MaskRec stamped
 // with another epoch means "no lane differs".
 struct NodeStage {
-  uint64_t m[4];    // fanin lane masks
+  uint64_t m[4];  // fanin lane masks
   int4 fin;
   int u;
   uint32_t info;
-  int fo0, fo1;     // live fanouts fo[fo0, fo1)
+  int fo0, fo1;   // live fanouts fo[fo0, fo1)
 };
 
 __device__ __forceinline__ uint64_t stamped_mask(const MaskRec *mrec, int f, uint32_t epoch) {
@@ -322,7 +397,7 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t x) {
 // mask record are written and its live fanouts marked pending (counted per
 // level in LDS); the deepest pending level is raised.
 template <int NB>
-__device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage *st, int cnt, int lane,
+__device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage *st, int cnt, int lane,
                                             const LaneSetup &s, uint32_t epoch, uint64_t *rows, MaskRec *mrec,
                                             uint32_t *pend, uint64_t &acc, uint64_t &bytes, int *s_maxl,
                                             int *s_lvlcnt) {
@@ -330,7 +405,8 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage 
   uint32_t type[NB], nin[NB];
   int4 f[NB];
   uint64_t m[NB][4], gu[NB], r[NB];
-  const int64_t RW = a.RW;
+  const int64_t RW = J.RW;
+  const uint64_t *good = J.good;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const NodeStage &n = st[k < cnt ? k : 0];
@@ -344,7 +420,7 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage 
     f[k].w = __builtin_amdgcn_readfirstlane(n.fin.w);
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[k][q] = uniform64(n.m[q]);
-    gu[k] = a.good[int64_t(u[k]) * RW + s.w];
+    gu[k] = good[int64_t(u[k]) * RW + s.w];
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -352,14 +428,14 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage 
     if (k < cnt && type[k] != G_PI) {
       const uint32_t n = nin[k];
       const uint64_t v0 =
-          ((m[k][0] >> lane) & 1) ? rows[int64_t(f[k].x) * 64 + lane] : a.good[int64_t(f[k].x) * RW + s.w];
+          ((m[k][0] >> lane) & 1) ? rows[int64_t(f[k].x) * 64 + lane] : good[int64_t(f[k].x) * RW + s.w];
       uint64_t v1 = 0, v2 = 0, v3 = 0;
       if (n > 1)
-        v1 = ((m[k][1] >> lane) & 1) ? rows[int64_t(f[k].y) * 64 + lane] : a.good[int64_t(f[k].y) * RW + s.w];
+        v1 = ((m[k][1] >> lane) & 1) ? rows[int64_t(f[k].y) * 64 + lane] : good[int64_t(f[k].y) * RW + s.w];
       if (n > 2)
-        v2 = ((m[k][2] >> lane) & 1) ? rows[int64_t(f[k].z) * 64 + lane] : a.good[int64_t(f[k].z) * RW + s.w];
+        v2 = ((m[k][2] >> lane) & 1) ? rows[int64_t(f[k].z) * 64 + lane] : good[int64_t(f[k].z) * RW + s.w];
       if (n > 3)
-        v3 = ((m[k][3] >> lane) & 1) ? rows[int64_t(f[k].w) * 64 + lane] : a.good[int64_t(f[k].w) * RW + s.w];
+        v3 = ((m[k][3] >> lane) & 1) ? rows[int64_t(f[k].w) * 64 + lane] : good[int64_t(f[k].w) * RW + s.w];
       r[k] = eval_gate(type[k], n, v0, v1, v2, v3);
     }
   }
@@ -381,48 +457,49 @@ __device__ __forceinline__ void event_nodes(const SweepArgs &a, const NodeStage 
       }
       const int e0 = __builtin_amdgcn_readfirstlane(st[k].fo0), e1 = __builtin_amdgcn_readfirstlane(st[k].fo1);
       for (int e = e0 + lane; e < e1; e += 64) {
-        const int2 v = a.fo[e];
+        const int2 v = J.fo[e];
         pend[v.x] = epoch;
         atomicAdd(&s_lvlcnt[v.y], 1);
       }
-      if (e1 > e0 && lane == 0) atomicMax(s_maxl, a.fo[e1 - 1].y);
+      if (e1 > e0 && lane == 0) atomicMax(s_maxl, J.fo[e1 - 1].y);
       bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
     }
     if (type[k] == G_PO) acc |= d;
   }
 }
 
-// Event form, persistent: a workgroup pulls tiles from a queue. For each tile
-// it walks the levels from the lowest injection, skipping levels with no
-// pending node (per-level counts in LDS). At a level with pending nodes, one
-// thread per node checks the pending flag; those that find one append the node
-// to the LDS stage together with its record and fanin masks; then the waves
-// evaluate the staged nodes, two per wave at a time. The walk ends after the
-// deepest level any pending node sits on. Pending flags and mask records
+// Event form, persistent: a workgroup pulls tiles (of any job) from a queue.
+// For each tile it walks the levels from the lowest injection, skipping levels
+// with no pending node (per-level counts in LDS). At a level with pending
+// nodes, one thread per node checks the pending flag; those that find one
+// append the node to the LDS stage with its record and fanin masks; then the
+// waves evaluate the staged nodes, two per wave at a time. The walk ends after
+// the deepest level any pending node sits on. Pending flags and mask records
 // carry the tile's epoch, so nothing in global memory is cleared between
 // tiles; each level's LDS count is zeroed once the level is done. Every
 // workgroup leaves once the queue passes the last tile.
 constexpr int kChunk = 512;
 constexpr int kNodesPerWave = 2;
-__global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
+__global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))) k_sweep_events(SweepArgs a) {
   __shared__ unsigned long long acc_lds[64];
   __shared__ int s_tile, s_cnt, s_maxl;
   __shared__ NodeStage s_stage[kChunk];
-  extern __shared__ int s_lvlcnt[];  // max_level + 1 entries
+  extern __shared__ int s_lvlcnt[];  // a.lvl_entries entries
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6, nwaves = blockDim.x >> 6;
   uint64_t *rows = a.rows + int64_t(blockIdx.x) * a.rows_stride;
   MaskRec *mrec = a.mrec + int64_t(blockIdx.x) * a.rec_stride;
   uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.pend_stride;
-  for (int i = tid; i <= a.max_level; i += blockDim.x) s_lvlcnt[i] = 0;
+  for (int i = tid; i < a.lvl_entries; i += blockDim.x) s_lvlcnt[i] = 0;
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(a.queue, 1);
     if (tid < 64) acc_lds[tid] = 0;
     __syncthreads();
-    const int t = s_tile;
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
     if (t >= a.ntiles) return;
     const Tile T = a.tiles[t];
+    const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
     const LaneSetup s = lane_setup(a, T, lane);
     const uint32_t epoch = a.epoch_base + uint32_t(t) + 1u;
     if (tid < 64 && s.site >= 0) {
@@ -434,7 +511,7 @@ __global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
     __syncthreads();
     for (int l = T.start_level; l <= s_maxl; ++l) {
       if (s_lvlcnt[l] == 0) continue;
-      const int lo = a.live_lvl[l], hi = a.live_lvl[l + 1];
+      const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += blockDim.x) {
         if (tid == 0) s_cnt = 0;
         __syncthreads();
@@ -448,14 +525,14 @@ __global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
           base = __shfl(base, 0);
           if (p) {
             NodeStage &n = s_stage[base + __popcll(b & ((1ull << lane) - 1ull))];
-            const uint32_t inf = a.info[u];
-            const int4 f = a.fin[u];
+            const uint32_t inf = J.info[u];
+            const int4 f = J.fin[u];
             const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
             n.u = u;
             n.info = inf;
             n.fin = f;
-            n.fo0 = a.fo_off[u];
-            n.fo1 = a.fo_off[u + 1];
+            n.fo0 = J.fo_off[u];
+            n.fo1 = J.fo_off[u + 1];
             const bool gate = type != G_PI;
             n.m[0] = gate ? stamped_mask(mrec, f.x, epoch) : 0;
             n.m[1] = gate && nin > 1 ? stamped_mask(mrec, f.y, epoch) : 0;
@@ -467,7 +544,7 @@ __global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
         __syncthreads();
         const int n = s_cnt;
         for (int i = wave * kNodesPerWave; i < n; i += nwaves * kNodesPerWave)
-          event_nodes<kNodesPerWave>(a, &s_stage[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
+          event_nodes<kNodesPerWave>(J, &s_stage[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
                                      acc, bytes, &s_maxl, s_lvlcnt);
         __syncthreads();
       }
@@ -485,49 +562,24 @@ __global__ void __launch_bounds__(kChunk) k_sweep_events(SweepArgs a) {
 
 // ---------------------------------------------------------------- detection
 
-struct DetectArgs {
-  const uint32_t *__restrict__ info;
-  const int4 *__restrict__ fin;
-  const uint64_t *__restrict__ good;
-  int64_t RW;
-  const int32_t *__restrict__ parent;     // per live position, -1 at region heads
-  const int32_t *__restrict__ head_slot;  // per live position: the stem's unit index in this step, else -1
-  const uint8_t *__restrict__ is_po;      // per live position
-  uint64_t *need;                         // [unit][Wn]: bits a unit's sweep must cover
-  const uint64_t *__restrict__ dout;      // [unit][Wn]: sweep results
-  int64_t w0;
-  int Wn;
-  const int32_t *__restrict__ flist;      // faults of this step (direct: unit k is fault flist[k])
-  int nf;
-  const int32_t *__restrict__ site;       // per fault, live position
-  const int32_t *__restrict__ head;       // per fault, region head (live position)
-  const uint8_t *__restrict__ sa;
-  int64_t num_patterns;
-  int32_t *first_round;                   // atomicMin, or null
-  uint64_t *det_bits;                     // [F][RB], or null
-  int64_t RB;
-  uint8_t *last_flag;                     // d(f, last) bytes, or null
-  int64_t last_word;
-};
-
-__device__ __forceinline__ void record(const DetectArgs &a, int f, int64_t w, uint64_t det) {
+__device__ __forceinline__ void record(const JobDev &J, int f, int64_t w, uint64_t det) {
   if (!det) return;
-  if (a.first_round) atomicMin(&a.first_round[f], int32_t(w));
-  if (a.det_bits)
-    atomicOr((unsigned long long *)&a.det_bits[int64_t(f) * a.RB + (w >> 6)], 1ull << (w & 63));
-  if (a.last_flag && w == a.last_word) a.last_flag[f] = 1;
+  if (J.first_round) atomicMin(&J.first_round[f], int32_t(w));
+  if (J.det_bits) atomicOr((unsigned long long *)&J.det_bits[int64_t(f) * J.RB + (w >> 6)], 1ull << (w & 63));
+  if (J.last_flag && w == J.last_word) J.last_flag[f] = 1;
 }
 
 // Flip sensitivity of gate g to its input node x at word w: the patterns where
 // complementing every fanin slot fed by x changes g's output.
-__device__ __forceinline__ uint64_t sens(const DetectArgs &a, int g, int x, int64_t w) {
-  const uint32_t inf = a.info[g];
+__device__ __forceinline__ uint64_t sens(const JobDev &J, int g, int x, int64_t w) {
+  const uint32_t inf = J.info[g];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-  const int4 f = a.fin[g];
-  const uint64_t v0 = a.good[int64_t(f.x) * a.RW + w];
-  const uint64_t v1 = nin > 1 ? a.good[int64_t(f.y) * a.RW + w] : 0;
-  const uint64_t v2 = nin > 2 ? a.good[int64_t(f.z) * a.RW + w] : 0;
-  const uint64_t v3 = nin > 3 ? a.good[int64_t(f.w) * a.RW + w] : 0;
+  const int4 f = J.fin[g];
+  const uint64_t *good = J.good;
+  const uint64_t v0 = good[int64_t(f.x) * J.RW + w];
+  const uint64_t v1 = nin > 1 ? good[int64_t(f.y) * J.RW + w] : 0;
+  const uint64_t v2 = nin > 2 ? good[int64_t(f.z) * J.RW + w] : 0;
+  const uint64_t v3 = nin > 3 ? good[int64_t(f.w) * J.RW + w] : 0;
   const uint64_t r0 = eval_gate(type, nin, v0, v1, v2, v3);
   const uint64_t r1 = eval_gate(type, nin, f.x == x ? ~v0 : v0, f.y == x ? ~v1 : v1, f.z == x ? ~v2 : v2,
                                 f.w == x ? ~v3 : v3);
@@ -535,72 +587,77 @@ __device__ __forceinline__ uint64_t sens(const DetectArgs &a, int g, int x, int6
 }
 
 // L = activation & path sensitisation to the region head & live bits.
-__device__ __forceinline__ uint64_t local_term(const DetectArgs &a, int f, int64_t w) {
-  const int x0 = a.site[f];
-  uint64_t D = (a.good[int64_t(x0) * a.RW + w] ^ (a.sa[f] ? ~uint64_t(0) : uint64_t(0))) &
-               round_mask(a.num_patterns, w);
-  for (int x = x0, g = a.parent[x]; g >= 0 && D; x = g, g = a.parent[x]) D &= sens(a, g, x, w);
+__device__ __forceinline__ uint64_t local_term(const JobDev &J, int f, int64_t w) {
+  const int x0 = J.site[f];
+  uint64_t D = (J.good[int64_t(x0) * J.RW + w] ^ (J.sa[f] ? ~uint64_t(0) : uint64_t(0))) &
+               round_mask(J.num_patterns, w);
+  for (int x = x0, g = J.parent[x]; g >= 0 && D; x = g, g = J.parent[x]) D &= sens(J, g, x, w);
   return D;
 }
 
-// Stem tracing, phase 1: thread = (fault, word). A fault whose region ends at
-// a PO is decided here; otherwise its bits are added to its stem's need.
-__global__ void k_local_cpt(DetectArgs a) {
+struct ItemArgs {
+  const JobDev *__restrict__ jobs;
+  int njobs;
+  const int64_t *__restrict__ item_off;  // njobs + 1
+  uint64_t *need;                         // [word] bits a unit's sweep must cover
+  const uint64_t *__restrict__ dout;      // [word] sweep results
+};
+
+// Phase 1, thread = (job, fault, word). Stem tracing: a fault whose region
+// ends at a PO is decided here, otherwise its bits join its stem's need.
+// Direct injection: a word needs a lane iff the stuck value differs from the
+// site's good value somewhere in the full word.
+__global__ void k_local(ItemArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(a.nf) * a.Wn) return;
-  const int fi = int(t / a.Wn);
-  const int j = int(t - int64_t(fi) * a.Wn);
-  const int f = a.flist[fi];
-  const int64_t w = a.w0 + j;
-  const uint64_t L = local_term(a, f, w);
+  if (t >= a.item_off[a.njobs]) return;
+  const int jb = find_job(a.item_off, a.njobs, t);
+  const JobDev &J = a.jobs[jb];
+  const int64_t lt = t - a.item_off[jb];
+  const int fi = int(lt / J.Wn);
+  const int j = int(lt - int64_t(fi) * J.Wn);
+  const int f = J.flist[fi];
+  const int64_t w = J.w0 + j;
+  if (J.direct) {
+    a.need[J.word0 + lt] = J.good[int64_t(J.site[f]) * J.RW + w] ^ (J.sa[f] ? ~uint64_t(0) : uint64_t(0));
+    return;
+  }
+  const uint64_t L = local_term(J, f, w);
   if (!L) return;
-  const int h = a.head[f];
-  if (a.is_po[h]) {
-    record(a, f, w, L);
+  const int h = J.head[f];
+  if (J.is_po[h]) {
+    record(J, f, w, L);
   } else {
-    atomicOr((unsigned long long *)&a.need[int64_t(a.head_slot[h]) * a.Wn + j], (unsigned long long)L);
+    atomicOr((unsigned long long *)&a.need[J.word0 + int64_t(J.head_slot[h]) * J.Wn + j], (unsigned long long)L);
   }
 }
 
-// Stem tracing, phase 2: d = L & D_stem.
-__global__ void k_detect_cpt(DetectArgs a) {
+// Phase 2: stem tracing d = L & D_stem; direct injection d = PO difference.
+__global__ void k_detect(ItemArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(a.nf) * a.Wn) return;
-  const int fi = int(t / a.Wn);
-  const int j = int(t - int64_t(fi) * a.Wn);
-  const int f = a.flist[fi];
-  const int h = a.head[f];
-  if (a.is_po[h]) return;
-  const uint64_t D = a.dout[int64_t(a.head_slot[h]) * a.Wn + j];
+  if (t >= a.item_off[a.njobs]) return;
+  const int jb = find_job(a.item_off, a.njobs, t);
+  const JobDev &J = a.jobs[jb];
+  const int64_t lt = t - a.item_off[jb];
+  const int fi = int(lt / J.Wn);
+  const int j = int(lt - int64_t(fi) * J.Wn);
+  const int f = J.flist[fi];
+  const int64_t w = J.w0 + j;
+  if (J.direct) {
+    record(J, f, w, a.dout[J.word0 + lt] & round_mask(J.num_patterns, w));
+    return;
+  }
+  const int h = J.head[f];
+  if (J.is_po[h]) return;
+  const uint64_t D = a.dout[J.word0 + int64_t(J.head_slot[h]) * J.Wn + j];
   if (!D) return;
-  const int64_t w = a.w0 + j;
-  record(a, f, w, local_term(a, f, w) & D);
+  record(J, f, w, local_term(J, f, w) & D);
 }
 
-// Direct injection, phase 1: a word needs a lane iff the stuck value differs
-// from the site's good value somewhere in the full word.
-__global__ void k_local_direct(DetectArgs a) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(a.nf) * a.Wn) return;
-  const int k = int(t / a.Wn);
-  const int j = int(t - int64_t(k) * a.Wn);
-  const int f = a.flist[k];
-  a.need[t] = a.good[int64_t(a.site[f]) * a.RW + a.w0 + j] ^ (a.sa[f] ? ~uint64_t(0) : uint64_t(0));
-}
-
-// Direct injection, phase 2: the sweep result is the PO difference.
-__global__ void k_detect_direct(DetectArgs a) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= int64_t(a.nf) * a.Wn) return;
-  const int k = int(t / a.Wn);
-  const int j = int(t - int64_t(k) * a.Wn);
-  const int64_t w = a.w0 + j;
-  record(a, a.flist[k], w, a.dout[t] & round_mask(a.num_patterns, w));
-}
-
-__global__ void k_set_slots(int32_t *head_slot, const int32_t *units, int n, int clear) {
+// units[i] = {job, stem position, unit index}: head_slot of that job maps the
+// stem to its unit index within the job (or back to -1).
+__global__ void k_set_slots(const JobDev *__restrict__ jobs, const int4 *__restrict__ units, int n, int clear) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) head_slot[units[i]] = clear ? -1 : i;
+  if (i < n) jobs[units[i].x].head_slot[units[i].y] = clear ? -1 : units[i].z;
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -624,6 +681,10 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  size_t bytes() const { return n * sizeof(T); }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
 };
 
@@ -631,43 +692,76 @@ inline unsigned blocks_for(int64_t threads, int bs) { return unsigned((threads +
 
 }  // namespace
 
+// ---------------------------------------------------------------- host state
+
+struct CircuitRes {
+  int64_t N = 0, NL = 0, P = 0, G = 0, O = 0, E = 0;
+  int L = 0;
+  bool has_mux = false;
+  std::vector<int32_t> pos2dp, dp2pos, level_dp, live_lvl, dead_lvl;
+  std::vector<int32_t> head_dp;     // per position: region head (device position), -1 if dead
+  std::vector<uint8_t> head_is_po;  // per live device position
+  DevBuf<uint32_t> info;
+  DevBuf<int4> fin;
+  DevBuf<int32_t> parent, d_live_lvl, d_dead_lvl, fo_off;
+  DevBuf<int2> fo;
+  DevBuf<uint8_t> is_po;
+  int64_t bytes() const {
+    return int64_t(info.bytes() + fin.bytes() + parent.bytes() + d_live_lvl.bytes() + d_dead_lvl.bytes() +
+                   fo_off.bytes() + fo.bytes() + is_po.bytes());
+  }
+};
+
+struct JobState {
+  CircuitRes *c = nullptr;
+  JobOptions opt;
+  bool direct = false;
+  int64_t F = 0, R = 0, num_patterns = 0, RB = 0;
+  std::vector<int32_t> site, head;  // per fault: device position, region head (-1: dead site)
+  std::vector<uint8_t> sa;
+  std::vector<int32_t> live, alive;
+  DevBuf<uint64_t> good, pi_words, detbits;
+  DevBuf<int32_t> d_site, d_head, first, head_slot, flist;
+  DevBuf<uint8_t> d_sa, last;
+  bool started = false, done = false, last_done = false;
+  int64_t w0 = 0;
+  int Wt = 1;
+  // current step
+  enum Kind { MATRIX, DROP, LAST } kind = MATRIX;
+  std::vector<int32_t> step_faults, units;
+  int64_t step_w0 = 0;
+  int step_Wn = 0;
+  RunResults res;
+};
+
 struct Engine::Impl {
   int device = 0;
   hipStream_t stream = nullptr;
   int cus = 0;
-  // host view
-  int64_t N = 0, NL = 0, P = 0, G = 0, O = 0, R = 0, F = 0, E = 0, num_patterns = 0;
-  int L = 0;
-  bool has_mux = false;
-  std::vector<int32_t> pos2dp, dp2pos, level_dp, live_lvl, dead_lvl;
-  std::vector<int32_t> f_site, f_head;  // device positions; head -1 = dead site
-  std::vector<uint8_t> f_sa, head_is_po;
-  int64_t live_faults = 0;
-  // device: circuit
-  DevBuf<uint32_t> d_info;
-  DevBuf<int4> d_fin;
-  DevBuf<int32_t> d_parent, d_live_lvl, d_site, d_head, d_head_slot, d_fo_off;
-  DevBuf<uint8_t> d_sa, d_is_po;
-  DevBuf<int2> d_fo;
-  DevBuf<uint64_t> d_pi_words, d_good;
-  // device: per step
-  DevBuf<int32_t> d_flist, d_units, d_first;
-  DevBuf<uint8_t> d_last;
+  std::vector<std::unique_ptr<CircuitRes>> circuits;
+  std::vector<std::unique_ptr<JobState>> jobs;
+  // per iteration
+  DevBuf<JobDev> d_jobs;
+  DevBuf<int64_t> d_item_off, d_good_off;
+  DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
-  DevBuf<uint64_t> d_need, d_dout, d_detbits, d_gather;
+  DevBuf<int4> d_units;
   DevBuf<int> d_queue;
   DevBuf<unsigned long long> d_bytes;
-  // device: sweep slots
+  std::vector<uint8_t> head_mark;
+  std::vector<int32_t> count_buf;
+  // sweep slots
   DevBuf<uint64_t> d_rows, d_masks;
   DevBuf<MaskRec> d_mrec;
   DevBuf<uint32_t> d_pend;
-  int64_t slots = 0;
+  int64_t slots = 0, slot_nl = 0;
   uint32_t epoch_base = 0;
-  std::vector<uint8_t> head_mark;
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   ~Impl() {
+    jobs.clear();
+    circuits.clear();
     if (ev[0]) (void)hipEventDestroy(ev[0]);
     if (ev[1]) (void)hipEventDestroy(ev[1]);
     if (stream) (void)hipStreamDestroy(stream);
@@ -680,11 +774,19 @@ struct Engine::Impl {
     LSIM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     return ms;
   }
-
-  void step(const RunOptions &opt, const std::vector<int32_t> &faults, int64_t w0, int Wn, bool direct,
-            int32_t *first_dev, uint64_t *detbits_dev, int64_t RB, uint8_t *last_dev, RunResults &res);
-  void sweep(const RunOptions &opt, const std::vector<Lane> &lanes, std::vector<Tile> &tiles, RunResults &res);
-  void ensure_slots(const RunOptions &opt, int64_t want);
+  void h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+  }
+  void d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  }
+  JobDev job_dev(JobState &j) const;
+  bool plan(JobState &j);
+  void finalize(JobState &j);
+  void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
+  void sweep(const EngineOptions &eo, const std::vector<Lane> &lanes, const std::vector<Tile> &tiles,
+             int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st);
+  void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max);
 };
 
 Engine::Engine(int device) : impl_(new Impl) {
@@ -704,6 +806,8 @@ Engine::Engine(int device) : impl_(new Impl) {
     LSIM_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
     LSIM_HIP(hipEventCreate(&impl_->ev[0]));
     LSIM_HIP(hipEventCreate(&impl_->ev[1]));
+    impl_->d_bytes.alloc(1);
+    LSIM_HIP(hipMemsetAsync(impl_->d_bytes.p, 0, sizeof(unsigned long long), impl_->stream));
   } catch (...) {
     delete impl_;
     impl_ = nullptr;
@@ -720,467 +824,652 @@ Engine::~Engine() {
 
 int Engine::device() const { return impl_->device; }
 
-void Engine::load(const Netlist &nl, const FaultStructure &fs) {
+int64_t Engine::free_bytes() const {
+  size_t fr = 0, tot = 0;
+  LSIM_HIP(hipSetDevice(impl_->device));
+  LSIM_HIP(hipMemGetInfo(&fr, &tot));
+  return int64_t(fr);
+}
+
+CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  m.N = nl.N;
-  m.P = nl.P;
-  m.G = nl.G;
-  m.O = nl.O;
-  m.E = nl.E();
-  m.R = nl.R;
-  m.F = int64_t(nl.fault_site.size());
-  m.num_patterns = nl.num_patterns;
-  m.L = nl.max_level;
-  m.has_mux = nl.has_mux;
-  const int64_t N = m.N;
+  auto cp = std::make_unique<CircuitRes>();
+  CircuitRes &c = *cp;
+  c.N = nl.N;
+  c.P = nl.P;
+  c.G = nl.G;
+  c.O = nl.O;
+  c.E = nl.E();
+  c.L = nl.max_level;
+  c.has_mux = nl.has_mux;
+  const int64_t N = c.N;
   // Device order: live nodes (a PO is reachable) first, dead nodes after, each
   // in topological order so every level is a contiguous range in both parts.
-  m.pos2dp.assign(N, 0);
-  m.dp2pos.assign(N, 0);
+  c.pos2dp.assign(N, 0);
+  c.dp2pos.assign(N, 0);
   int64_t nl_count = 0;
   for (int64_t p = 0; p < N; ++p)
-    if (fs.reach_po[p]) m.pos2dp[p] = int32_t(nl_count++);
-  m.NL = nl_count;
+    if (fs.reach_po[p]) c.pos2dp[p] = int32_t(nl_count++);
+  c.NL = nl_count;
   int64_t nd = nl_count;
   for (int64_t p = 0; p < N; ++p)
-    if (!fs.reach_po[p]) m.pos2dp[p] = int32_t(nd++);
-  for (int64_t p = 0; p < N; ++p) m.dp2pos[m.pos2dp[p]] = int32_t(p);
-  m.level_dp.assign(N, 0);
-  for (int64_t p = 0; p < N; ++p) m.level_dp[m.pos2dp[p]] = nl.level[p];
-  m.live_lvl.assign(m.L + 2, 0);
-  m.dead_lvl.assign(m.L + 2, 0);
+    if (!fs.reach_po[p]) c.pos2dp[p] = int32_t(nd++);
+  for (int64_t p = 0; p < N; ++p) c.dp2pos[c.pos2dp[p]] = int32_t(p);
+  c.level_dp.assign(N, 0);
+  for (int64_t p = 0; p < N; ++p) c.level_dp[c.pos2dp[p]] = nl.level[p];
+  c.live_lvl.assign(c.L + 2, 0);
+  c.dead_lvl.assign(c.L + 2, 0);
   {
-    std::vector<int64_t> cl(m.L + 2, 0), cd(m.L + 2, 0);
+    std::vector<int64_t> cl(c.L + 2, 0), cd(c.L + 2, 0);
     for (int64_t p = 0; p < N; ++p) (fs.reach_po[p] ? cl : cd)[nl.level[p] + 1]++;
-    m.live_lvl[0] = 0;
-    m.dead_lvl[0] = int32_t(m.NL);
-    for (int l = 0; l <= m.L; ++l) {
-      m.live_lvl[l + 1] = int32_t(m.live_lvl[l] + cl[l + 1]);
-      m.dead_lvl[l + 1] = int32_t(m.dead_lvl[l] + cd[l + 1]);
+    c.live_lvl[0] = 0;
+    c.dead_lvl[0] = int32_t(c.NL);
+    for (int l = 0; l <= c.L; ++l) {
+      c.live_lvl[l + 1] = int32_t(c.live_lvl[l] + cl[l + 1]);
+      c.dead_lvl[l + 1] = int32_t(c.dead_lvl[l] + cd[l + 1]);
     }
   }
-  const int64_t nl1 = std::max<int64_t>(m.NL, 1);
+  const int64_t nl1 = std::max<int64_t>(c.NL, 1);
   std::vector<uint32_t> info(N);
   std::vector<int4> fin(N);
   std::vector<int32_t> parent(nl1, -1);
   for (int64_t p = 0; p < N; ++p) {
-    const int32_t u = m.pos2dp[p];
+    const int32_t u = c.pos2dp[p];
     const int nin = nl.fin_off[p + 1] - nl.fin_off[p];
     info[u] = uint32_t(nl.type[p]) | (uint32_t(nin) << 4);
     int v[4] = {0, 0, 0, 0};
     if (nl.type[p] == G_PI) v[0] = int(p);  // PI index == position
-    for (int k = 0; k < nin; ++k) v[k] = m.pos2dp[nl.fin[nl.fin_off[p] + k]];
+    for (int k = 0; k < nin; ++k) v[k] = c.pos2dp[nl.fin[nl.fin_off[p] + k]];
     fin[u] = make_int4(v[0], v[1], v[2], v[3]);
-    if (fs.reach_po[p] && fs.parent[p] >= 0) parent[u] = m.pos2dp[fs.parent[p]];
+    if (fs.reach_po[p] && fs.parent[p] >= 0) parent[u] = c.pos2dp[fs.parent[p]];
   }
   // Live fanouts of live nodes, ascending position, with the fanout's level.
   std::vector<int32_t> fo_off(nl1 + 1, 0);
   std::vector<int2> fo;
-  for (int64_t u = 0; u < m.NL; ++u) {
-    const int32_t p = m.dp2pos[u];
-    std::vector<int32_t> outs;
+  std::vector<int32_t> outs;
+  for (int64_t u = 0; u < c.NL; ++u) {
+    const int32_t p = c.dp2pos[u];
+    outs.clear();
     for (int32_t e = fs.fo_off[p]; e < fs.fo_off[p + 1]; ++e)
-      if (fs.reach_po[fs.fo[e]]) outs.push_back(m.pos2dp[fs.fo[e]]);
+      if (fs.reach_po[fs.fo[e]]) outs.push_back(c.pos2dp[fs.fo[e]]);
     std::sort(outs.begin(), outs.end());
-    for (int32_t v : outs) fo.push_back(make_int2(v, m.level_dp[v]));
+    for (int32_t v : outs) fo.push_back(make_int2(v, c.level_dp[v]));
     fo_off[u + 1] = int32_t(fo.size());
   }
   if (fo.empty()) fo.push_back(make_int2(0, 0));
-  // Faults: site, and the head of the site's fanout-free region.
-  m.f_site.resize(m.F);
-  m.f_head.resize(m.F);
-  m.f_sa.resize(m.F);
-  m.head_is_po.assign(nl1, 0);
+  c.head_is_po.assign(nl1, 0);
   for (int64_t p = 0; p < N; ++p)
-    if (nl.type[p] == G_PO) m.head_is_po[m.pos2dp[p]] = 1;
-  m.live_faults = 0;
-  for (int64_t f = 0; f < m.F; ++f) {
-    const int32_t p = nl.fault_site[f];
-    m.f_site[f] = m.pos2dp[p];
-    m.f_sa[f] = nl.fault_sa[f];
-    m.f_head[f] = fs.reach_po[p] ? m.pos2dp[fs.stem_of[p]] : -1;
-    m.live_faults += fs.reach_po[p] ? 1 : 0;
-  }
-  // PI words transposed to [PI][round] for coalesced level-0 reads.
-  std::vector<uint64_t> piw(std::max<int64_t>(m.P * m.R, 1));
-  for (int64_t r = 0; r < m.R; ++r)
-    for (int64_t i = 0; i < m.P; ++i) piw[i * m.R + r] = nl.patterns[r * m.P + i];
+    if (nl.type[p] == G_PO) c.head_is_po[c.pos2dp[p]] = 1;
+  c.head_dp.assign(N, -1);
+  for (int64_t p = 0; p < N; ++p)
+    if (fs.reach_po[p]) c.head_dp[p] = c.pos2dp[fs.stem_of[p]];
 
-  const int64_t F1 = std::max<int64_t>(m.F, 1);
-  m.d_info.alloc(N);
-  m.d_fin.alloc(N);
-  m.d_parent.alloc(nl1);
-  m.d_live_lvl.alloc(m.live_lvl.size());
-  m.d_site.alloc(F1);
-  m.d_head.alloc(F1);
-  m.d_sa.alloc(F1);
-  m.d_is_po.alloc(nl1);
-  m.d_head_slot.alloc(nl1);
-  m.d_fo_off.alloc(fo_off.size());
-  m.d_fo.alloc(fo.size());
-  m.d_pi_words.alloc(piw.size());
-  m.d_good.alloc(size_t(std::max<int64_t>(N * m.R, 1)));
-  auto h2d = [&](void *dst, const void *src, size_t bytes) {
-    if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m.stream));
-  };
-  h2d(m.d_info.p, info.data(), N * sizeof(uint32_t));
-  h2d(m.d_fin.p, fin.data(), N * sizeof(int4));
-  h2d(m.d_parent.p, parent.data(), parent.size() * sizeof(int32_t));
-  h2d(m.d_live_lvl.p, m.live_lvl.data(), m.live_lvl.size() * sizeof(int32_t));
-  h2d(m.d_site.p, m.f_site.data(), m.F * sizeof(int32_t));
-  h2d(m.d_head.p, m.f_head.data(), m.F * sizeof(int32_t));
-  h2d(m.d_sa.p, m.f_sa.data(), m.F * sizeof(uint8_t));
-  h2d(m.d_is_po.p, m.head_is_po.data(), m.head_is_po.size());
-  h2d(m.d_fo_off.p, fo_off.data(), fo_off.size() * sizeof(int32_t));
-  h2d(m.d_fo.p, fo.data(), fo.size() * sizeof(int2));
-  h2d(m.d_pi_words.p, piw.data(), piw.size() * sizeof(uint64_t));
-  k_fill_i32<<<blocks_for(nl1, 256), 256, 0, m.stream>>>(m.d_head_slot.p, nl1, -1);
-  LSIM_HIP(hipGetLastError());
+  c.info.alloc(N);
+  c.fin.alloc(N);
+  c.parent.alloc(nl1);
+  c.d_live_lvl.alloc(c.live_lvl.size());
+  c.d_dead_lvl.alloc(c.dead_lvl.size());
+  c.fo_off.alloc(fo_off.size());
+  c.fo.alloc(fo.size());
+  c.is_po.alloc(nl1);
+  m.h2d(c.info.p, info.data(), N * sizeof(uint32_t));
+  m.h2d(c.fin.p, fin.data(), N * sizeof(int4));
+  m.h2d(c.parent.p, parent.data(), parent.size() * sizeof(int32_t));
+  m.h2d(c.d_live_lvl.p, c.live_lvl.data(), c.live_lvl.size() * sizeof(int32_t));
+  m.h2d(c.d_dead_lvl.p, c.dead_lvl.data(), c.dead_lvl.size() * sizeof(int32_t));
+  m.h2d(c.fo_off.p, fo_off.data(), fo_off.size() * sizeof(int32_t));
+  m.h2d(c.fo.p, fo.data(), fo.size() * sizeof(int2));
+  m.h2d(c.is_po.p, c.head_is_po.data(), c.head_is_po.size());
   LSIM_HIP(hipStreamSynchronize(m.stream));
+  m.circuits.push_back(std::move(cp));
+  return m.circuits.back().get();
 }
 
-void Engine::Impl::ensure_slots(const RunOptions &opt, int64_t want) {
+void Engine::remove_circuit(CircuitRes *c) {
+  Impl &m = *impl_;
+  LSIM_HIP(hipSetDevice(m.device));
+  LSIM_HIP(hipStreamSynchronize(m.stream));
+  for (auto it = m.circuits.begin(); it != m.circuits.end(); ++it)
+    if (it->get() == c) {
+      m.circuits.erase(it);
+      return;
+    }
+}
+
+int64_t Engine::circuit_bytes(const CircuitRes *c) const { return c->bytes(); }
+
+int64_t Engine::job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop) {
+  const int64_t RB = (R + 63) / 64;
+  return 8 * c->N * R + 8 * c->P * R + 4 * std::max<int64_t>(c->NL, 1) + faults * (4 + 4 + 1 + 4 + 4 + 1) +
+         (drop ? 0 : 8 * faults * RB);
+}
+
+JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::vector<uint8_t> fault_sa,
+                          int64_t num_patterns, int64_t R, const std::vector<uint64_t> &patterns,
+                          const JobOptions &opt) {
+  Impl &m = *impl_;
+  LSIM_HIP(hipSetDevice(m.device));
+  auto jp = std::make_unique<JobState>();
+  JobState &j = *jp;
+  j.c = c;
+  j.opt = opt;
+  j.direct = opt.direct || c->has_mux;
+  j.F = int64_t(fault_site.size());
+  j.R = R;
+  j.num_patterns = num_patterns;
+  j.RB = (R + 63) / 64;
+  j.site.resize(j.F);
+  j.head.resize(j.F);
+  j.sa = std::move(fault_sa);
+  for (int64_t f = 0; f < j.F; ++f) {
+    const int32_t p = fault_site[f];
+    j.site[f] = c->pos2dp[p];
+    j.head[f] = c->head_dp[p];
+    if (j.head[f] >= 0) j.live.push_back(int32_t(f));
+  }
+  // PI words transposed to [PI][round] for coalesced level-0 reads.
+  std::vector<uint64_t> piw(std::max<int64_t>(c->P * R, 1));
+  for (int64_t r = 0; r < R; ++r)
+    for (int64_t i = 0; i < c->P; ++i) piw[i * R + r] = patterns[r * c->P + i];
+  const int64_t F1 = std::max<int64_t>(j.F, 1);
+  j.good.alloc(size_t(std::max<int64_t>(c->N * R, 1)));
+  j.pi_words.alloc(piw.size());
+  j.d_site.alloc(F1);
+  j.d_head.alloc(F1);
+  j.d_sa.alloc(F1);
+  j.first.alloc(F1);
+  j.flist.alloc(F1);
+  j.head_slot.alloc(std::max<int64_t>(c->NL, 1));
+  if (!opt.drop) j.detbits.alloc(size_t(std::max<int64_t>(j.F * j.RB, 1)));
+  if (opt.drop && opt.report_last_round) j.last.alloc(F1);
+  m.h2d(j.pi_words.p, piw.data(), piw.size() * sizeof(uint64_t));
+  m.h2d(j.d_site.p, j.site.data(), j.F * sizeof(int32_t));
+  m.h2d(j.d_head.p, j.head.data(), j.F * sizeof(int32_t));
+  m.h2d(j.d_sa.p, j.sa.data(), j.F);
+  k_fill_i32<<<blocks_for(F1, 256), 256, 0, m.stream>>>(j.first.p, F1, INT32_MAX);
+  k_fill_i32<<<blocks_for(int64_t(j.head_slot.n), 256), 256, 0, m.stream>>>(j.head_slot.p, int64_t(j.head_slot.n),
+                                                                             -1);
+  LSIM_HIP(hipGetLastError());
+  if (j.detbits.p) LSIM_HIP(hipMemsetAsync(j.detbits.p, 0, j.detbits.bytes(), m.stream));
+  if (j.last.p) LSIM_HIP(hipMemsetAsync(j.last.p, 0, j.last.bytes(), m.stream));
+  j.alive = j.live;
+  j.res.stats.live_nodes = c->NL;
+  j.res.stats.live_faults = int64_t(j.live.size());
+  m.jobs.push_back(std::move(jp));
+  return m.jobs.back().get();
+}
+
+void Engine::release_job(JobState *j) {
+  Impl &m = *impl_;
+  LSIM_HIP(hipSetDevice(m.device));
+  LSIM_HIP(hipStreamSynchronize(m.stream));
+  for (auto it = m.jobs.begin(); it != m.jobs.end(); ++it)
+    if (it->get() == j) {
+      m.jobs.erase(it);
+      return;
+    }
+}
+
+bool Engine::done(const JobState *j) const { return j->done; }
+RunResults &Engine::results(JobState *j) { return j->res; }
+
+JobDev Engine::Impl::job_dev(JobState &j) const {
+  const CircuitRes &c = *j.c;
+  JobDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.info = c.info.p;
+  d.fin = c.fin.p;
+  d.live_lvl = c.d_live_lvl.p;
+  d.dead_lvl = c.d_dead_lvl.p;
+  d.fo_off = c.fo_off.p;
+  d.fo = c.fo.p;
+  d.parent = c.parent.p;
+  d.is_po = c.is_po.p;
+  d.max_level = c.L;
+  d.NL = int(c.NL);
+  d.good = j.good.p;
+  d.pi_words = j.pi_words.p;
+  d.RW = j.R;
+  d.num_patterns = j.num_patterns;
+  d.site = j.d_site.p;
+  d.head = j.d_head.p;
+  d.sa = j.d_sa.p;
+  d.head_slot = j.head_slot.p;
+  d.RB = j.RB;
+  d.last_word = j.R - 1;
+  d.direct = j.direct ? 1 : 0;
+  return d;
+}
+
+void Engine::start(const std::vector<JobState *> &jobs) {
+  Impl &m = *impl_;
+  LSIM_HIP(hipSetDevice(m.device));
+  std::vector<JobState *> js;
+  for (JobState *j : jobs)
+    if (!j->started) js.push_back(j);
+  if (js.empty()) return;
+  // One launch per level index over all jobs' (node, word) pairs of that level.
+  const int nj = int(js.size());
+  int Lmax = 0;
+  for (JobState *j : js) Lmax = std::max(Lmax, j->R > 0 ? j->c->L : 0);
+  std::vector<JobDev> devs(nj);
+  for (int i = 0; i < nj; ++i) devs[i] = m.job_dev(*js[i]);
+  std::vector<int64_t> off(size_t(Lmax + 1) * (nj + 1), 0);
+  for (int l = 0; l <= Lmax; ++l) {
+    int64_t *o = &off[size_t(l) * (nj + 1)];
+    for (int i = 0; i < nj; ++i) {
+      const CircuitRes &c = *js[i]->c;
+      int64_t items = 0;
+      if (js[i]->R > 0 && l <= c.L)
+        items = int64_t(c.live_lvl[l + 1] - c.live_lvl[l] + c.dead_lvl[l + 1] - c.dead_lvl[l]) * js[i]->R;
+      o[i + 1] = o[i] + items;
+    }
+  }
+  m.d_jobs.alloc(nj);
+  m.d_good_off.alloc(off.size());
+  m.h2d(m.d_jobs.p, devs.data(), nj * sizeof(JobDev));
+  m.h2d(m.d_good_off.p, off.data(), off.size() * sizeof(int64_t));
+  m.time_begin();
+  for (int l = 0; l <= Lmax; ++l) {
+    const int64_t total = off[size_t(l) * (nj + 1) + nj];
+    if (!total) continue;
+    k_good_level<<<blocks_for(total, 256), 256, 0, m.stream>>>(m.d_jobs.p, nj, m.d_good_off.p + size_t(l) * (nj + 1),
+                                                              l);
+    LSIM_HIP(hipGetLastError());
+  }
+  const double ms = m.time_end();
+  for (JobState *j : js) {
+    j->res.stats.t_good_ms += ms;
+    // One word per node written, one per fanin and per PI pattern read.
+    j->res.stats.good_bytes += 8 * (j->c->N + j->c->E + j->c->P) * j->R;
+    j->started = true;
+    if (j->opt.keep_good_last) {
+      const CircuitRes &c = *j->c;
+      j->res.good_last.assign(c.N, 0);
+      if (j->R > 0) {
+        const int64_t bits = (j->num_patterns / (64 * j->R)) ? 64 : (j->num_patterns % 64);
+        const uint64_t mask = bits < 64 ? ((uint64_t(1) << bits) - 1) : ~uint64_t(0);
+        m.d_gather.alloc(c.N);
+        k_gather_round<<<blocks_for(c.N, 256), 256, 0, m.stream>>>(j->good.p, j->R, j->R - 1, int(c.N), mask,
+                                                                  m.d_gather.p);
+        LSIM_HIP(hipGetLastError());
+        std::vector<uint64_t> byd(c.N);
+        m.d2h(byd.data(), m.d_gather.p, c.N * sizeof(uint64_t));
+        LSIM_HIP(hipStreamSynchronize(m.stream));
+        for (int64_t p = 0; p < c.N; ++p) j->res.good_last[p] = byd[c.pos2dp[p]];
+      }
+    }
+  }
+}
+
+// Decides the job's next step; false (and the job finalised) when none is left.
+bool Engine::Impl::plan(JobState &j) {
+  if (j.done) return false;
+  if (j.R == 0 || j.live.empty()) {
+    finalize(j);
+    return false;
+  }
+  if (!j.opt.drop) {
+    if (j.w0 >= j.R) {
+      finalize(j);
+      return false;
+    }
+    j.kind = JobState::MATRIX;
+    j.step_faults = j.live;
+    j.step_w0 = j.w0;
+    j.step_Wn = int(std::min<int64_t>(64, j.R - j.w0));
+    return true;
+  }
+  if (j.w0 < j.R && !j.alive.empty()) {
+    // Words per step grow geometrically: faults left after the first rounds
+    // are the hard ones and are dropped less often.
+    j.kind = JobState::DROP;
+    j.step_faults = j.alive;
+    j.step_w0 = j.w0;
+    j.step_Wn = int(std::min<int64_t>(j.Wt, j.R - j.w0));
+    return true;
+  }
+  if (j.opt.report_last_round && !j.last_done) {
+    j.kind = JobState::LAST;
+    j.step_faults = j.live;
+    j.step_w0 = j.R - 1;
+    j.step_Wn = 1;
+    return true;
+  }
+  finalize(j);
+  return false;
+}
+
+void Engine::Impl::finalize(JobState &j) {
+  RunResults &res = j.res;
+  const int64_t F = j.F, R = j.R;
+  res.first_round.assign(F, INT32_MAX);
+  if (F) d2h(res.first_round.data(), j.first.p, F * sizeof(int32_t));
+  res.RB = j.RB;
+  if (!j.opt.drop) {
+    res.det_bits.assign(size_t(F * j.RB), 0);
+    if (F && j.detbits.p) d2h(res.det_bits.data(), j.detbits.p, size_t(F * j.RB) * sizeof(uint64_t));
+  }
+  res.last_round.assign(F, 0);
+  if (j.opt.drop && j.last.p && j.last_done && F) d2h(res.last_round.data(), j.last.p, F);
+  LSIM_HIP(hipStreamSynchronize(stream));
+  if (!j.opt.drop) {
+    res.has_matrix = true;
+    res.has_last = true;
+    if (R > 0)
+      for (int64_t f = 0; f < F; ++f)
+        res.last_round[f] = uint8_t((res.det_bits[f * j.RB + ((R - 1) >> 6)] >> ((R - 1) & 63)) & 1);
+  } else {
+    res.has_matrix = false;
+    res.has_last = j.opt.report_last_round;
+  }
+  for (auto &v : res.first_round)
+    if (v == INT32_MAX) v = -1;
+  res.stats.slots = slots;
+  j.done = true;
+}
+
+void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max) {
   // Per slot: faulty rows [NL][64], and the plain form's lane masks or the
   // event form's mask records and pending flags.
-  const int64_t per_slot = NL * (64 * 8 + 8 + int64_t(sizeof(MaskRec)) + 4);
+  const int64_t nl = std::max<int64_t>(nl_max, 1);
+  const int64_t per_slot = nl * (64 * 8 + 8 + int64_t(sizeof(MaskRec)) + 4);
+  const int64_t held = int64_t(d_rows.bytes() + d_masks.bytes() + d_mrec.bytes() + d_pend.bytes());
   int64_t budget;
-  if (opt.scratch_mb > 0) {
-    budget = opt.scratch_mb << 20;
+  if (eo.scratch_mb > 0) {
+    budget = eo.scratch_mb << 20;
   } else {
     size_t fr = 0, tot = 0;
     LSIM_HIP(hipMemGetInfo(&fr, &tot));
-    const int64_t held = int64_t(d_rows.n * 8 + d_masks.n * 8 + d_mrec.n * sizeof(MaskRec) + d_pend.n * 4);
     budget = (int64_t(fr) + held) / 10 * 7;
   }
-  const int64_t cap = std::max<int64_t>(1, budget / std::max<int64_t>(per_slot, 1));
+  const int64_t cap = std::max<int64_t>(1, budget / per_slot);
   // Up to three resident 8-wave workgroups per CU (the event sweep's register
   // use allows 6 waves per SIMD).
   const int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 3)));
-  if (s > slots || !d_rows.p) {
+  if (s > slots || nl > slot_nl || !d_rows.p) {
+    const int64_t ns = std::min(cap, std::max(s, slots));
     d_rows.release();
     d_masks.release();
     d_mrec.release();
     d_pend.release();
-    const int64_t nl = std::max<int64_t>(NL, 1);
-    d_rows.alloc(size_t(s) * nl * 64);
-    d_masks.alloc(size_t(s) * nl);
-    d_mrec.alloc(size_t(s) * nl);
-    d_pend.alloc(size_t(s) * nl);
+    d_rows.alloc(size_t(ns) * nl * 64);
+    d_masks.alloc(size_t(ns) * nl);
+    d_mrec.alloc(size_t(ns) * nl);
+    d_pend.alloc(size_t(ns) * nl);
     // Epoch 0 is never used by a tile, so zeroed records read as "no lane differs".
-    LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, size_t(s) * nl * sizeof(MaskRec), stream));
-    LSIM_HIP(hipMemsetAsync(d_pend.p, 0, size_t(s) * nl * sizeof(uint32_t), stream));
-    slots = s;
+    LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, d_mrec.bytes(), stream));
+    LSIM_HIP(hipMemsetAsync(d_pend.p, 0, d_pend.bytes(), stream));
+    slots = ns;
+    slot_nl = nl;
   }
 }
 
-void Engine::Impl::sweep(const RunOptions &opt, const std::vector<Lane> &lanes, std::vector<Tile> &tiles,
-                         RunResults &res) {
-  lsim_run_stats &st = res.stats;
+void Engine::Impl::sweep(const EngineOptions &eo, const std::vector<Lane> &lanes, const std::vector<Tile> &tiles,
+                         int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st) {
   const int64_t ntiles = int64_t(tiles.size());
   if (!ntiles) return;
   d_lanes.alloc(lanes.size());
   d_tiles.alloc(tiles.size());
-  LSIM_HIP(hipMemcpyAsync(d_lanes.p, lanes.data(), lanes.size() * sizeof(Lane), hipMemcpyHostToDevice, stream));
-  LSIM_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
-  ensure_slots(opt, ntiles);
+  h2d(d_lanes.p, lanes.data(), lanes.size() * sizeof(Lane));
+  h2d(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile));
+  ensure_slots(eo, ntiles, nl_max);
   SweepArgs a;
-  a.info = d_info.p;
-  a.fin = d_fin.p;
-  a.good = d_good.p;
-  a.RW = R;
-  a.live_lvl = d_live_lvl.p;
-  a.max_level = L;
+  a.jobs = d_jobs.p;
   a.tiles = d_tiles.p;
   a.ntiles = int(ntiles);
+  a.lvl_entries = lvl_entries;
   a.lanes = d_lanes.p;
   a.rows = d_rows.p;
   a.masks = d_masks.p;
-  a.rows_stride = NL * 64;
-  a.masks_stride = NL;
+  a.rows_stride = slot_nl * 64;
+  a.masks_stride = slot_nl;
   a.out = d_dout.p;
   d_queue.alloc(1);
   a.queue = d_queue.p;
   a.bytes = d_bytes.p;
   a.mrec = d_mrec.p;
   a.pend = d_pend.p;
-  a.rec_stride = NL;
-  a.pend_stride = NL;
-  a.fo_off = d_fo_off.p;
-  a.fo = d_fo.p;
+  a.rec_stride = slot_nl;
+  a.pend_stride = slot_nl;
   a.epoch_base = 0;
   st.sweep_tiles += ntiles;
   time_begin();
-  if (opt.mode == LSIM_MODE_PARTITIONED) {
+  if (!plain_circuit) {
     // Tile epochs are unique over the engine's lifetime; before the 32-bit
     // counter wraps, the records are cleared and numbering restarts.
     if (uint64_t(epoch_base) + uint64_t(ntiles) + 1 >= (uint64_t(1) << 31)) {
-      LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, d_mrec.n * sizeof(MaskRec), stream));
-      LSIM_HIP(hipMemsetAsync(d_pend.p, 0, d_pend.n * sizeof(uint32_t), stream));
+      LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, d_mrec.bytes(), stream));
+      LSIM_HIP(hipMemsetAsync(d_pend.p, 0, d_pend.bytes(), stream));
       epoch_base = 0;
     }
     a.epoch_base = epoch_base;
     epoch_base += uint32_t(ntiles);
     LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
-    const int bw = std::max(1, std::min(kChunk / 64, opt.block_waves));
+    const int bw = std::max(1, std::min(kChunk / 64, eo.block_waves));
     const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
-    k_sweep_events<<<grid, 64 * bw, size_t(L + 1) * sizeof(int), stream>>>(a);
+    k_sweep_events<<<grid, 64 * bw, size_t(lvl_entries) * sizeof(int), stream>>>(a);
     LSIM_HIP(hipGetLastError());
     st.sweep_launches++;
   } else {
+    const CircuitRes &c = *plain_circuit;
     for (int64_t t0 = 0; t0 < ntiles; t0 += slots) {
       const int nb = int(std::min<int64_t>(slots, ntiles - t0));
       int lmin = tiles[t0].start_level;
       for (int64_t t = t0; t < t0 + nb; ++t) lmin = std::min(lmin, tiles[t].start_level);
-      for (int l = lmin; l <= L; ++l) {
-        const int64_t n = live_lvl[l + 1] - live_lvl[l];
+      for (int l = lmin; l <= c.L; ++l) {
+        const int64_t n = c.live_lvl[l + 1] - c.live_lvl[l];
         if (n <= 0) continue;
         const int64_t threads = int64_t(nb) * n * 64;
         k_sweep_level<<<blocks_for(threads, 256), 256, 0, stream>>>(a, l, int(t0), nb);
         LSIM_HIP(hipGetLastError());
         st.sweep_launches++;
       }
-      for (int64_t t = t0; t < t0 + nb; ++t) st.sweep_node_steps += NL - live_lvl[tiles[t].start_level];
+      for (int64_t t = t0; t < t0 + nb; ++t) st.sweep_node_steps += c.NL - c.live_lvl[tiles[t].start_level];
     }
   }
   st.t_sweep_ms += time_end();
 }
 
-void Engine::Impl::step(const RunOptions &opt, const std::vector<int32_t> &faults, int64_t w0, int Wn, bool direct,
-                        int32_t *first_dev, uint64_t *detbits_dev, int64_t RB, uint8_t *last_dev, RunResults &res) {
-  lsim_run_stats &st = res.stats;
-  st.steps++;
-  const int64_t nf = int64_t(faults.size());
-  if (!nf) return;
-  // Units: stems of the faults' regions (stem tracing) or the faults
-  // themselves in site order (direct injection).
-  std::vector<int32_t> flist, units;
-  if (direct) {
-    std::vector<int32_t> cnt(size_t(NL) + 1, 0);
-    for (int32_t f : faults) cnt[f_site[f] + 1]++;
-    for (int64_t u = 0; u < NL; ++u) cnt[u + 1] += cnt[u];
-    flist.resize(faults.size());
-    for (int32_t f : faults) flist[cnt[f_site[f]]++] = f;
-    units.resize(flist.size());
-    for (size_t k = 0; k < flist.size(); ++k) units[k] = f_site[flist[k]];
-  } else {
-    flist = faults;
-    head_mark.assign(size_t(NL), 0);
-    for (int32_t f : flist) {
-      const int32_t h = f_head[f];
-      if (!head_is_po[h]) head_mark[h] = 1;
+void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOptions &eo) {
+  const int nj = int(act.size());
+  std::vector<JobDev> devs(nj);
+  std::vector<int64_t> item_off(nj + 1, 0);
+  std::vector<int64_t> word0(nj + 1, 0);
+  std::vector<int4> units_all;
+  int lvl_entries = 1;
+  int64_t nl_max = 1;
+  lsim_run_stats st{};
+  // Per job: the step's faults in unit order and its units (stems of the
+  // faults' regions, or the fault sites themselves for direct injection).
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    const CircuitRes &c = *j.c;
+    j.units.clear();
+    if (j.direct) {
+      count_buf.assign(size_t(c.NL) + 1, 0);
+      for (int32_t f : j.step_faults) count_buf[j.site[f] + 1]++;
+      for (int64_t u = 0; u < c.NL; ++u) count_buf[u + 1] += count_buf[u];
+      std::vector<int32_t> sorted(j.step_faults.size());
+      for (int32_t f : j.step_faults) sorted[count_buf[j.site[f]]++] = f;
+      j.step_faults.swap(sorted);
+      j.units.resize(j.step_faults.size());
+      for (size_t k = 0; k < j.step_faults.size(); ++k) j.units[k] = j.site[j.step_faults[k]];
+    } else {
+      head_mark.assign(size_t(c.NL), 0);
+      for (int32_t f : j.step_faults) {
+        const int32_t h = j.head[f];
+        if (!c.head_is_po[h]) head_mark[h] = 1;
+      }
+      for (int64_t u = 0; u < c.NL; ++u)
+        if (head_mark[u]) {
+          units_all.push_back(make_int4(i, int(u), int(j.units.size()), 0));
+          j.units.push_back(int32_t(u));
+        }
     }
-    for (int64_t u = 0; u < NL; ++u)
-      if (head_mark[u]) units.push_back(int32_t(u));
+    const int64_t nf = int64_t(j.step_faults.size());
+    h2d(j.flist.p, j.step_faults.data(), nf * sizeof(int32_t));
+    JobDev d = job_dev(j);
+    d.flist = j.flist.p;
+    d.nf = nf;
+    d.w0 = j.step_w0;
+    d.Wn = j.step_Wn;
+    d.first_round = (j.kind == JobState::LAST) ? nullptr : j.first.p;
+    d.det_bits = (j.kind == JobState::MATRIX) ? j.detbits.p : nullptr;
+    d.last_flag = (j.kind == JobState::LAST) ? j.last.p : nullptr;
+    d.item0 = item_off[i];
+    d.word0 = word0[i];
+    item_off[i + 1] = item_off[i] + nf * j.step_Wn;
+    word0[i + 1] = word0[i] + int64_t(j.units.size()) * j.step_Wn;
+    devs[i] = d;
+    lvl_entries = std::max(lvl_entries, c.L + 1);
+    nl_max = std::max(nl_max, c.NL);
   }
-  const int64_t nu = int64_t(units.size());
-  d_flist.alloc(nf);
-  LSIM_HIP(hipMemcpyAsync(d_flist.p, flist.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-  const int64_t nuw = std::max<int64_t>(nu * Wn, 1);
-  d_need.alloc(nuw);
-  d_dout.alloc(nuw);
-  LSIM_HIP(hipMemsetAsync(d_need.p, 0, nuw * sizeof(uint64_t), stream));
-  LSIM_HIP(hipMemsetAsync(d_dout.p, 0, nuw * sizeof(uint64_t), stream));
-  if (!direct && nu) {
-    d_units.alloc(nu);
-    LSIM_HIP(hipMemcpyAsync(d_units.p, units.data(), nu * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    k_set_slots<<<blocks_for(nu, 256), 256, 0, stream>>>(d_head_slot.p, d_units.p, int(nu), 0);
+  const int64_t items = item_off[nj], words = std::max<int64_t>(word0[nj], 1);
+  d_jobs.alloc(nj);
+  d_item_off.alloc(nj + 1);
+  h2d(d_jobs.p, devs.data(), nj * sizeof(JobDev));
+  h2d(d_item_off.p, item_off.data(), (nj + 1) * sizeof(int64_t));
+  d_need.alloc(words);
+  d_dout.alloc(words);
+  LSIM_HIP(hipMemsetAsync(d_need.p, 0, words * sizeof(uint64_t), stream));
+  LSIM_HIP(hipMemsetAsync(d_dout.p, 0, words * sizeof(uint64_t), stream));
+  const int nu_all = int(units_all.size());
+  if (nu_all) {
+    d_units.alloc(nu_all);
+    h2d(d_units.p, units_all.data(), nu_all * sizeof(int4));
+    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, d_units.p, nu_all, 0);
     LSIM_HIP(hipGetLastError());
   }
-  DetectArgs d;
-  d.info = d_info.p;
-  d.fin = d_fin.p;
-  d.good = d_good.p;
-  d.RW = R;
-  d.parent = d_parent.p;
-  d.head_slot = d_head_slot.p;
-  d.is_po = d_is_po.p;
-  d.need = d_need.p;
-  d.dout = d_dout.p;
-  d.w0 = w0;
-  d.Wn = Wn;
-  d.flist = d_flist.p;
-  d.nf = int(nf);
-  d.site = d_site.p;
-  d.head = d_head.p;
-  d.sa = d_sa.p;
-  d.num_patterns = num_patterns;
-  d.first_round = first_dev;
-  d.det_bits = detbits_dev;
-  d.RB = RB;
-  d.last_flag = last_dev;
-  d.last_word = R - 1;
+  ItemArgs ia;
+  ia.jobs = d_jobs.p;
+  ia.njobs = nj;
+  ia.item_off = d_item_off.p;
+  ia.need = d_need.p;
+  ia.dout = d_dout.p;
   time_begin();
-  if (direct) k_local_direct<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-  else k_local_cpt<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-  LSIM_HIP(hipGetLastError());
+  if (items) {
+    k_local<<<blocks_for(items, 256), 256, 0, stream>>>(ia);
+    LSIM_HIP(hipGetLastError());
+  }
   // The needed bits decide which (unit, word) pairs become sweep lanes.
-  std::vector<uint64_t> need(size_t(nu) * Wn);
-  if (nu)
-    LSIM_HIP(hipMemcpyAsync(need.data(), d_need.p, need.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+  std::vector<uint64_t> need(static_cast<size_t>(word0[nj]));
+  d2h(need.data(), d_need.p, need.size() * sizeof(uint64_t));
   st.t_detect_ms += time_end();
   std::vector<Lane> lanes;
-  for (int64_t k = 0; k < nu; ++k)
-    for (int j = 0; j < Wn; ++j) {
-      const uint64_t m = need[size_t(k) * Wn + j];
-      if (!m) continue;
-      Lane ln;
-      ln.mask = m;
-      ln.site = units[k];
-      ln.word = int32_t(w0 + j);
-      ln.out = int32_t(k * Wn + j);
-      ln.kind = direct ? (f_sa[flist[k]] ? K_SA1 : K_SA0) : K_FLIP;
-      ln.level = level_dp[units[k]];
-      ln.pad = 0;
-      lanes.push_back(ln);
-    }
   std::vector<Tile> tiles;
-  for (size_t i = 0; i < lanes.size(); i += 64) {
-    Tile t;
-    t.lane0 = int32_t(i);
-    t.nlanes = int32_t(std::min<size_t>(64, lanes.size() - i));
-    t.start_level = level_dp[lanes[i].site];
-    t.end_level = level_dp[lanes[i + t.nlanes - 1].site];
-    tiles.push_back(t);
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    const size_t first_lane = lanes.size();
+    const int Wn = j.step_Wn;
+    for (size_t k = 0; k < j.units.size(); ++k)
+      for (int jw = 0; jw < Wn; ++jw) {
+        const int64_t o = word0[i] + int64_t(k) * Wn + jw;
+        const uint64_t msk = need[size_t(o)];
+        if (!msk) continue;
+        Lane ln;
+        ln.mask = msk;
+        ln.site = j.units[k];
+        ln.word = int32_t(j.step_w0 + jw);
+        ln.out = int32_t(o);
+        ln.kind = j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP;
+        ln.level = j.c->level_dp[j.units[k]];
+        ln.pad = 0;
+        lanes.push_back(ln);
+      }
+    for (size_t l0 = first_lane; l0 < lanes.size(); l0 += 64) {
+      Tile t;
+      t.lane0 = int32_t(l0);
+      t.nlanes = int32_t(std::min<size_t>(64, lanes.size() - l0));
+      t.start_level = lanes[l0].level;
+      t.end_level = lanes[l0 + t.nlanes - 1].level;
+      t.job = i;
+      tiles.push_back(t);
+    }
+    j.res.stats.stems += j.direct ? 0 : int64_t(j.units.size());
   }
-  st.stems += direct ? 0 : nu;
-  const double before = st.t_sweep_ms;
-  sweep(opt, lanes, tiles, res);
-  if (std::getenv("LSIM_TRACE"))
-    std::fprintf(stderr, "[lsim] step w0=%lld Wn=%d faults=%lld units=%lld lanes=%lld tiles=%lld sweep=%.2fms\n",
-                 (long long)w0, Wn, (long long)nf, (long long)nu, (long long)lanes.size(), (long long)tiles.size(),
-                 st.t_sweep_ms - before);
+  // Deep tiles first: the queue then ends with short tiles, balancing the tail.
+  std::stable_sort(tiles.begin(), tiles.end(), [&](const Tile &x, const Tile &y) {
+    return (devs[x.job].max_level - x.start_level) > (devs[y.job].max_level - y.start_level);
+  });
+  const bool plain = eo.mode == LSIM_MODE_PLAIN;
+  sweep(eo, lanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, st);
   time_begin();
-  if (direct) k_detect_direct<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-  else k_detect_cpt<<<blocks_for(nf * Wn, 256), 256, 0, stream>>>(d);
-  LSIM_HIP(hipGetLastError());
-  if (!direct && nu) {
-    k_set_slots<<<blocks_for(nu, 256), 256, 0, stream>>>(d_head_slot.p, d_units.p, int(nu), 1);
+  if (items) {
+    k_detect<<<blocks_for(items, 256), 256, 0, stream>>>(ia);
     LSIM_HIP(hipGetLastError());
   }
+  if (nu_all) {
+    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, d_units.p, nu_all, 1);
+    LSIM_HIP(hipGetLastError());
+  }
+  unsigned long long bytes = 0;
+  d2h(&bytes, d_bytes.p, sizeof(bytes));
+  LSIM_HIP(hipMemsetAsync(d_bytes.p, 0, sizeof(unsigned long long), stream));
   st.t_detect_ms += time_end();
+  st.sweep_bytes = int64_t(bytes);
+  if (std::getenv("LSIM_TRACE"))
+    std::fprintf(stderr, "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld sweep=%.2fms\n", nj,
+                 (long long)items, (long long)word0[nj], (long long)lanes.size(), (long long)tiles.size(),
+                 st.t_sweep_ms);
+  // Advance every job.
+  std::vector<int32_t> first;
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    lsim_run_stats &js = j.res.stats;
+    js.steps++;
+    js.t_sweep_ms += st.t_sweep_ms;
+    js.t_detect_ms += st.t_detect_ms;
+    js.sweep_launches += st.sweep_launches;
+    js.sweep_tiles += st.sweep_tiles;
+    js.sweep_node_steps += st.sweep_node_steps;
+    js.sweep_bytes += st.sweep_bytes;
+    switch (j.kind) {
+      case JobState::MATRIX: j.w0 += j.step_Wn; break;
+      case JobState::DROP: {
+        first.resize(size_t(j.F));
+        d2h(first.data(), j.first.p, j.F * sizeof(int32_t));
+        LSIM_HIP(hipStreamSynchronize(stream));
+        std::vector<int32_t> next;
+        next.reserve(j.alive.size());
+        for (int32_t f : j.alive)
+          if (first[f] == INT32_MAX) next.push_back(f);
+        j.alive.swap(next);
+        j.w0 += j.step_Wn;
+        j.Wt = std::min(64, j.Wt * 2);
+        break;
+      }
+      case JobState::LAST: j.last_done = true; break;
+    }
+  }
 }
 
-void Engine::run(const RunOptions &opt, RunResults &res) {
+int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  res = RunResults();
-  lsim_run_stats &st = res.stats;
-  const auto t0 = std::chrono::steady_clock::now();
-  const int64_t N = m.N, R = m.R, F = m.F;
-  st.live_nodes = m.NL;
-  st.live_faults = m.live_faults;
-  m.d_bytes.alloc(1);
-  LSIM_HIP(hipMemsetAsync(m.d_bytes.p, 0, sizeof(unsigned long long), m.stream));
-  // Good machine over every round.
-  if (R > 0) {
-    m.time_begin();
-    for (int l = 0; l <= m.L; ++l) {
-      const int a0 = m.live_lvl[l], n0 = m.live_lvl[l + 1] - a0;
-      const int a1 = m.dead_lvl[l], n1 = m.dead_lvl[l + 1] - a1;
-      const int64_t threads = int64_t(n0 + n1) * R;
-      if (threads == 0) continue;
-      k_good_level<<<blocks_for(threads, 256), 256, 0, m.stream>>>(m.d_info.p, m.d_fin.p, m.d_good.p,
-                                                                  m.d_pi_words.p, R, a0, n0, a1, n1);
-      LSIM_HIP(hipGetLastError());
-    }
-    st.t_good_ms = m.time_end();
-    // One word per node written, one per fanin and per PI pattern read.
-    st.good_bytes = 8 * (N + m.E + m.P) * R;
-  }
-  // Last-round good values, masked (fsim/simulation.cpp:163-178).
-  res.good_last.assign(N, 0);
-  if (R > 0) {
-    const int64_t bits = (m.num_patterns / (64 * R)) ? 64 : (m.num_patterns % 64);
-    const uint64_t mask = bits < 64 ? ((uint64_t(1) << bits) - 1) : ~uint64_t(0);
-    m.d_gather.alloc(N);
-    k_gather_round<<<blocks_for(N, 256), 256, 0, m.stream>>>(m.d_good.p, R, R - 1, int(N), mask, m.d_gather.p);
-    LSIM_HIP(hipGetLastError());
-    std::vector<uint64_t> byd(N);
-    LSIM_HIP(hipMemcpyAsync(byd.data(), m.d_gather.p, N * sizeof(uint64_t), hipMemcpyDeviceToHost, m.stream));
-    LSIM_HIP(hipStreamSynchronize(m.stream));
-    for (int64_t p = 0; p < N; ++p) res.good_last[p] = byd[m.pos2dp[p]];
-  }
-  std::vector<int32_t> live;
-  live.reserve(m.live_faults);
-  for (int64_t f = 0; f < F; ++f)
-    if (m.f_head[f] >= 0) live.push_back(int32_t(f));
-  const bool direct = opt.direct || m.has_mux;
-  const int64_t F1 = std::max<int64_t>(F, 1);
-  m.d_first.alloc(F1);
-  k_fill_i32<<<blocks_for(F1, 256), 256, 0, m.stream>>>(m.d_first.p, F1, INT32_MAX);
-  LSIM_HIP(hipGetLastError());
-  res.first_round.assign(F, INT32_MAX);
-  const int64_t RB = (R + 63) / 64;
-  if (R > 0 && !live.empty()) {
-    if (!opt.drop) {
-      m.d_detbits.alloc(size_t(std::max<int64_t>(F * RB, 1)));
-      LSIM_HIP(hipMemsetAsync(m.d_detbits.p, 0, size_t(std::max<int64_t>(F * RB, 1)) * sizeof(uint64_t), m.stream));
-      for (int64_t w0 = 0; w0 < R;) {
-        const int Wn = int(std::min<int64_t>(64, R - w0));
-        m.step(opt, live, w0, Wn, direct, m.d_first.p, m.d_detbits.p, RB, nullptr, res);
-        w0 += Wn;
-      }
-      res.RB = RB;
-      res.det_bits.assign(size_t(F * RB), 0);
-      if (F)
-        LSIM_HIP(hipMemcpyAsync(res.det_bits.data(), m.d_detbits.p, size_t(F * RB) * sizeof(uint64_t),
-                                hipMemcpyDeviceToHost, m.stream));
-      LSIM_HIP(hipStreamSynchronize(m.stream));
-      res.has_matrix = true;
-      res.last_round.assign(F, 0);
-      for (int64_t f = 0; f < F; ++f)
-        res.last_round[f] = uint8_t((res.det_bits[f * RB + ((R - 1) >> 6)] >> ((R - 1) & 63)) & 1);
-      res.has_last = true;
+  std::vector<JobState *> act;
+  for (JobState *j : jobs)
+    if (j->started && !j->done && m.plan(*j)) act.push_back(j);
+  if (!act.empty()) {
+    if (eo.mode == LSIM_MODE_PLAIN) {
+      for (JobState *j : act) m.step_batch({j}, eo);  // the per-level form runs one job at a time
     } else {
-      // Words per step grow geometrically: the faults left after the first
-      // rounds are the hard ones and are dropped less often.
-      std::vector<int32_t> alive = live;
-      std::vector<int32_t> first(F);
-      int Wt = 1;
-      for (int64_t w0 = 0; w0 < R && !alive.empty();) {
-        const int Wn = int(std::min<int64_t>(Wt, R - w0));
-        m.step(opt, alive, w0, Wn, direct, m.d_first.p, nullptr, 0, nullptr, res);
-        LSIM_HIP(hipMemcpyAsync(first.data(), m.d_first.p, F * sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
-        LSIM_HIP(hipStreamSynchronize(m.stream));
-        std::vector<int32_t> next;
-        next.reserve(alive.size());
-        for (int32_t f : alive)
-          if (first[f] == INT32_MAX) next.push_back(f);
-        alive.swap(next);
-        w0 += Wn;
-        Wt = std::min(64, Wt * 2);
-      }
-      if (opt.report_last_round) {
-        m.d_last.alloc(F1);
-        LSIM_HIP(hipMemsetAsync(m.d_last.p, 0, F1, m.stream));
-        m.step(opt, live, R - 1, 1, direct, nullptr, nullptr, 0, m.d_last.p, res);
-        res.last_round.assign(F, 0);
-        if (F) LSIM_HIP(hipMemcpyAsync(res.last_round.data(), m.d_last.p, F, hipMemcpyDeviceToHost, m.stream));
-        res.has_last = true;
-      }
+      m.step_batch(act, eo);
     }
-    LSIM_HIP(hipMemcpyAsync(res.first_round.data(), m.d_first.p, F * sizeof(int32_t), hipMemcpyDeviceToHost,
-                            m.stream));
-    LSIM_HIP(hipStreamSynchronize(m.stream));
-  } else {
-    // No rounds or no observable fault: nothing is ever detected.
-    res.has_matrix = !opt.drop;
-    res.RB = RB;
-    res.det_bits.assign(size_t(F * RB), 0);
-    res.last_round.assign(F, 0);
-    res.has_last = !opt.drop || opt.report_last_round;
   }
-  for (auto &v : res.first_round)
-    if (v == INT32_MAX) v = -1;
-  {
-    unsigned long long b = 0;
-    LSIM_HIP(hipMemcpyAsync(&b, m.d_bytes.p, sizeof(b), hipMemcpyDeviceToHost, m.stream));
-    LSIM_HIP(hipStreamSynchronize(m.stream));
-    st.sweep_bytes = int64_t(b);
+  int64_t left = 0;
+  for (JobState *j : jobs) left += j->done ? 0 : 1;
+  return left;
+}
+
+void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
+  const auto t0 = std::chrono::steady_clock::now();
+  start(jobs);
+  while (iterate(jobs, eo) > 0) {
   }
-  st.slots = m.slots;
-  st.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (JobState *j : jobs) j->res.stats.t_total_ms = ms;
 }
 
 }  // namespace lsim

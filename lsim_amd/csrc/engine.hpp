@@ -1,6 +1,14 @@
 // GPU fault-simulation engine (host interface; the HIP code lives in engine.hip).
+//
+// A circuit is made resident once (structure in HBM). A job is one fault
+// simulation on a resident circuit: a fault list, a pattern block and options.
+// The engine advances any set of jobs together: each iteration moves every
+// active job forward by one word step and sweeps the tiles of all of them in
+// one persistent launch, so jobs of different circuits and sizes share the
+// device and new jobs can join between iterations.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -15,19 +23,26 @@ struct DeviceError : std::runtime_error {
   DeviceError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
-struct RunOptions {
-  int mode = LSIM_MODE_PLAIN;
-  bool drop = false;
-  bool report_last_round = true;
-  int64_t scratch_mb = 0;
-  bool direct = false;
-  int block_waves = 8;
+// Per-job options.
+struct JobOptions {
+  bool drop = false;              // fault dropping
+  bool report_last_round = true;  // with dropping: evaluate the last round for every fault
+  bool direct = false;            // inject every fault directly (forced for circuits with MUX)
+  bool keep_good_last = false;    // gather the last round's good values (reference accessors)
+};
+
+// Engine-wide options of a run / iteration.
+struct EngineOptions {
+  int mode = LSIM_MODE_PARTITIONED;  // PLAIN (per-level launches) or PARTITIONED (persistent events)
+  int64_t scratch_mb = 0;            // sweep scratch budget, 0 = automatic
+  int block_waves = 8;               // waves per workgroup of the persistent sweep (1..8)
 };
 
 struct RunResults {
-  // Good-machine values of the last round, masked, per topological position.
+  // Good-machine values of the last round, masked, per topological position
+  // (only with keep_good_last).
   std::vector<uint64_t> good_last;
-  // First detecting round per fault, -1 if never (all modes).
+  // First detecting round per fault, -1 if never.
   std::vector<int32_t> first_round;
   // d(f, r) bit matrix, F rows of RB words (only without dropping).
   std::vector<uint64_t> det_bits;
@@ -39,16 +54,41 @@ struct RunResults {
   lsim_run_stats stats{};
 };
 
+struct CircuitRes;  // resident circuit (engine.hip)
+struct JobState;    // job (engine.hip)
+
 class Engine {
  public:
   explicit Engine(int device);
   ~Engine();
   Engine(const Engine &) = delete;
   Engine &operator=(const Engine &) = delete;
-  // Makes the circuit, faults and patterns resident on the device.
-  void load(const Netlist &nl, const FaultStructure &fs);
-  void run(const RunOptions &opt, RunResults &res);
   int device() const;
+
+  // Makes a circuit resident. The engine owns it until remove_circuit.
+  CircuitRes *add_circuit(const Netlist &nl, const FaultStructure &fs);
+  void remove_circuit(CircuitRes *c);
+  int64_t circuit_bytes(const CircuitRes *c) const;
+
+  // Creates a job on a resident circuit (faults as positions, patterns as
+  // R x P words) and uploads its inputs. The engine owns it until release_job.
+  JobState *add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::vector<uint8_t> fault_sa,
+                    int64_t num_patterns, int64_t R, const std::vector<uint64_t> &patterns, const JobOptions &opt);
+  void release_job(JobState *j);
+  // Device bytes a job on circuit c would hold (admission by footprint).
+  static int64_t job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop);
+  int64_t free_bytes() const;
+
+  // Good machine for jobs not started yet (one batched launch per level).
+  void start(const std::vector<JobState *> &jobs);
+  // One word step for every started, unfinished job. Jobs that finish are
+  // finalised (results copied to the host). Returns the number still running.
+  int64_t iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo);
+  // start + iterate until every job is done.
+  void run(const std::vector<JobState *> &jobs, const EngineOptions &eo);
+
+  bool done(const JobState *j) const;
+  RunResults &results(JobState *j);
 
  private:
   struct Impl;

@@ -30,8 +30,7 @@ uint64_t need(TextIn &in, const char *what) {
 
 }  // namespace
 
-void read_netlist(Netlist &nl, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len,
-                  const char *ptn, size_t ptn_len) {
+void read_circuit(Netlist &nl, const char *ckt, size_t ckt_len) {
   nl = Netlist();
   TextIn c(ckt, ckt_len);
   nl.P = int64_t(need(c, "circuit header"));
@@ -155,27 +154,41 @@ void read_netlist(Netlist &nl, const char *ckt, size_t ckt_len, const char *flst
   for (int64_t p = 1; p < N; ++p)
     if (nl.level[p] < nl.level[p - 1]) throw InputError("topological order is not level-sorted");
 
-  // Faults.
+}
+
+void read_faults(const Netlist &nl, const char *flst, size_t flst_len, std::vector<int32_t> &site,
+                 std::vector<uint8_t> &sa) {
   TextIn f(flst, flst_len);
   const uint64_t F = need(f, "fault list header");
-  nl.fault_site.resize(F);
-  nl.fault_sa.resize(F);
+  if (F > uint64_t(INT32_MAX)) throw InputError("fault list too long");
+  site.resize(F);
+  sa.resize(F);
   for (uint64_t i = 0; i < F; ++i) {
     const uint64_t id = need(f, "fault list");
     const uint64_t v = need(f, "fault list");
-    if (id >= uint64_t(N)) throw InputError("fault id out of range");
-    nl.fault_site[i] = int32_t(nl.order_idx[id]);
-    nl.fault_sa[i] = v ? 1 : 0;
+    if (id >= uint64_t(nl.N)) throw InputError("fault id out of range");
+    site[i] = int32_t(nl.order_idx[id]);
+    sa[i] = v ? 1 : 0;
   }
+}
 
-  // Patterns: the reference stores round i at _patterns[order_idx[i]], which
-  // is round i itself whenever R <= P.
+void read_patterns(const Netlist &nl, const char *ptn, size_t ptn_len, int64_t &num_patterns, int64_t &R,
+                   std::vector<uint64_t> &words) {
+  // The reference stores round i at _patterns[order_idx[i]], which is round i
+  // itself whenever R <= P.
   TextIn t(ptn, ptn_len);
-  nl.num_patterns = int64_t(need(t, "pattern header"));
-  nl.R = (nl.num_patterns + 63) / 64;
-  if (nl.R > P) throw InputError("more 64-pattern rounds than PIs");
-  nl.patterns.resize(size_t(nl.R * P));
-  for (int64_t i = 0; i < nl.R * P; ++i) nl.patterns[i] = need(t, "pattern words");
+  num_patterns = int64_t(need(t, "pattern header"));
+  R = (num_patterns + 63) / 64;
+  if (R > nl.P) throw InputError("more 64-pattern rounds than PIs");
+  words.resize(size_t(R * nl.P));
+  for (int64_t i = 0; i < R * nl.P; ++i) words[i] = need(t, "pattern words");
+}
+
+void read_netlist(Netlist &nl, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len,
+                  const char *ptn, size_t ptn_len) {
+  read_circuit(nl, ckt, ckt_len);
+  read_faults(nl, flst, flst_len, nl.fault_site, nl.fault_sa);
+  read_patterns(nl, ptn, ptn_len, nl.num_patterns, nl.R, nl.patterns);
 }
 
 void build_fault_structure(const Netlist &nl, FaultStructure &fs) {

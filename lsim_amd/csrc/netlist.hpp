@@ -63,6 +63,13 @@ struct Netlist {
 // reader does (fsim/simulation.cpp:37-63,664-976). Throws InputError.
 void read_netlist(Netlist &nl, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len,
                   const char *ptn, size_t ptn_len);
+// The three parts separately: the circuit (fsim/simulation.cpp:40,664-941),
+// a fault list against it (:943-959) and a pattern set against it (:961-976).
+void read_circuit(Netlist &nl, const char *ckt, size_t ckt_len);
+void read_faults(const Netlist &nl, const char *flst, size_t flst_len, std::vector<int32_t> &site,
+                 std::vector<uint8_t> &sa);
+void read_patterns(const Netlist &nl, const char *ptn, size_t ptn_len, int64_t &num_patterns, int64_t &R,
+                   std::vector<uint64_t> &words);
 
 // Derived structure for fault simulation.
 struct FaultStructure {
