@@ -91,6 +91,12 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
             "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
 
 
+def _kernel_name(args) -> str:
+    if args.mode == "plain":
+        return "k_sweep_level"
+    return "k_sweep_pairs" if args.sweep_kernel == "pairs" else "k_sweep_events"
+
+
 def bench_batch(args) -> int:
     """256 circuits of uneven size, each with its full fault list and its own
     pattern block, run together through the batch API (one persistent sweep
@@ -101,7 +107,8 @@ def bench_batch(args) -> int:
 
     specs = workloads.c4_specs()
     t0 = time.perf_counter()
-    batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0)
+    batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0,
+                           sweep_kernel=0 if args.sweep_kernel == "pairs" else 1)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
@@ -131,7 +138,7 @@ def bench_batch(args) -> int:
         for j in ids:
             batch.release_job(j)
     elapsed = sum(times)
-    kernel = "k_sweep_events" if args.mode == "partitioned" else "k_sweep_level"
+    kernel = _kernel_name(args)
     out = {
         "metric": "fault x pattern evals/sec",
         "value": evals / (elapsed / args.steps),
@@ -269,6 +276,8 @@ def main() -> int:
     ap.add_argument("--c5-jobs", type=int, default=96, help="jobs per phase of the serving run")
     ap.add_argument("--c5-load", type=float, default=0.5, help="arrival rate as a fraction of the saturated rate")
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
+    ap.add_argument("--sweep-kernel", default="pairs", choices=["pairs", "nodewave"],
+                    help="persistent sweep kernel of the partitioned engine")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -304,6 +313,7 @@ def main() -> int:
     sim = lsim_amd.Simulator(device=0)
     sim.set_option("drop", 1 if preset["drop"] else 0)
     sim.set_option("report_last_round", 0)
+    sim.set_option("sweep_kernel", 0 if args.sweep_kernel == "pairs" else 1)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")
@@ -332,7 +342,7 @@ def main() -> int:
     ms = 1000.0 * elapsed / args.steps
     value = F * n / (elapsed / args.steps)
     detected = int(sim.detected().sum())
-    kernel = "k_sweep_events" if args.mode == "partitioned" else "k_sweep_level"
+    kernel = _kernel_name(args)
     out = {
         "metric": "fault x pattern evals/sec",
         "value": value,

@@ -143,6 +143,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "block_waves") {
     if (value < 1 || value > 8) return fail(err, LSIM_E_ARG, "block_waves must be 1..8");
     eo.block_waves = int(value);
+  } else if (k == "sweep_kernel") {
+    if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "sweep_kernel must be 0 or 1");
+    eo.sweep_kernel = int(value);
   } else {
     known = false;
   }

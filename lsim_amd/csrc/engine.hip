@@ -35,7 +35,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -262,6 +265,11 @@ struct SweepArgs {
   uint32_t *pend;    // per slot: [NLmax], == epoch when the node must be evaluated
   int64_t rec_stride, pend_stride;
   uint32_t epoch_base;
+  // Optional tile statistics (event form), null unless traced:
+  // [0] tiles, [1] levels with pending nodes, [2] level scan passes,
+  // [3] staged nodes, [4] nodes written (some lane differs), [5] max nodes
+  // written by one tile, [6] levels walked, [7] (node, lane) evaluations.
+  unsigned long long *tstats;
 };
 
 struct LaneSetup {
@@ -327,9 +335,12 @@ __device__ __forceinline__ void sweep_node(const SweepView &J, int u, int base, 
   }
   const int64_t RW = J.RW;
   const uint64_t *good = J.good;
-  const uint64_t gu = good[int64_t(u) * RW + s.w];
+  // Lanes without a differing fanin and without an injection keep the good
+  // value and load nothing.
+  const bool act = (((m0 | m1 | m2 | m3) >> lane) & 1) || inj;
+  const uint64_t gu = act ? good[int64_t(u) * RW + s.w] : 0;
   uint64_t r = gu;
-  if (type != G_PI) {
+  if (act && type != G_PI) {
     const uint64_t v0 = ((m0 >> lane) & 1) ? rows[int64_t(f.x) * 64 + lane] : good[int64_t(f.x) * RW + s.w];
     uint64_t v1 = 0, v2 = 0, v3 = 0;
     if (nin > 1) v1 = ((m1 >> lane) & 1) ? rows[int64_t(f.y) * 64 + lane] : good[int64_t(f.y) * RW + s.w];
@@ -343,7 +354,7 @@ __device__ __forceinline__ void sweep_node(const SweepView &J, int u, int base, 
   if (d) rows[int64_t(u) * 64 + lane] = r;
   if (lane == 0) masks[u] = nm;
   if (type == G_PO) acc |= d;
-  bytes += uint64_t(s.active) * 8 * (type == G_PI ? 1 : nin + 1) + 8 * uint64_t(__popcll(nm));
+  bytes += uint64_t(__popcll(__ballot(act))) * 8 * (type == G_PI ? 1 : nin + 1) + 8 * uint64_t(__popcll(nm));
 }
 
 // Plain form: one launch per level; a wave handles one (tile, node) pair. All
@@ -396,15 +407,16 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t x) {
 // their fanin loads interleaved. When some lane differs, the node's row and
 // mask record are written and its live fanouts marked pending (counted per
 // level in LDS); the deepest pending level is raised.
-template <int NB>
+template <int NB, bool kStats>
 __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage *st, int cnt, int lane,
                                             const LaneSetup &s, uint32_t epoch, uint64_t *rows, MaskRec *mrec,
                                             uint32_t *pend, uint64_t &acc, uint64_t &bytes, int *s_maxl,
-                                            int *s_lvlcnt) {
+                                            int *s_lvlcnt, uint32_t &n_written, uint64_t &n_active) {
   int u[NB];
   uint32_t type[NB], nin[NB];
   int4 f[NB];
   uint64_t m[NB][4], gu[NB], r[NB];
+  bool act[NB];
   const int64_t RW = J.RW;
   const uint64_t *good = J.good;
 #pragma unroll
@@ -420,12 +432,16 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
     f[k].w = __builtin_amdgcn_readfirstlane(n.fin.w);
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[k][q] = uniform64(n.m[q]);
-    gu[k] = good[int64_t(u[k]) * RW + s.w];
+    // A lane takes part iff one of its fanins differs or it injects here;
+    // in every other lane the node keeps its good value and loads nothing.
+    const uint64_t any = m[k][0] | m[k][1] | m[k][2] | m[k][3];
+    act[k] = k < cnt && (((any >> lane) & 1) || s.site == u[k]);
+    gu[k] = act[k] ? good[int64_t(u[k]) * RW + s.w] : 0;
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     r[k] = gu[k];
-    if (k < cnt && type[k] != G_PI) {
+    if (act[k] && type[k] != G_PI) {
       const uint32_t n = nin[k];
       const uint64_t v0 =
           ((m[k][0] >> lane) & 1) ? rows[int64_t(f[k].x) * 64 + lane] : good[int64_t(f[k].x) * RW + s.w];
@@ -445,8 +461,11 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
     if (s.site == u[k]) r[k] = inject(s, gu[k]);
     const uint64_t d = r[k] ^ gu[k];
     const uint64_t nm = __ballot(d != 0);
-    bytes += uint64_t(s.active) * 8 * (type[k] == G_PI ? 1 : nin[k] + 1);
+    const uint64_t na = __ballot(act[k]);
+    if (kStats) n_active += __popcll(na);
+    bytes += uint64_t(__popcll(na)) * 8 * (type[k] == G_PI ? 1 : nin[k] + 1);
     if (nm) {
+      if (kStats) ++n_written;
       if (d) rows[int64_t(u[k]) * 64 + lane] = r[k];
       if (lane == 0) {
         MaskRec rec;
@@ -480,6 +499,7 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
 // workgroup leaves once the queue passes the last tile.
 constexpr int kChunk = 512;
 constexpr int kNodesPerWave = 2;
+template <bool kStats>
 __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))) k_sweep_events(SweepArgs a) {
   __shared__ unsigned long long acc_lds[64];
   __shared__ int s_tile, s_cnt, s_maxl;
@@ -508,11 +528,17 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
     }
     if (tid == 0) s_maxl = T.end_level;
     uint64_t acc = 0, bytes = 0, stage_bytes = 0;  // bytes: wave-uniform; stage_bytes: per lane
+    uint32_t n_lv = 0, n_pass = 0, n_staged = 0, n_written = 0;  // tile statistics
+    uint64_t n_active = 0;
     __syncthreads();
+    int l_end = T.start_level;
     for (int l = T.start_level; l <= s_maxl; ++l) {
+      l_end = l;
       if (s_lvlcnt[l] == 0) continue;
+      ++n_lv;
       const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += blockDim.x) {
+        ++n_pass;
         if (tid == 0) s_cnt = 0;
         __syncthreads();
         const int u = c0 + tid;
@@ -543,12 +569,30 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
         }
         __syncthreads();
         const int n = s_cnt;
+        n_staged += n;
         for (int i = wave * kNodesPerWave; i < n; i += nwaves * kNodesPerWave)
-          event_nodes<kNodesPerWave>(J, &s_stage[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec, pend,
-                                     acc, bytes, &s_maxl, s_lvlcnt);
+          event_nodes<kNodesPerWave, kStats>(J, &s_stage[i], min(kNodesPerWave, n - i), lane, s, epoch, rows, mrec,
+                                             pend, acc, bytes, &s_maxl, s_lvlcnt, n_written, n_active);
         __syncthreads();
       }
       if (tid == 0) s_lvlcnt[l] = 0;
+    }
+    if (kStats) {
+      // n_written is per wave: sum it over the workgroup through LDS.
+      if (tid == 0) s_cnt = 0;
+      __syncthreads();
+      if (lane == 0) atomicAdd(&s_cnt, int(n_written));
+      __syncthreads();
+      if (tid == 0) {
+        atomicAdd(&a.tstats[0], 1ull);
+        atomicAdd(&a.tstats[1], (unsigned long long)n_lv);
+        atomicAdd(&a.tstats[2], (unsigned long long)n_pass);
+        atomicAdd(&a.tstats[3], (unsigned long long)n_staged);
+        atomicAdd(&a.tstats[4], (unsigned long long)s_cnt);
+        atomicMax(&a.tstats[5], (unsigned long long)s_cnt);
+        atomicAdd(&a.tstats[6], (unsigned long long)(l_end - T.start_level + 1));
+      }
+      if (lane == 0) atomicAdd(&a.tstats[7], (unsigned long long)n_active);
     }
     for (int o = 32; o > 0; o >>= 1) stage_bytes += __shfl_xor(stage_bytes, o);
     bytes += stage_bytes;
@@ -556,6 +600,278 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
     if (acc) atomicOr(&acc_lds[lane], (unsigned long long)acc);
     __syncthreads();
     if (tid < s.active) a.out[s.out] = acc_lds[tid];
+    __syncthreads();
+  }
+}
+
+// Event form with flattened work. Per pass over a level chunk the workgroup
+//  A. scans the chunk's pending flags and stages each pending node (record,
+//     fanin masks, active lanes = lanes with a differing fanin or an
+//     injection), numbering nodes and their (node, active lane) pairs in one
+//     LDS counter;
+//  B. evaluates every (node, active lane) pair, one per thread: lanes where
+//     no fanin differs keep the good value and are never touched;
+//  C. for the nodes that differ somewhere, writes the mask record and numbers
+//     their (node, live fanout) pairs;
+//  D. marks every such fanout pending, one per thread.
+// A level thus costs a few dependent memory round trips however many nodes
+// and lanes it has. Injection bits are passed through the site's own mask
+// record (epoch with kInjFlag), written at tile start.
+constexpr int kPairChunk = 512;
+constexpr uint32_t kInjFlag = 0x80000000u;
+
+// Position of the r-th set bit (r < popcount) of m.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+  int pos = 0;
+#pragma unroll
+  for (int half = 32; half > 0; half >>= 1) {
+    const uint64_t low = m & ((uint64_t(1) << half) - 1);
+    const int c = __popcll(low);
+    if (r >= c) {
+      r -= c;
+      m >>= half;
+      pos += half;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// Largest k in [0, n) with pref[k] <= q (pref ascending, pref[0] == 0 <= q).
+__device__ __forceinline__ int find_range(const int *pref, int n, int q) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= q) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Wave-wide exclusive prefix of a packed (count << 32 | amount) value; the
+// last lane adds the wave's total to *counter and the base is added back.
+__device__ __forceinline__ uint64_t wave_claim(uint64_t x, int lane, unsigned long long *counter) {
+  uint64_t inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  uint64_t base = 0;
+  if (lane == 63 && inc) base = atomicAdd(counter, (unsigned long long)inc);
+  base = __shfl(base, 63);
+  return base + inc - x;
+}
+
+template <bool kStats>
+__global__ void __launch_bounds__(kPairChunk) k_sweep_pairs(SweepArgs a) {
+  __shared__ unsigned long long s_acc[64], s_lmask[64];
+  __shared__ int s_lsite[64], s_lword[64], s_lkind[64];
+  __shared__ int s_u[kPairChunk];
+  __shared__ uint32_t s_info[kPairChunk];
+  __shared__ int4 s_fin[kPairChunk];
+  __shared__ unsigned long long s_m[kPairChunk][4];
+  __shared__ unsigned long long s_act[kPairChunk], s_nm[kPairChunk];
+  __shared__ int s_pref[kPairChunk], s_wfo0[kPairChunk], s_epref[kPairChunk];
+  __shared__ int s_tile, s_maxl;
+  __shared__ unsigned long long s_cnt2, s_wcnt2;  // (nodes << 32 | pairs), (written << 32 | fanouts)
+  extern __shared__ int s_lvlcnt[];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  uint64_t *rows = a.rows + int64_t(blockIdx.x) * a.rows_stride;
+  MaskRec *mrec = a.mrec + int64_t(blockIdx.x) * a.rec_stride;
+  uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.pend_stride;
+  for (int i = tid; i < a.lvl_entries; i += blockDim.x) s_lvlcnt[i] = 0;
+  if (tid == 0) {
+    s_cnt2 = 0;
+    s_wcnt2 = 0;
+  }
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
+    if (t >= a.ntiles) return;
+    const Tile T = a.tiles[t];
+    const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
+    const uint32_t epoch = a.epoch_base + uint32_t(t) + 1u;
+    const int64_t RW = J.RW;
+    const uint64_t *good = J.good;
+    uint32_t bytes = 0;  // per thread and tile
+    uint32_t n_lv = 0, n_pass = 0, n_staged = 0, n_written = 0, n_pairs = 0;
+    if (tid < 64) {
+      const bool valid = lane < T.nlanes;
+      const Lane L = a.lanes[T.lane0 + (valid ? lane : 0)];
+      const int site = valid ? L.site : -1;
+      s_lsite[lane] = site;
+      s_lword[lane] = L.word;
+      s_lkind[lane] = int(L.kind);
+      s_lmask[lane] = L.mask;
+      s_acc[lane] = 0;
+      // Lanes injecting at the same site share one record.
+      uint64_t im = 0;
+      for (int j = 0; j < 64; ++j) im |= (__shfl(site, j) == site) ? (uint64_t(1) << j) : 0;
+      if (valid) {
+        MaskRec rec;
+        rec.mask = im;
+        rec.epoch = epoch | kInjFlag;
+        rec.pad = 0;
+        mrec[site] = rec;
+        pend[site] = epoch;
+        atomicAdd(&s_lvlcnt[L.level], 1);
+      }
+    }
+    if (tid == 0) s_maxl = T.end_level;
+    __syncthreads();
+    int l_end = T.start_level;
+    for (int l = T.start_level; l <= s_maxl; ++l) {
+      l_end = l;
+      if (s_lvlcnt[l] == 0) continue;
+      ++n_lv;
+      const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
+      for (int c0 = lo; c0 < hi; c0 += kPairChunk) {
+        ++n_pass;
+        // A. scan and stage
+        if (tid == 0) s_wcnt2 = 0;
+        const int u = c0 + tid;
+        const bool p = u < hi && pend[u] == epoch;
+        uint64_t act = 0;
+        uint32_t inf = 0;
+        int4 f = make_int4(0, 0, 0, 0);
+        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        if (p) {
+          inf = J.info[u];
+          f = J.fin[u];
+          const MaskRec own = mrec[u];
+          const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
+          if (type != G_PI) {
+            m0 = stamped_mask(mrec, f.x, epoch);
+            if (nin > 1) m1 = stamped_mask(mrec, f.y, epoch);
+            if (nin > 2) m2 = stamped_mask(mrec, f.z, epoch);
+            if (nin > 3) m3 = stamped_mask(mrec, f.w, epoch);
+          }
+          act = m0 | m1 | m2 | m3 | (own.epoch == (epoch | kInjFlag) ? own.mask : 0);
+          bytes += 40 + (type != G_PI ? 16 * nin : 0);
+        }
+        bytes += 4;
+        const uint64_t slot = wave_claim(p ? ((uint64_t(1) << 32) | uint64_t(__popcll(act))) : 0, lane, &s_cnt2);
+        if (p) {
+          const int k = int(slot >> 32);
+          s_u[k] = u;
+          s_info[k] = inf;
+          s_fin[k] = f;
+          s_m[k][0] = m0;
+          s_m[k][1] = m1;
+          s_m[k][2] = m2;
+          s_m[k][3] = m3;
+          s_act[k] = act;
+          s_nm[k] = 0;
+          s_pref[k] = int(uint32_t(slot));
+        }
+        __syncthreads();
+        const uint64_t c2 = s_cnt2;
+        const int n = int(c2 >> 32), P = int(uint32_t(c2));
+        // B. (node, active lane) pairs
+#pragma unroll 1
+        for (int q = tid; q < P; q += kPairChunk) {
+          const int k = find_range(s_pref, n, q);
+          const int ln = nth_set_bit(s_act[k], q - s_pref[k]);
+          const int v = s_u[k];
+          const uint32_t vi = s_info[k];
+          const uint32_t type = vi & 15u, nin = (vi >> 4) & 7u;
+          const int4 fv = s_fin[k];
+          const int64_t w = s_lword[ln];
+          const uint64_t gu = good[int64_t(v) * RW + w];
+          uint64_t r = gu;
+          if (type != G_PI) {
+            const uint64_t v0 = ((s_m[k][0] >> ln) & 1) ? rows[int64_t(fv.x) * 64 + ln] : good[int64_t(fv.x) * RW + w];
+            uint64_t v1 = 0, v2 = 0, v3 = 0;
+            if (nin > 1)
+              v1 = ((s_m[k][1] >> ln) & 1) ? rows[int64_t(fv.y) * 64 + ln] : good[int64_t(fv.y) * RW + w];
+            if (nin > 2)
+              v2 = ((s_m[k][2] >> ln) & 1) ? rows[int64_t(fv.z) * 64 + ln] : good[int64_t(fv.z) * RW + w];
+            if (nin > 3)
+              v3 = ((s_m[k][3] >> ln) & 1) ? rows[int64_t(fv.w) * 64 + ln] : good[int64_t(fv.w) * RW + w];
+            r = eval_gate(type, nin, v0, v1, v2, v3);
+          }
+          if (s_lsite[ln] == v) {
+            const uint32_t kind = uint32_t(s_lkind[ln]);
+            r = kind == K_FLIP ? (gu ^ s_lmask[ln]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+          }
+          const uint64_t d = r ^ gu;
+          bytes += 8 * (type == G_PI ? 1 : nin + 1);
+          if (d) {
+            rows[int64_t(v) * 64 + ln] = r;
+            atomicOr(&s_nm[k], 1ull << ln);
+            if (type == G_PO) atomicOr(&s_acc[ln], (unsigned long long)d);
+            bytes += 8;
+          }
+          if (kStats) ++n_pairs;
+        }
+        __syncthreads();
+        // C. nodes that differ somewhere: record, deepest fanout level, fanout pairs
+        if (tid == 0) s_cnt2 = 0;
+        int e0 = 0, deg = 0;
+        bool wr = false;
+        if (tid < n) {
+          const uint64_t nm = s_nm[tid];
+          if (nm) {
+            wr = true;
+            const int v = s_u[tid];
+            MaskRec rec;
+            rec.mask = nm;
+            rec.epoch = epoch;
+            rec.pad = 0;
+            mrec[v] = rec;
+            e0 = J.fo_off[v];
+            deg = J.fo_off[v + 1] - e0;
+            if (deg) atomicMax(&s_maxl, J.fo[e0 + deg - 1].y);
+            bytes += 32;
+          }
+        }
+        const uint64_t wslot = wave_claim(wr ? ((uint64_t(1) << 32) | uint64_t(deg)) : 0, lane, &s_wcnt2);
+        if (wr) {
+          const int k = int(wslot >> 32);
+          s_wfo0[k] = e0;
+          s_epref[k] = int(uint32_t(wslot));
+        }
+        __syncthreads();
+        const uint64_t w2 = s_wcnt2;
+        const int nw = int(w2 >> 32), E = int(uint32_t(w2));
+        // D. mark fanouts pending
+#pragma unroll 1
+        for (int e = tid; e < E; e += kPairChunk) {
+          const int k = find_range(s_epref, nw, e);
+          const int2 fo = J.fo[s_wfo0[k] + (e - s_epref[k])];
+          pend[fo.x] = epoch;
+          atomicAdd(&s_lvlcnt[fo.y], 1);
+          bytes += 12;
+        }
+        if (kStats) {
+          n_staged += n;
+          n_written += nw;
+        }
+        __syncthreads();
+      }
+      if (tid == 0) s_lvlcnt[l] = 0;
+    }
+    if (tid < T.nlanes) a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+    unsigned long long wbytes = bytes;
+    for (int o = 32; o > 0; o >>= 1) wbytes += __shfl_xor(wbytes, o);
+    if (lane == 0 && wbytes) atomicAdd(a.bytes, wbytes);
+    if (kStats) {
+      unsigned long long np = n_pairs;
+      for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o);
+      if (lane == 0) atomicAdd(&a.tstats[7], np);
+      if (tid == 0) {
+        atomicAdd(&a.tstats[0], 1ull);
+        atomicAdd(&a.tstats[1], (unsigned long long)n_lv);
+        atomicAdd(&a.tstats[2], (unsigned long long)n_pass);
+        atomicAdd(&a.tstats[3], (unsigned long long)n_staged);
+        atomicAdd(&a.tstats[4], (unsigned long long)n_written);
+        atomicMax(&a.tstats[5], (unsigned long long)n_written);
+        atomicAdd(&a.tstats[6], (unsigned long long)(l_end - T.start_level + 1));
+      }
+    }
     __syncthreads();
   }
 }
@@ -653,6 +969,17 @@ __global__ void k_detect(ItemArgs a) {
   record(J, f, w, local_term(J, f, w) & D);
 }
 
+// First detecting rounds of the step's faults, packed per dropping job at
+// foff (jobs without a first_round table own an empty range).
+__global__ void k_gather_first(const JobDev *__restrict__ jobs, int njobs, const int64_t *__restrict__ foff,
+                               int32_t *__restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= foff[njobs]) return;
+  const int jb = find_job(foff, njobs, t);
+  const JobDev &J = jobs[jb];
+  out[t] = J.first_round[J.flist[t - foff[jb]]];
+}
+
 // units[i] = {job, stem position, unit index}: head_slot of that job maps the
 // stem to its unit index within the job (or back to -1).
 __global__ void k_set_slots(const JobDev *__restrict__ jobs, const int4 *__restrict__ units, int n, int clear) {
@@ -688,7 +1015,63 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Page-locked host staging buffer (grows, never shrinks).
+template <typename T>
+struct HostBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    LSIM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf() { release(); }
+};
+
 inline unsigned blocks_for(int64_t threads, int bs) { return unsigned((threads + bs - 1) / bs); }
+
+// Runs fn(order[k]) for k in [0, n) on up to 16 host threads (items are
+// handed out in order, so put the largest first); rethrows the first error.
+template <typename Fn>
+void parallel_for(const std::vector<int> &order, Fn &&fn) {
+  const int n = int(order.size());
+  const int hw = int(std::max(1u, std::thread::hardware_concurrency()));
+  const int T = std::min(n, std::min(16, hw));
+  if (T <= 1) {
+    for (int k = 0; k < n; ++k) fn(order[k]);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::exception_ptr err;
+  std::atomic<bool> failed{false};
+  auto body = [&] {
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= n || failed.load()) break;
+      try {
+        fn(order[k]);
+      } catch (...) {
+        if (!failed.exchange(true)) err = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  th.reserve(size_t(T - 1));
+  for (int t = 1; t < T; ++t) th.emplace_back(body);
+  body();
+  for (auto &x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
 
 }  // namespace
 
@@ -721,7 +1104,7 @@ struct JobState {
   std::vector<uint8_t> sa;
   std::vector<int32_t> live, alive;
   DevBuf<uint64_t> good, pi_words, detbits;
-  DevBuf<int32_t> d_site, d_head, first, head_slot, flist;
+  DevBuf<int32_t> d_site, d_head, first, head_slot;
   DevBuf<uint8_t> d_sa, last;
   bool started = false, done = false, last_done = false;
   int64_t w0 = 0;
@@ -742,15 +1125,18 @@ struct Engine::Impl {
   std::vector<std::unique_ptr<JobState>> jobs;
   // per iteration
   DevBuf<JobDev> d_jobs;
-  DevBuf<int64_t> d_item_off, d_good_off;
+  DevBuf<int64_t> d_item_off, d_good_off, d_fault_off;
+  DevBuf<int32_t> d_flist, d_first_out;
+  HostBuf<int32_t> h_flist, h_first_out;
+  HostBuf<uint64_t> h_need;
+  HostBuf<Lane> h_lanes;
+  HostBuf<unsigned long long> h_bytes;
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
   DevBuf<int4> d_units;
   DevBuf<int> d_queue;
-  DevBuf<unsigned long long> d_bytes;
-  std::vector<uint8_t> head_mark;
-  std::vector<int32_t> count_buf;
+  DevBuf<unsigned long long> d_bytes, d_tstats;
   // sweep slots
   DevBuf<uint64_t> d_rows, d_masks;
   DevBuf<MaskRec> d_mrec;
@@ -784,9 +1170,9 @@ struct Engine::Impl {
   bool plan(JobState &j);
   void finalize(JobState &j);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
-  void sweep(const EngineOptions &eo, const std::vector<Lane> &lanes, const std::vector<Tile> &tiles,
+  void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
              int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st);
-  void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max);
+  void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu);
 };
 
 Engine::Engine(int device) : impl_(new Impl) {
@@ -979,7 +1365,6 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   j.d_head.alloc(F1);
   j.d_sa.alloc(F1);
   j.first.alloc(F1);
-  j.flist.alloc(F1);
   j.head_slot.alloc(std::max<int64_t>(c->NL, 1));
   if (!opt.drop) j.detbits.alloc(size_t(std::max<int64_t>(j.F * j.RB, 1)));
   if (opt.drop && opt.report_last_round) j.last.alloc(F1);
@@ -1170,7 +1555,7 @@ void Engine::Impl::finalize(JobState &j) {
   j.done = true;
 }
 
-void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max) {
+void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu) {
   // Per slot: faulty rows [NL][64], and the plain form's lane masks or the
   // event form's mask records and pending flags.
   const int64_t nl = std::max<int64_t>(nl_max, 1);
@@ -1185,9 +1570,9 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
     budget = (int64_t(fr) + held) / 10 * 7;
   }
   const int64_t cap = std::max<int64_t>(1, budget / per_slot);
-  // Up to three resident 8-wave workgroups per CU (the event sweep's register
-  // use allows 6 waves per SIMD).
-  const int64_t s = std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * 3)));
+  // No more slots than workgroups the device keeps resident at once.
+  const int64_t s =
+      std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(cap, int64_t(cus) * std::max(1, per_cu))));
   if (s > slots || nl > slot_nl || !d_rows.p) {
     const int64_t ns = std::min(cap, std::max(s, slots));
     d_rows.release();
@@ -1206,15 +1591,26 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
   }
 }
 
-void Engine::Impl::sweep(const EngineOptions &eo, const std::vector<Lane> &lanes, const std::vector<Tile> &tiles,
+void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
                          int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st) {
   const int64_t ntiles = int64_t(tiles.size());
   if (!ntiles) return;
-  d_lanes.alloc(lanes.size());
+  d_lanes.alloc(size_t(nlanes));
   d_tiles.alloc(tiles.size());
-  h2d(d_lanes.p, lanes.data(), lanes.size() * sizeof(Lane));
+  h2d(d_lanes.p, lanes, size_t(nlanes) * sizeof(Lane));
   h2d(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile));
-  ensure_slots(eo, ntiles, nl_max);
+  const size_t lds = size_t(lvl_entries) * sizeof(int);
+  const int bw = std::max(1, std::min(kChunk / 64, eo.block_waves));
+  int per_cu = 3;
+  if (!plain_circuit) {
+    if (eo.sweep_kernel == 1)
+      LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_events<false>, 64 * bw, lds));
+    else
+      LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pairs<false>, kPairChunk, lds));
+    if (per_cu < 1) throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) +
+                                                         " B of level counters)");
+  }
+  ensure_slots(eo, ntiles, nl_max, per_cu);
   SweepArgs a;
   a.jobs = d_jobs.p;
   a.tiles = d_tiles.p;
@@ -1234,6 +1630,13 @@ void Engine::Impl::sweep(const EngineOptions &eo, const std::vector<Lane> &lanes
   a.rec_stride = slot_nl;
   a.pend_stride = slot_nl;
   a.epoch_base = 0;
+  a.tstats = nullptr;
+  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
+  if (trace) {
+    d_tstats.alloc(8);
+    LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, 8 * sizeof(unsigned long long), stream));
+    a.tstats = d_tstats.p;
+  }
   st.sweep_tiles += ntiles;
   time_begin();
   if (!plain_circuit) {
@@ -1247,9 +1650,18 @@ void Engine::Impl::sweep(const EngineOptions &eo, const std::vector<Lane> &lanes
     a.epoch_base = epoch_base;
     epoch_base += uint32_t(ntiles);
     LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
-    const int bw = std::max(1, std::min(kChunk / 64, eo.block_waves));
-    const unsigned grid = unsigned(std::min<int64_t>(slots, ntiles));
-    k_sweep_events<<<grid, 64 * bw, size_t(lvl_entries) * sizeof(int), stream>>>(a);
+    const unsigned grid = unsigned(std::min<int64_t>(std::min<int64_t>(slots, ntiles), int64_t(cus) * per_cu));
+    if (eo.sweep_kernel == 1) {
+      if (a.tstats)
+        k_sweep_events<true><<<grid, 64 * bw, lds, stream>>>(a);
+      else
+        k_sweep_events<false><<<grid, 64 * bw, lds, stream>>>(a);
+    } else {
+      if (a.tstats)
+        k_sweep_pairs<true><<<grid, kPairChunk, lds, stream>>>(a);
+      else
+        k_sweep_pairs<false><<<grid, kPairChunk, lds, stream>>>(a);
+    }
     LSIM_HIP(hipGetLastError());
     st.sweep_launches++;
   } else {
@@ -1270,48 +1682,72 @@ void Engine::Impl::sweep(const EngineOptions &eo, const std::vector<Lane> &lanes
     }
   }
   st.t_sweep_ms += time_end();
+  if (trace && !plain_circuit) {
+    unsigned long long ts[8];
+    d2h(ts, d_tstats.p, sizeof(ts));
+    LSIM_HIP(hipStreamSynchronize(stream));
+    const double nt = double(std::max<unsigned long long>(ts[0], 1));
+    std::fprintf(stderr,
+                 "[lsim] tiles %llu: per tile levels walked %.1f, pending levels %.1f, scan passes %.1f, staged %.1f, "
+                 "written %.1f (max %llu), active lanes per staged node %.2f\n",
+                 ts[0], ts[6] / nt, ts[1] / nt, ts[2] / nt, ts[3] / nt, ts[4] / nt, ts[5],
+                 double(ts[7]) / double(std::max<unsigned long long>(ts[3], 1)));
+  }
 }
 
 void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOptions &eo) {
   const int nj = int(act.size());
+  using clk = std::chrono::steady_clock;
+  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
+  const auto h0 = clk::now();
   std::vector<JobDev> devs(nj);
   std::vector<int64_t> item_off(nj + 1, 0);
   std::vector<int64_t> word0(nj + 1, 0);
+  std::vector<int64_t> flist_off(nj + 1, 0);  // step faults of job i at [flist_off[i], flist_off[i+1])
+  std::vector<int64_t> fault_off(nj + 1, 0);  // dropping jobs' faults in the first-round gather
   std::vector<int4> units_all;
   int lvl_entries = 1;
   int64_t nl_max = 1;
   lsim_run_stats st{};
+  // Largest jobs first when the per-job host work runs in parallel.
+  std::vector<int> order(nj);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return act[x]->step_faults.size() > act[y]->step_faults.size(); });
   // Per job: the step's faults in unit order and its units (stems of the
   // faults' regions, or the fault sites themselves for direct injection).
-  for (int i = 0; i < nj; ++i) {
+  parallel_for(order, [&](int i) {
     JobState &j = *act[i];
     const CircuitRes &c = *j.c;
     j.units.clear();
     if (j.direct) {
-      count_buf.assign(size_t(c.NL) + 1, 0);
-      for (int32_t f : j.step_faults) count_buf[j.site[f] + 1]++;
-      for (int64_t u = 0; u < c.NL; ++u) count_buf[u + 1] += count_buf[u];
+      std::vector<int32_t> cnt(size_t(c.NL) + 1, 0);
+      for (int32_t f : j.step_faults) cnt[j.site[f] + 1]++;
+      for (int64_t u = 0; u < c.NL; ++u) cnt[u + 1] += cnt[u];
       std::vector<int32_t> sorted(j.step_faults.size());
-      for (int32_t f : j.step_faults) sorted[count_buf[j.site[f]]++] = f;
+      for (int32_t f : j.step_faults) sorted[cnt[j.site[f]]++] = f;
       j.step_faults.swap(sorted);
       j.units.resize(j.step_faults.size());
       for (size_t k = 0; k < j.step_faults.size(); ++k) j.units[k] = j.site[j.step_faults[k]];
     } else {
-      head_mark.assign(size_t(c.NL), 0);
+      std::vector<uint8_t> mark(size_t(c.NL), 0);
       for (int32_t f : j.step_faults) {
         const int32_t h = j.head[f];
-        if (!c.head_is_po[h]) head_mark[h] = 1;
+        if (!c.head_is_po[h]) mark[h] = 1;
       }
       for (int64_t u = 0; u < c.NL; ++u)
-        if (head_mark[u]) {
-          units_all.push_back(make_int4(i, int(u), int(j.units.size()), 0));
-          j.units.push_back(int32_t(u));
-        }
+        if (mark[u]) j.units.push_back(int32_t(u));
     }
+  });
+  std::vector<int64_t> unit_off(nj + 1, 0);  // stem units of job i in units_all
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    const CircuitRes &c = *j.c;
     const int64_t nf = int64_t(j.step_faults.size());
-    h2d(j.flist.p, j.step_faults.data(), nf * sizeof(int32_t));
+    flist_off[i + 1] = flist_off[i] + nf;
+    fault_off[i + 1] = fault_off[i] + (j.kind == JobState::DROP ? nf : 0);
+    unit_off[i + 1] = unit_off[i] + (j.direct ? 0 : int64_t(j.units.size()));
     JobDev d = job_dev(j);
-    d.flist = j.flist.p;
     d.nf = nf;
     d.w0 = j.step_w0;
     d.Wn = j.step_Wn;
@@ -1327,10 +1763,27 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     nl_max = std::max(nl_max, c.NL);
   }
   const int64_t items = item_off[nj], words = std::max<int64_t>(word0[nj], 1);
+  // All jobs' step faults in one page-locked block, one copy.
+  const int64_t nf_all = flist_off[nj];
+  h_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
+  d_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
+  units_all.resize(size_t(unit_off[nj]));
+  parallel_for(order, [&](int i) {
+    const JobState &j = *act[i];
+    std::memcpy(h_flist.p + flist_off[i], j.step_faults.data(), j.step_faults.size() * sizeof(int32_t));
+    if (!j.direct)
+      for (size_t k = 0; k < j.units.size(); ++k)
+        units_all[size_t(unit_off[i]) + k] = make_int4(i, j.units[k], int(k), 0);
+  });
+  for (int i = 0; i < nj; ++i) devs[i].flist = d_flist.p + flist_off[i];
+  h2d(d_flist.p, h_flist.p, size_t(nf_all) * sizeof(int32_t));
   d_jobs.alloc(nj);
   d_item_off.alloc(nj + 1);
+  d_fault_off.alloc(nj + 1);
   h2d(d_jobs.p, devs.data(), nj * sizeof(JobDev));
   h2d(d_item_off.p, item_off.data(), (nj + 1) * sizeof(int64_t));
+  h2d(d_fault_off.p, fault_off.data(), (nj + 1) * sizeof(int64_t));
+  const auto h1 = clk::now();
   d_need.alloc(words);
   d_dout.alloc(words);
   LSIM_HIP(hipMemsetAsync(d_need.p, 0, words * sizeof(uint64_t), stream));
@@ -1354,34 +1807,54 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     LSIM_HIP(hipGetLastError());
   }
   // The needed bits decide which (unit, word) pairs become sweep lanes.
-  std::vector<uint64_t> need(static_cast<size_t>(word0[nj]));
-  d2h(need.data(), d_need.p, need.size() * sizeof(uint64_t));
+  h_need.alloc(size_t(words));
+  const uint64_t *need = h_need.p;
+  d2h(h_need.p, d_need.p, size_t(word0[nj]) * sizeof(uint64_t));
   st.t_detect_ms += time_end();
-  std::vector<Lane> lanes;
-  std::vector<Tile> tiles;
-  for (int i = 0; i < nj; ++i) {
-    JobState &j = *act[i];
-    const size_t first_lane = lanes.size();
+  const auto h2 = clk::now();
+  // Lanes: one per (unit, word) with needed bits, per job in unit order
+  // (units are level-sorted), counted then filled in parallel.
+  std::vector<int64_t> lane_off(nj + 1, 0);
+  parallel_for(order, [&](int i) {
+    int64_t n = 0;
+    for (int64_t o = word0[i]; o < word0[i + 1]; ++o) n += need[o] != 0;
+    lane_off[i + 1] = n;
+  });
+  for (int i = 0; i < nj; ++i) lane_off[i + 1] += lane_off[i];
+  const int64_t nlanes = lane_off[nj];
+  h_lanes.alloc(size_t(std::max<int64_t>(nlanes, 1)));
+  Lane *lanes = h_lanes.p;
+  parallel_for(order, [&](int i) {
+    const JobState &j = *act[i];
     const int Wn = j.step_Wn;
-    for (size_t k = 0; k < j.units.size(); ++k)
+    Lane *out = lanes + lane_off[i];
+    for (size_t k = 0; k < j.units.size(); ++k) {
+      const int32_t u = j.units[k];
+      const int32_t kind = j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP;
+      const int32_t level = j.c->level_dp[u];
       for (int jw = 0; jw < Wn; ++jw) {
         const int64_t o = word0[i] + int64_t(k) * Wn + jw;
-        const uint64_t msk = need[size_t(o)];
+        const uint64_t msk = need[o];
         if (!msk) continue;
         Lane ln;
         ln.mask = msk;
-        ln.site = j.units[k];
+        ln.site = u;
         ln.word = int32_t(j.step_w0 + jw);
         ln.out = int32_t(o);
-        ln.kind = j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP;
-        ln.level = j.c->level_dp[j.units[k]];
+        ln.kind = kind;
+        ln.level = level;
         ln.pad = 0;
-        lanes.push_back(ln);
+        *out++ = ln;
       }
-    for (size_t l0 = first_lane; l0 < lanes.size(); l0 += 64) {
+    }
+  });
+  std::vector<Tile> tiles;
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    for (int64_t l0 = lane_off[i]; l0 < lane_off[i + 1]; l0 += 64) {
       Tile t;
       t.lane0 = int32_t(l0);
-      t.nlanes = int32_t(std::min<size_t>(64, lanes.size() - l0));
+      t.nlanes = int32_t(std::min<int64_t>(64, lane_off[i + 1] - l0));
       t.start_level = lanes[l0].level;
       t.end_level = lanes[l0 + t.nlanes - 1].level;
       t.job = i;
@@ -1394,7 +1867,9 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     return (devs[x.job].max_level - x.start_level) > (devs[y.job].max_level - y.start_level);
   });
   const bool plain = eo.mode == LSIM_MODE_PLAIN;
-  sweep(eo, lanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, st);
+  const auto h3 = clk::now();
+  sweep(eo, lanes, nlanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, st);
+  const auto h4 = clk::now();
   time_begin();
   if (items) {
     k_detect<<<blocks_for(items, 256), 256, 0, stream>>>(ia);
@@ -1404,17 +1879,22 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, d_units.p, nu_all, 1);
     LSIM_HIP(hipGetLastError());
   }
-  unsigned long long bytes = 0;
-  d2h(&bytes, d_bytes.p, sizeof(bytes));
+  // One copy of every dropping job's first rounds for the step's faults.
+  const int64_t ng = fault_off[nj];
+  if (ng) {
+    d_first_out.alloc(size_t(ng));
+    h_first_out.alloc(size_t(ng));
+    k_gather_first<<<blocks_for(ng, 256), 256, 0, stream>>>(d_jobs.p, nj, d_fault_off.p, d_first_out.p);
+    LSIM_HIP(hipGetLastError());
+    d2h(h_first_out.p, d_first_out.p, size_t(ng) * sizeof(int32_t));
+  }
+  h_bytes.alloc(1);
+  d2h(h_bytes.p, d_bytes.p, sizeof(unsigned long long));
   LSIM_HIP(hipMemsetAsync(d_bytes.p, 0, sizeof(unsigned long long), stream));
   st.t_detect_ms += time_end();
-  st.sweep_bytes = int64_t(bytes);
-  if (std::getenv("LSIM_TRACE"))
-    std::fprintf(stderr, "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld sweep=%.2fms\n", nj,
-                 (long long)items, (long long)word0[nj], (long long)lanes.size(), (long long)tiles.size(),
-                 st.t_sweep_ms);
+  st.sweep_bytes = int64_t(h_bytes.p[0]);
+  const auto h5 = clk::now();
   // Advance every job.
-  std::vector<int32_t> first;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
     lsim_run_stats &js = j.res.stats;
@@ -1428,13 +1908,11 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     switch (j.kind) {
       case JobState::MATRIX: j.w0 += j.step_Wn; break;
       case JobState::DROP: {
-        first.resize(size_t(j.F));
-        d2h(first.data(), j.first.p, j.F * sizeof(int32_t));
-        LSIM_HIP(hipStreamSynchronize(stream));
+        const int32_t *fr = h_first_out.p + fault_off[i];
         std::vector<int32_t> next;
-        next.reserve(j.alive.size());
-        for (int32_t f : j.alive)
-          if (first[f] == INT32_MAX) next.push_back(f);
+        next.reserve(j.step_faults.size());
+        for (size_t k = 0; k < j.step_faults.size(); ++k)
+          if (fr[k] == INT32_MAX) next.push_back(j.step_faults[k]);
         j.alive.swap(next);
         j.w0 += j.step_Wn;
         j.Wt = std::min(64, j.Wt * 2);
@@ -1442,6 +1920,14 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       }
       case JobState::LAST: j.last_done = true; break;
     }
+  }
+  if (trace) {
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    std::fprintf(stderr,
+                 "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld | prep %.2f local %.2f "
+                 "lanes %.2f sweep %.2f (dev %.2f) detect %.2f advance %.2f ms\n",
+                 nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)tiles.size(),
+                 ms(h0, h1), ms(h1, h2), ms(h2, h3), ms(h3, h4), st.t_sweep_ms, ms(h4, h5), ms(h5, clk::now()));
   }
 }
 

@@ -35,7 +35,8 @@ struct JobOptions {
 struct EngineOptions {
   int mode = LSIM_MODE_PARTITIONED;  // PLAIN (per-level launches) or PARTITIONED (persistent events)
   int64_t scratch_mb = 0;            // sweep scratch budget, 0 = automatic
-  int block_waves = 8;               // waves per workgroup of the persistent sweep (1..8)
+  int block_waves = 8;               // waves per workgroup of the node-wave persistent sweep (1..8)
+  int sweep_kernel = 0;              // persistent sweep: 0 = flattened (node, lane) pairs, 1 = one node per wave
 };
 
 struct RunResults {

@@ -45,8 +45,9 @@ def _texts(spec, pattern_seed=None, fraction=None):
     return ckt, flst, ptn
 
 
-def test_batch_matches_single_runs():
-    b = lsim_amd.Batch(device=0)
+@pytest.mark.parametrize("kernel", [0, 1], ids=["pairs", "nodewave"])
+def test_batch_matches_single_runs(kernel):
+    b = lsim_amd.Batch(device=0, sweep_kernel=kernel)
     jobs = []
     for i, spec in enumerate(SPECS):
         ckt, flst, ptn = _texts(spec)

@@ -13,13 +13,16 @@ from lsim_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-MODES = [lsim_amd.Simulator.Mode.PLAIN, lsim_amd.Simulator.Mode.PARTITIONED]
-MODE_IDS = ["plain", "partitioned"]
+# Engines: the per-level form, and the persistent event form with each of its
+# two sweep kernels (flattened (node, lane) pairs; one node per wave).
+MODES = [(lsim_amd.Simulator.Mode.PLAIN, {}), (lsim_amd.Simulator.Mode.PARTITIONED, {}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"sweep_kernel": 1})]
+MODE_IDS = ["plain", "pairs", "nodewave"]
 
 
-def _sim(ckt, flst, ptn, **opts):
+def _sim(ckt, flst, ptn, mode=None, **opts):
     s = lsim_amd.Simulator(device=0)
-    for k, v in opts.items():
+    for k, v in dict(mode[1] if mode else {}, **opts).items():
         s.set_option(k, v)
     s.read_text(ckt, flst, ptn)
     return s
@@ -29,8 +32,8 @@ def _sim(ckt, flst, ptn, **opts):
 def test_reference_unit_vectors(unit_vectors, mode):
     """unittests/test_basics.cpp:528-605: good values and fault table."""
     for c in unit_vectors:
-        s = _sim(c["ckt"], c["flst"], c["ptn"])
-        s.run(mode, 4)
+        s = _sim(c["ckt"], c["flst"], c["ptn"], mode)
+        s.run(mode[0], 4)
         pi, gate, po = s.good_results()
         assert list(pi) == c["pi"], c["name"]
         assert list(gate) == c["gate"], c["name"]
@@ -43,8 +46,8 @@ def test_reference_unit_vectors(unit_vectors, mode):
 def test_reference_made_vectors(ref_vectors, mode, direct):
     """Per-round detection and report against vectors written by the reference."""
     for c in ref_vectors:
-        s = _sim(c["ckt"], c["flst"], c["ptn"], direct=direct)
-        s.run(mode)
+        s = _sim(c["ckt"], c["flst"], c["ptn"], mode, direct=direct)
+        s.run(mode[0])
         F, R = s.num_faults(), s.num_rounds()
         det = np.array(c["det"], dtype=np.uint8).reshape(F, R)
         assert np.array_equal(s.detect_matrix(), det), c["name"]
@@ -68,8 +71,8 @@ def c1():
 def test_c1_full_against_oracle(c1, mode, direct):
     """5k gates, 10k faults, 16 rounds: every d(f, r) against the literal restatement."""
     ckt, flst, ptn, ref = c1
-    s = _sim(ckt, flst, ptn, direct=direct)
-    s.run(mode)
+    s = _sim(ckt, flst, ptn, mode, direct=direct)
+    s.run(mode[0])
     assert np.array_equal(s.detect_matrix(), ref["det"])
     assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
     pi, gate, po = s.good_results()
@@ -80,8 +83,8 @@ def test_c1_full_against_oracle(c1, mode, direct):
 def test_c1_fault_dropping(c1, mode):
     """With dropping, the first detecting round and the detected set equal the full matrix's."""
     ckt, flst, ptn, ref = c1
-    s = _sim(ckt, flst, ptn, drop=1, report_last_round=1)
-    s.run(mode)
+    s = _sim(ckt, flst, ptn, mode, drop=1, report_last_round=1)
+    s.run(mode[0])
     det = ref["det"]
     want = np.where(det.any(1), det.argmax(1), -1)
     assert np.array_equal(s.first_detect_round(), want)
@@ -103,15 +106,17 @@ def test_c2_sampled_against_oracle(c2):
     other on every fault."""
     ckt, flst, ptn = c2
     runs = {}
-    for name, mode, direct in (("plain", MODES[0], 0), ("partitioned", MODES[1], 0), ("direct", MODES[1], 1)):
-        s = _sim(ckt, flst, ptn, direct=direct)
-        s.run(mode)
+    for name, mode, direct in (("plain", MODES[0], 0), ("pairs", MODES[1], 0), ("nodewave", MODES[2], 0),
+                               ("direct", MODES[1], 1)):
+        s = _sim(ckt, flst, ptn, mode, direct=direct)
+        s.run(mode[0])
         runs[name] = s.detect_matrix()
         if name == "plain":
             table = s.found_fault_to_pattern()
             good = s.good_results()
         s.close()
-    assert np.array_equal(runs["plain"], runs["partitioned"])
+    assert np.array_equal(runs["plain"], runs["pairs"])
+    assert np.array_equal(runs["plain"], runs["nodewave"])
     assert np.array_equal(runs["plain"], runs["direct"])
     o = oracle.Oracle(ckt, flst, ptn)
     rng = np.random.default_rng(2002)
