@@ -618,6 +618,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 // and lanes it has. Injection bits are passed through the site's own mask
 // record (epoch with kInjFlag), written at tile start.
 constexpr int kPairChunk = 512;
+constexpr int kPairsPerThread = 2;
 constexpr uint32_t kInjFlag = 0x80000000u;
 
 // Position of the r-th set bit (r < popcount) of m.
@@ -664,7 +665,7 @@ __device__ __forceinline__ uint64_t wave_claim(uint64_t x, int lane, unsigned lo
 }
 
 template <bool kStats>
-__global__ void __launch_bounds__(kPairChunk) k_sweep_pairs(SweepArgs a) {
+__global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu(4))) k_sweep_pairs(SweepArgs a) {
   __shared__ unsigned long long s_acc[64], s_lmask[64];
   __shared__ int s_lsite[64], s_lword[64], s_lkind[64];
   __shared__ int s_u[kPairChunk];
@@ -770,42 +771,67 @@ __global__ void __launch_bounds__(kPairChunk) k_sweep_pairs(SweepArgs a) {
         __syncthreads();
         const uint64_t c2 = s_cnt2;
         const int n = int(c2 >> 32), P = int(uint32_t(c2));
-        // B. (node, active lane) pairs
+        // B. (node, active lane) pairs, kPairsPerThread per thread at a time
+        // with their loads interleaved.
 #pragma unroll 1
-        for (int q = tid; q < P; q += kPairChunk) {
-          const int k = find_range(s_pref, n, q);
-          const int ln = nth_set_bit(s_act[k], q - s_pref[k]);
-          const int v = s_u[k];
-          const uint32_t vi = s_info[k];
-          const uint32_t type = vi & 15u, nin = (vi >> 4) & 7u;
-          const int4 fv = s_fin[k];
-          const int64_t w = s_lword[ln];
-          const uint64_t gu = good[int64_t(v) * RW + w];
-          uint64_t r = gu;
-          if (type != G_PI) {
-            const uint64_t v0 = ((s_m[k][0] >> ln) & 1) ? rows[int64_t(fv.x) * 64 + ln] : good[int64_t(fv.x) * RW + w];
-            uint64_t v1 = 0, v2 = 0, v3 = 0;
-            if (nin > 1)
-              v1 = ((s_m[k][1] >> ln) & 1) ? rows[int64_t(fv.y) * 64 + ln] : good[int64_t(fv.y) * RW + w];
-            if (nin > 2)
-              v2 = ((s_m[k][2] >> ln) & 1) ? rows[int64_t(fv.z) * 64 + ln] : good[int64_t(fv.z) * RW + w];
-            if (nin > 3)
-              v3 = ((s_m[k][3] >> ln) & 1) ? rows[int64_t(fv.w) * 64 + ln] : good[int64_t(fv.w) * RW + w];
-            r = eval_gate(type, nin, v0, v1, v2, v3);
+        for (int q0 = tid; q0 < P; q0 += kPairsPerThread * kPairChunk) {
+          int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread];
+          uint32_t type[kPairsPerThread], nin[kPairsPerThread];
+          int4 fv[kPairsPerThread];
+          int64_t w[kPairsPerThread];
+          uint64_t gu[kPairsPerThread], r[kPairsPerThread], mk[kPairsPerThread][4];
+          bool ok[kPairsPerThread];
+#pragma unroll
+          for (int i = 0; i < kPairsPerThread; ++i) {
+            const int q = q0 + i * kPairChunk;
+            ok[i] = q < P;
+            k[i] = find_range(s_pref, n, ok[i] ? q : q0);
+            ln[i] = nth_set_bit(s_act[k[i]], (ok[i] ? q : q0) - s_pref[k[i]]);
+            v[i] = s_u[k[i]];
+            const uint32_t vi = s_info[k[i]];
+            type[i] = vi & 15u;
+            nin[i] = (vi >> 4) & 7u;
+            fv[i] = s_fin[k[i]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) mk[i][j] = s_m[k[i]][j];
+            w[i] = s_lword[ln[i]];
+            gu[i] = ok[i] ? good[int64_t(v[i]) * RW + w[i]] : 0;
           }
-          if (s_lsite[ln] == v) {
-            const uint32_t kind = uint32_t(s_lkind[ln]);
-            r = kind == K_FLIP ? (gu ^ s_lmask[ln]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+#pragma unroll
+          for (int i = 0; i < kPairsPerThread; ++i) {
+            r[i] = gu[i];
+            if (ok[i] && type[i] != G_PI) {
+              const int b = ln[i];
+              const uint64_t v0 =
+                  ((mk[i][0] >> b) & 1) ? rows[int64_t(fv[i].x) * 64 + b] : good[int64_t(fv[i].x) * RW + w[i]];
+              uint64_t v1 = 0, v2 = 0, v3 = 0;
+              if (nin[i] > 1)
+                v1 = ((mk[i][1] >> b) & 1) ? rows[int64_t(fv[i].y) * 64 + b] : good[int64_t(fv[i].y) * RW + w[i]];
+              if (nin[i] > 2)
+                v2 = ((mk[i][2] >> b) & 1) ? rows[int64_t(fv[i].z) * 64 + b] : good[int64_t(fv[i].z) * RW + w[i]];
+              if (nin[i] > 3)
+                v3 = ((mk[i][3] >> b) & 1) ? rows[int64_t(fv[i].w) * 64 + b] : good[int64_t(fv[i].w) * RW + w[i]];
+              r[i] = eval_gate(type[i], nin[i], v0, v1, v2, v3);
+            }
           }
-          const uint64_t d = r ^ gu;
-          bytes += 8 * (type == G_PI ? 1 : nin + 1);
-          if (d) {
-            rows[int64_t(v) * 64 + ln] = r;
-            atomicOr(&s_nm[k], 1ull << ln);
-            if (type == G_PO) atomicOr(&s_acc[ln], (unsigned long long)d);
-            bytes += 8;
+#pragma unroll
+          for (int i = 0; i < kPairsPerThread; ++i) {
+            if (!ok[i]) continue;
+            const int b = ln[i];
+            if (s_lsite[b] == v[i]) {
+              const uint32_t kind = uint32_t(s_lkind[b]);
+              r[i] = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+            }
+            const uint64_t d = r[i] ^ gu[i];
+            bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
+            if (d) {
+              rows[int64_t(v[i]) * 64 + b] = r[i];
+              atomicOr(&s_nm[k[i]], 1ull << b);
+              if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
+              bytes += 8;
+            }
+            if (kStats) ++n_pairs;
           }
-          if (kStats) ++n_pairs;
         }
         __syncthreads();
         // C. nodes that differ somewhere: record, deepest fanout level, fanout pairs
