@@ -91,10 +91,17 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
             "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
 
 
-def _kernel_name(args) -> str:
+def _kernel_name(args, tuned=None) -> str:
     if args.mode == "plain":
         return "k_sweep_level"
-    return "k_sweep_pairs" if args.sweep_kernel == "pairs" else "k_sweep_events"
+    kernel = (tuned & 0xff) if tuned is not None else (0 if args.sweep_kernel == "pairs" else 1)
+    return "k_sweep_pairs" if kernel == 0 else "k_sweep_events"
+
+
+def _tuned(st: dict) -> dict:
+    t = int(st["tuned"])
+    return {"sweep_kernel": "pairs" if (t & 0xff) == 0 else "nodewave", "first_step_words": (t >> 8) & 0xff,
+            "word_growth": (t >> 16) & 0xff}
 
 
 def bench_batch(args) -> int:
@@ -108,7 +115,7 @@ def bench_batch(args) -> int:
     specs = workloads.c4_specs()
     t0 = time.perf_counter()
     batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0,
-                           sweep_kernel=0 if args.sweep_kernel == "pairs" else 1)
+                           sweep_kernel=0 if args.sweep_kernel == "pairs" else 1, autotune=args.autotune)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
@@ -117,13 +124,14 @@ def bench_batch(args) -> int:
             log(f"[bench] c4: {i + 1} circuits resident ({time.perf_counter() - t0:.1f}s)")
     F_tot = sum(2 * sp["num_gates"] for sp in specs)
     evals = sum(2 * sp["num_gates"] * sp["num_patterns"] for sp in specs)
-    times, agg, detected = [], None, 0
+    times, agg, detected, trials = [], None, 0, 0
     for it in range(args.warmup + args.steps):
         ids = [batch.add_job(c, f, p, drop=True) for c, (f, p) in zip(cids, jobs_text)]
         t0 = time.perf_counter()
         batch.run()
         dt = time.perf_counter() - t0
         st = batch.last_run_stats()
+        trials = st["tune_trials"]
         timed = it >= args.warmup
         if timed:
             times.append(dt)
@@ -138,7 +146,7 @@ def bench_batch(args) -> int:
         for j in ids:
             batch.release_job(j)
     elapsed = sum(times)
-    kernel = _kernel_name(args)
+    kernel = _kernel_name(args, st["tuned"])
     out = {
         "metric": "fault x pattern evals/sec",
         "value": evals / (elapsed / args.steps),
@@ -153,6 +161,7 @@ def bench_batch(args) -> int:
             "engine": args.mode, "faults": F_tot, "evals_per_step": evals, "detected": detected,
         },
         "roofline": _roofline(kernel, agg, "c4"),
+        "autotune": dict(_tuned(st), enabled=bool(args.autotune), candidates_measured=trials),
         "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
     }
     print(json.dumps(out), flush=True)
@@ -277,7 +286,10 @@ def main() -> int:
     ap.add_argument("--c5-load", type=float, default=0.5, help="arrival rate as a fraction of the saturated rate")
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
     ap.add_argument("--sweep-kernel", default="pairs", choices=["pairs", "nodewave"],
-                    help="persistent sweep kernel of the partitioned engine")
+                    help="persistent sweep kernel of the partitioned engine (without --autotune)")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
+                         "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -314,6 +326,7 @@ def main() -> int:
     sim.set_option("drop", 1 if preset["drop"] else 0)
     sim.set_option("report_last_round", 0)
     sim.set_option("sweep_kernel", 0 if args.sweep_kernel == "pairs" else 1)
+    sim.set_option("autotune", args.autotune)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")
@@ -321,7 +334,9 @@ def main() -> int:
     for i in range(args.warmup):
         t0 = time.perf_counter()
         sim.run(mode)
-        log(f"[bench] warmup {i} {time.perf_counter() - t0:.3f}s")
+        wst = sim.last_run_stats()
+        log(f"[bench] warmup {i} {time.perf_counter() - t0:.3f}s config {_tuned(wst)} "
+            f"({wst['tune_trials']} autotune candidates measured)")
     times = []
     agg = None
     t_all = time.perf_counter()
@@ -342,7 +357,7 @@ def main() -> int:
     ms = 1000.0 * elapsed / args.steps
     value = F * n / (elapsed / args.steps)
     detected = int(sim.detected().sum())
-    kernel = _kernel_name(args)
+    kernel = _kernel_name(args, st["tuned"])
     out = {
         "metric": "fault x pattern evals/sec",
         "value": value,
@@ -366,6 +381,7 @@ def main() -> int:
             "detected": detected,
         },
         "roofline": _roofline(kernel, agg, args.config),
+        "autotune": dict(_tuned(st), enabled=bool(args.autotune), candidates_measured=st["tune_trials"]),
         "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
     }
     if not args.no_cpu_baseline:

@@ -139,6 +139,9 @@ typedef struct {
   int64_t steps;           /* word steps executed */
   int64_t slots;           /* concurrent sweep tiles (scratch slots) */
   int64_t live_nodes;
+  int64_t tune_trials;     /* autotuner candidates measured so far for this job set (0: off) */
+  int64_t tuned;           /* configuration used: sweep kernel | first step words << 8 |
+                              word growth << 16 */
 } lsim_run_stats;
 LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
 

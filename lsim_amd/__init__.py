@@ -54,6 +54,8 @@ class RunStats(ctypes.Structure):
         ("steps", ctypes.c_int64),
         ("slots", ctypes.c_int64),
         ("live_nodes", ctypes.c_int64),
+        ("tune_trials", ctypes.c_int64),
+        ("tuned", ctypes.c_int64),
     ]
 
 

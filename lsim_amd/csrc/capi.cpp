@@ -143,6 +143,10 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "block_waves") {
     if (value < 1 || value > 8) return fail(err, LSIM_E_ARG, "block_waves must be 1..8");
     eo.block_waves = int(value);
+  } else if (k == "autotune") {
+    eo.autotune = value != 0;
+  } else if (k == "good_graph") {
+    eo.good_graph = value != 0;
   } else if (k == "sweep_kernel") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "sweep_kernel must be 0 or 1");
     eo.sweep_kernel = int(value);
@@ -482,6 +486,8 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.sweep_tiles = std::max(b->stats.sweep_tiles, s.sweep_tiles);
       b->stats.sweep_node_steps = std::max(b->stats.sweep_node_steps, s.sweep_node_steps);
       b->stats.sweep_bytes = std::max(b->stats.sweep_bytes, s.sweep_bytes);
+      b->stats.tune_trials = s.tune_trials;
+      b->stats.tuned = s.tuned;
       b->stats.slots = s.slots;
       b->stats.good_bytes += s.good_bytes;
     }
@@ -652,7 +658,7 @@ void lsim_server::loop() {
       int rc = guarded(&msg, [&] {
         std::vector<lsim::JobState *> fs;
         for (ServerJob *j : fresh) fs.push_back(j->js);
-        engine->start(fs);
+        engine->start(fs, eopt);
         engine->iterate(js, eopt);
       });
       std::vector<ServerJob *> still;

@@ -38,6 +38,8 @@
 #include <atomic>
 #include <chrono>
 #include <exception>
+#include <map>
+#include <memory>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -1103,9 +1105,25 @@ void parallel_for(const std::vector<int> &order, Fn &&fn) {
 
 // ---------------------------------------------------------------- host state
 
+// The good-machine launch sequence of one circuit at one round count,
+// captured once: kernels read the job from `job` (rewritten before each
+// replay) and per-level item offsets from `off`. Only kernel nodes.
+struct GoodGraph {
+  int64_t R = 0;
+  DevBuf<JobDev> job;
+  DevBuf<int64_t> off;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~GoodGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 struct CircuitRes {
   int64_t N = 0, NL = 0, P = 0, G = 0, O = 0, E = 0;
   int L = 0;
+  std::vector<std::unique_ptr<GoodGraph>> graphs;  // most recent last, at most kMaxGraphs
   bool has_mux = false;
   std::vector<int32_t> pos2dp, dp2pos, level_dp, live_lvl, dead_lvl;
   std::vector<int32_t> head_dp;     // per position: region head (device position), -1 if dead
@@ -1134,7 +1152,8 @@ struct JobState {
   DevBuf<uint8_t> d_sa, last;
   bool started = false, done = false, last_done = false;
   int64_t w0 = 0;
-  int Wt = 1;
+  int Wt = 1;                          // words of the next dropping step
+  int Wg = 2;                          // growth of Wt per step
   // current step
   enum Kind { MATRIX, DROP, LAST } kind = MATRIX;
   std::vector<int32_t> step_faults, units;
@@ -1193,6 +1212,37 @@ struct Engine::Impl {
     if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
   }
   JobDev job_dev(JobState &j) const;
+  GoodGraph *good_graph(CircuitRes &c, int64_t R);
+  void finish_start(const std::vector<JobState *> &js, double ms);
+  int64_t iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo);
+  // Autotuner: sweep kernel and dropping word schedule. Per job-set
+  // signature (resident circuits, rounds, dropping, injection mode) the first
+  // runs each use one untried candidate and time their word steps; later
+  // runs use the fastest. Exploration runs are ordinary runs (same results).
+  struct TuneChoice {
+    int kernel = 0, first = 1, growth = 2;
+  };
+  struct TuneState {
+    std::vector<TuneChoice> cands;
+    std::vector<double> ms;  // measured sweep-loop time per candidate, -1 until tried
+    // The next run's candidate: the first untried one, else the fastest.
+    int pick() const {
+      int best = 0;
+      for (size_t i = 0; i < ms.size(); ++i) {
+        if (ms[i] < 0) return int(i);
+        if (ms[i] < ms[size_t(best)]) best = int(i);
+      }
+      return best;
+    }
+    int tried() const {
+      int n = 0;
+      for (double x : ms) n += x >= 0 ? 1 : 0;
+      return n;
+    }
+  };
+  std::map<std::string, TuneState> tuned;
+  std::string tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo, bool &any_drop) const;
+  TuneState &tune_state(const std::string &key, const EngineOptions &eo, bool any_drop);
   bool plan(JobState &j);
   void finalize(JobState &j);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
@@ -1405,6 +1455,8 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   if (j.detbits.p) LSIM_HIP(hipMemsetAsync(j.detbits.p, 0, j.detbits.bytes(), m.stream));
   if (j.last.p) LSIM_HIP(hipMemsetAsync(j.last.p, 0, j.last.bytes(), m.stream));
   j.alive = j.live;
+  j.Wt = std::max(1, std::min(64, opt.first_words));
+  j.Wg = std::max(1, std::min(64, opt.word_growth));
   j.res.stats.live_nodes = c->NL;
   j.res.stats.live_faults = int64_t(j.live.size());
   m.jobs.push_back(std::move(jp));
@@ -1453,13 +1505,70 @@ JobDev Engine::Impl::job_dev(JobState &j) const {
   return d;
 }
 
-void Engine::start(const std::vector<JobState *> &jobs) {
+namespace {
+constexpr size_t kMaxGraphs = 4;
+
+// Items (node, word) of level l for a job with R rounds on circuit c.
+int64_t good_items(const CircuitRes &c, int64_t R, int l) {
+  if (R <= 0 || l > c.L) return 0;
+  return int64_t(c.live_lvl[l + 1] - c.live_lvl[l] + c.dead_lvl[l + 1] - c.dead_lvl[l]) * R;
+}
+}  // namespace
+
+GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R) {
+  for (size_t i = 0; i < c.graphs.size(); ++i)
+    if (c.graphs[i]->R == R) {
+      std::unique_ptr<GoodGraph> g = std::move(c.graphs[i]);
+      c.graphs.erase(c.graphs.begin() + long(i));
+      c.graphs.push_back(std::move(g));
+      return c.graphs.back().get();
+    }
+  auto g = std::make_unique<GoodGraph>();
+  g->R = R;
+  g->job.alloc(1);
+  std::vector<int64_t> off(size_t(c.L + 1) * 2, 0);
+  for (int l = 0; l <= c.L; ++l) off[size_t(l) * 2 + 1] = good_items(c, R, l);
+  g->off.alloc(off.size());
+  h2d(g->off.p, off.data(), off.size() * sizeof(int64_t));
+  LSIM_HIP(hipStreamSynchronize(stream));
+  LSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  for (int l = 0; l <= c.L; ++l) {
+    const int64_t total = off[size_t(l) * 2 + 1];
+    if (!total) continue;
+    k_good_level<<<blocks_for(total, 256), 256, 0, stream>>>(g->job.p, 1, g->off.p + size_t(l) * 2, l);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream, &graph);
+  if (e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    LSIM_HIP(e);
+  }
+  g->graph = graph;
+  LSIM_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  if (c.graphs.size() >= kMaxGraphs) c.graphs.erase(c.graphs.begin());
+  c.graphs.push_back(std::move(g));
+  return c.graphs.back().get();
+}
+
+void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
   std::vector<JobState *> js;
   for (JobState *j : jobs)
     if (!j->started) js.push_back(j);
   if (js.empty()) return;
+  if (js.size() == 1 && eo.good_graph && js[0]->R > 0) {
+    // One job: replay its circuit's captured good-machine graph.
+    JobState &j = *js[0];
+    GoodGraph *g = m.good_graph(*j.c, j.R);
+    const JobDev d = m.job_dev(j);
+    m.time_begin();
+    m.h2d(g->job.p, &d, sizeof(JobDev));
+    LSIM_HIP(hipGraphLaunch(g->exec, m.stream));
+    const double ms = m.time_end();
+    m.finish_start(js, ms);
+    return;
+  }
   // One launch per level index over all jobs' (node, word) pairs of that level.
   const int nj = int(js.size());
   int Lmax = 0;
@@ -1489,7 +1598,11 @@ void Engine::start(const std::vector<JobState *> &jobs) {
                                                               l);
     LSIM_HIP(hipGetLastError());
   }
-  const double ms = m.time_end();
+  m.finish_start(js, m.time_end());
+}
+
+void Engine::Impl::finish_start(const std::vector<JobState *> &js, double ms) {
+  Impl &m = *this;
   for (JobState *j : js) {
     j->res.stats.t_good_ms += ms;
     // One word per node written, one per fanin and per PI pattern read.
@@ -1941,7 +2054,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
           if (fr[k] == INT32_MAX) next.push_back(j.step_faults[k]);
         j.alive.swap(next);
         j.w0 += j.step_Wn;
-        j.Wt = std::min(64, j.Wt * 2);
+        j.Wt = std::min(64, j.Wt * j.Wg);
         break;
       }
       case JobState::LAST: j.last_done = true; break;
@@ -1957,17 +2070,15 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   }
 }
 
-int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
-  Impl &m = *impl_;
-  LSIM_HIP(hipSetDevice(m.device));
+int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   std::vector<JobState *> act;
   for (JobState *j : jobs)
-    if (j->started && !j->done && m.plan(*j)) act.push_back(j);
+    if (j->started && !j->done && plan(*j)) act.push_back(j);
   if (!act.empty()) {
     if (eo.mode == LSIM_MODE_PLAIN) {
-      for (JobState *j : act) m.step_batch({j}, eo);  // the per-level form runs one job at a time
+      for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
     } else {
-      m.step_batch(act, eo);
+      step_batch(act, eo);
     }
   }
   int64_t left = 0;
@@ -1975,13 +2086,88 @@ int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions
   return left;
 }
 
-void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
-  const auto t0 = std::chrono::steady_clock::now();
-  start(jobs);
-  while (iterate(jobs, eo) > 0) {
+int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
+  Impl &m = *impl_;
+  LSIM_HIP(hipSetDevice(m.device));
+  return m.iterate_jobs(jobs, eo);
+}
+
+std::string Engine::Impl::tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo,
+                                   bool &any_drop) const {
+  std::vector<std::string> parts;
+  any_drop = false;
+  for (const JobState *j : jobs) {
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "%p/%lld/%d/%d", static_cast<const void *>(j->c), (long long)j->R,
+                  int(j->opt.drop), int(j->direct));
+    parts.push_back(buf);
+    any_drop = any_drop || j->opt.drop;
   }
-  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  for (JobState *j : jobs) j->res.stats.t_total_ms = ms;
+  std::sort(parts.begin(), parts.end());
+  std::string key = std::to_string(eo.mode) + ":";
+  for (const std::string &x : parts) key += x + ",";
+  return key;
+}
+
+Engine::Impl::TuneState &Engine::Impl::tune_state(const std::string &key, const EngineOptions &eo, bool any_drop) {
+  auto it = tuned.find(key);
+  if (it != tuned.end()) return it->second;
+  TuneState st;
+  const int nk = eo.mode == LSIM_MODE_PLAIN ? 1 : 2;
+  const int sched[4][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}};
+  for (int k = 0; k < nk; ++k)
+    for (int s = 0; s < (any_drop ? 4 : 1); ++s) {
+      TuneChoice c;
+      c.kernel = k;
+      c.first = sched[s][0];
+      c.growth = sched[s][1];
+      st.cands.push_back(c);
+      st.ms.push_back(-1.0);
+    }
+  return tuned.emplace(key, std::move(st)).first->second;
+}
+
+void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
+  using clk = std::chrono::steady_clock;
+  Impl &m = *impl_;
+  const auto t0 = clk::now();
+  start(jobs, eo);
+  EngineOptions e2 = eo;
+  int first = jobs.empty() ? 1 : jobs[0]->Wt, growth = jobs.empty() ? 2 : jobs[0]->Wg;
+  Impl::TuneState *ts = nullptr;
+  int cand = -1;
+  if (eo.autotune && !jobs.empty()) {
+    bool any_drop = false;
+    ts = &m.tune_state(m.tune_key(jobs, eo, any_drop), eo, any_drop);
+    cand = ts->pick();
+    const Impl::TuneChoice &tc = ts->cands[size_t(cand)];
+    e2.sweep_kernel = tc.kernel;
+    first = tc.first;
+    growth = tc.growth;
+    for (JobState *j : jobs)
+      if (j->opt.drop && j->w0 == 0 && !j->done) {
+        j->Wt = tc.first;
+        j->Wg = tc.growth;
+      }
+  }
+  const auto t1 = clk::now();
+  while (iterate(jobs, e2) > 0) {
+  }
+  const auto t2 = clk::now();
+  if (ts) {
+    const double steps_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    double &slot = ts->ms[size_t(cand)];
+    slot = slot < 0 ? steps_ms : std::min(slot, steps_ms);
+    if (std::getenv("LSIM_TRACE"))
+      std::fprintf(stderr, "[lsim] autotune kernel=%d first=%d growth=%d steps %.2f ms (%d of %zu tried)\n",
+                   e2.sweep_kernel, first, growth, steps_ms, ts->tried(), ts->cands.size());
+  }
+  const double ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  for (JobState *j : jobs) {
+    j->res.stats.t_total_ms = ms;
+    j->res.stats.tune_trials = ts ? ts->tried() : 0;
+    j->res.stats.tuned = int64_t(e2.sweep_kernel) | (int64_t(first) << 8) | (int64_t(growth) << 16);
+  }
 }
 
 }  // namespace lsim

@@ -29,6 +29,8 @@ struct JobOptions {
   bool report_last_round = true;  // with dropping: evaluate the last round for every fault
   bool direct = false;            // inject every fault directly (forced for circuits with MUX)
   bool keep_good_last = false;    // gather the last round's good values (reference accessors)
+  int first_words = 1;            // with dropping: words of the first step
+  int word_growth = 2;            // with dropping: factor of words per step after each step (up to 64)
 };
 
 // Engine-wide options of a run / iteration.
@@ -37,6 +39,8 @@ struct EngineOptions {
   int64_t scratch_mb = 0;            // sweep scratch budget, 0 = automatic
   int block_waves = 8;               // waves per workgroup of the node-wave persistent sweep (1..8)
   int sweep_kernel = 0;              // persistent sweep: 0 = flattened (node, lane) pairs, 1 = one node per wave
+  bool good_graph = true;            // a lone starting job replays its circuit's captured good-machine graph
+  bool autotune = false;             // run(): pick sweep kernel and word schedule by timed probes (cached)
 };
 
 struct RunResults {
@@ -81,7 +85,7 @@ class Engine {
   int64_t free_bytes() const;
 
   // Good machine for jobs not started yet (one batched launch per level).
-  void start(const std::vector<JobState *> &jobs);
+  void start(const std::vector<JobState *> &jobs, const EngineOptions &eo);
   // One word step for every started, unfinished job. Jobs that finish are
   // finalised (results copied to the host). Returns the number still running.
   int64_t iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo);

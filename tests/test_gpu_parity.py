@@ -126,3 +126,22 @@ def test_c2_sampled_against_oracle(c2):
     R = o.R
     assert np.array_equal(table[0::2][sample], ev[sample, R - 1])
     assert np.all(table[1::2] == R - 1)
+
+
+def test_good_machine_graph_replays(c1):
+    """The captured good-machine graph, replayed for several runs on one
+    circuit with eager work in between, gives the eager launches' results."""
+    ckt, flst, ptn, ref = c1
+    eager = _sim(ckt, flst, ptn, good_graph=0)
+    eager.run(lsim_amd.Simulator.Mode.PARTITIONED)
+    want_good = eager.good_results()
+    want_det = eager.detect_matrix()
+    g = _sim(ckt, flst, ptn)
+    for _ in range(3):
+        g.run(lsim_amd.Simulator.Mode.PARTITIONED)
+        got = g.good_results()
+        assert all(np.array_equal(a, b) for a, b in zip(got, want_good))
+        assert np.array_equal(g.detect_matrix(), want_det)
+        eager.run(lsim_amd.Simulator.Mode.PLAIN)  # eager launches on another engine between replays
+        assert np.array_equal(eager.detect_matrix(), want_det)
+    assert np.array_equal(want_det, ref["det"])
