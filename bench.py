@@ -68,17 +68,38 @@ def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: fl
     sample = (f"{n_faults} seeded faults x {n_rounds} rounds ({n_pat} patterns) of the full circuit "
               f"({N} nodes)")
     log(f"[bench] cpu baseline: {sample}")
+    # Host cores to use for the level-parallel mode: the box's CPU share (16 per GPU) at most.
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    par = None
     if oracle.ref_available():
         _, secs = oracle.run_reference(ckt, sub_flst, sub_ptn, mode=0, threads=1, want_output=False)
         kind = "reference"
+        # PARALLEL_OMP (fsim/simulation.cpp:544-661): one parallel-for per level.
+        _, psecs = oracle.run_reference(ckt, sub_flst, sub_ptn, mode=2, threads=threads, want_output=False)
+        par = {"value": n_faults * n_pat / psecs, "cores": threads, "mode": "PARALLEL_OMP",
+               "seconds": round(psecs, 3)}
     else:
         o = oracle.Oracle(ckt, sub_flst, sub_ptn)
         t0 = time.perf_counter()
         o.run_literal(threads=1)
         secs = time.perf_counter() - t0
         kind = "port"
-    return {"value": n_faults * n_pat / secs, "unit": "fault*pattern evals/s", "cores": 1, "kind": kind,
-            "sample": sample, "seconds": round(secs, 3)}
+    out = {"value": n_faults * n_pat / secs, "unit": "fault*pattern evals/s", "cores": 1, "kind": kind,
+           "sample": sample, "seconds": round(secs, 3), "mode": "SEQUENTIAL", "cpu": _cpu_model()}
+    if par:
+        out["level_parallel"] = par
+    return out
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _roofline(kernel: str, agg: dict, config: str) -> dict:
