@@ -143,6 +143,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "block_waves") {
     if (value < 1 || value > 8) return fail(err, LSIM_E_ARG, "block_waves must be 1..8");
     eo.block_waves = int(value);
+  } else if (k == "pair_threads") {
+    if (value != 0 && value != 256 && value != 512) return fail(err, LSIM_E_ARG, "pair_threads must be 0, 256 or 512");
+    eo.pair_threads = int(value);
   } else if (k == "autotune") {
     eo.autotune = value != 0;
   } else if (k == "good_graph") {

@@ -264,7 +264,7 @@ struct SweepArgs {
   int *queue;        // event form: next tile
   unsigned long long *bytes;  // operand bytes touched
   MaskRec *mrec;     // per slot: [NLmax] (event form)
-  uint32_t *pend;    // per slot: [NLmax], == epoch when the node must be evaluated
+  uint32_t *pend;    // per slot: [NLmax], == epoch when the node must be evaluated (one-node-per-wave form)
   int64_t rec_stride, pend_stride;
   uint32_t epoch_base;
   // Optional tile statistics (event form), null unless traced:
@@ -611,17 +611,16 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 //     fanin masks, active lanes = lanes with a differing fanin or an
 //     injection), numbering nodes and their (node, active lane) pairs in one
 //     LDS counter;
-//  B. evaluates every (node, active lane) pair, one per thread: lanes where
+//  B. evaluates every (node, active lane) pair, two per thread: lanes where
 //     no fanin differs keep the good value and are never touched;
 //  C. for the nodes that differ somewhere, writes the mask record and numbers
 //     their (node, live fanout) pairs;
 //  D. marks every such fanout pending, one per thread.
 // A level thus costs a few dependent memory round trips however many nodes
-// and lanes it has. Injection bits are passed through the site's own mask
-// record (epoch with kInjFlag), written at tile start.
-constexpr int kPairChunk = 512;
+// and lanes it has. A node's injecting lanes are found by a binary search
+// over the tile's lane sites in LDS. Pending flags and mask records carry
+// the tile's epoch, so nothing is cleared between tiles.
 constexpr int kPairsPerThread = 2;
-constexpr uint32_t kInjFlag = 0x80000000u;
 
 // Position of the r-th set bit (r < popcount) of m.
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
@@ -666,25 +665,82 @@ __device__ __forceinline__ uint64_t wave_claim(uint64_t x, int lane, unsigned lo
   return base + inc - x;
 }
 
-template <bool kStats>
-__global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu(4))) k_sweep_pairs(SweepArgs a) {
+// Lanes of the tile that inject at position u. Lane sites are non-decreasing
+// over the tile (idle lanes hold INT_MAX), so they form one run.
+__device__ __forceinline__ uint64_t inj_lanes(const int *s_lsite, int u) {
+  int lo = 0, hi = 64;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_lsite[mid] < u) lo = mid + 1; else hi = mid;
+  }
+  const int a = lo;
+  hi = 64;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_lsite[mid] <= u) lo = mid + 1; else hi = mid;
+  }
+  const int n = lo - a;
+  if (n <= 0) return 0;
+  return n == 64 ? ~uint64_t(0) : (((uint64_t(1) << n) - 1) << a);
+}
+
+// Record of a node that differs from the good machine in some lane of the
+// tile (pair sweep): the differing lanes, the evaluated (active) lanes, the
+// tile's epoch and the node's first pool word. The node's faulty word of
+// active lane ln is pool[base + popcount(act & (2^ln - 1))]; it is written
+// only where the lane differs, which is all a fanout reads.
+struct alignas(32) PairRec {  // 32 B: a record never straddles a memory sector
+  unsigned long long nm;
+  unsigned long long act;
+  uint32_t epoch;
+  uint32_t base;
+  unsigned long long pad;
+};
+
+struct PairArgs {
+  const JobDev *__restrict__ jobs;
+  const Tile *__restrict__ tiles;
+  const int *__restrict__ tile_list;  // tiles to sweep (queue index -> tile), or null for all
+  int ntiles;                         // entries of the queue
+  int lvl_entries;
+  const Lane *__restrict__ lanes;
+  uint64_t *out;
+  int *queue;
+  unsigned long long *bytes;
+  PairRec *rec;      // per slot [rec_stride]
+  uint32_t *pend;    // per slot [rec_stride], == epoch when pending
+  int64_t rec_stride;
+  uint64_t *pool;    // per slot [pool_words]
+  int64_t pool_words;
+  int *ovf_count;    // tiles that outgrew their pool, listed in ovf_list
+  int *ovf_list;
+  uint32_t epoch_base;
+  unsigned long long *tstats;  // as SweepArgs::tstats, or null
+};
+
+template <bool kStats, int CHUNK>
+__global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4))) k_sweep_pairs(PairArgs a) {
   __shared__ unsigned long long s_acc[64], s_lmask[64];
   __shared__ int s_lsite[64], s_lword[64], s_lkind[64];
-  __shared__ int s_u[kPairChunk];
-  __shared__ uint32_t s_info[kPairChunk];
-  __shared__ int4 s_fin[kPairChunk];
-  __shared__ unsigned long long s_m[kPairChunk][4];
-  __shared__ unsigned long long s_act[kPairChunk], s_nm[kPairChunk];
-  __shared__ int s_pref[kPairChunk], s_wfo0[kPairChunk], s_epref[kPairChunk];
+  __shared__ int s_u[CHUNK];
+  __shared__ uint32_t s_info[CHUNK];
+  __shared__ int4 s_fin[CHUNK];
+  __shared__ unsigned long long s_m[CHUNK][4], s_fa[CHUNK][4];  // fanin differing / active lanes
+  __shared__ uint32_t s_fb[CHUNK][4];                            // fanin pool bases; phases C-D reuse it
+  __shared__ unsigned long long s_act[CHUNK], s_nm[CHUNK];
+  __shared__ int s_pref[CHUNK];
   __shared__ int s_tile, s_maxl;
+  __shared__ uint32_t s_pool;  // next free word of the slot's pool for this tile
   __shared__ unsigned long long s_cnt2, s_wcnt2;  // (nodes << 32 | pairs), (written << 32 | fanouts)
-  extern __shared__ int s_lvlcnt[];
+  extern __shared__ uint8_t s_lvl[];  // per level: some node of the level is pending
+  int *s_wfo0 = reinterpret_cast<int *>(&s_fb[0][0]);  // phases C-D: written node fanout start
+  int *s_epref = s_wfo0 + CHUNK;                       // phases C-D: fanout prefix
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  uint64_t *rows = a.rows + int64_t(blockIdx.x) * a.rows_stride;
-  MaskRec *mrec = a.mrec + int64_t(blockIdx.x) * a.rec_stride;
-  uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.pend_stride;
-  for (int i = tid; i < a.lvl_entries; i += blockDim.x) s_lvlcnt[i] = 0;
+  PairRec *rec = a.rec + int64_t(blockIdx.x) * a.rec_stride;
+  uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.rec_stride;
+  uint64_t *pool = a.pool + int64_t(blockIdx.x) * a.pool_words;
+  for (int i = tid; i < a.lvl_entries; i += CHUNK) s_lvl[i] = 0;
   if (tid == 0) {
     s_cnt2 = 0;
     s_wcnt2 = 0;
@@ -692,11 +748,12 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(a.queue, 1);
     __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(s_tile);
-    if (t >= a.ntiles) return;
+    const int q = __builtin_amdgcn_readfirstlane(s_tile);
+    if (q >= a.ntiles) return;
+    const int t = a.tile_list ? a.tile_list[q] : q;
     const Tile T = a.tiles[t];
     const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
-    const uint32_t epoch = a.epoch_base + uint32_t(t) + 1u;
+    const uint32_t epoch = a.epoch_base + uint32_t(q) + 1u;
     const int64_t RW = J.RW;
     const uint64_t *good = J.good;
     uint32_t bytes = 0;  // per thread and tile
@@ -704,34 +761,30 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
     if (tid < 64) {
       const bool valid = lane < T.nlanes;
       const Lane L = a.lanes[T.lane0 + (valid ? lane : 0)];
-      const int site = valid ? L.site : -1;
+      const int site = valid ? L.site : INT_MAX;
       s_lsite[lane] = site;
       s_lword[lane] = L.word;
       s_lkind[lane] = int(L.kind);
       s_lmask[lane] = L.mask;
       s_acc[lane] = 0;
-      // Lanes injecting at the same site share one record.
-      uint64_t im = 0;
-      for (int j = 0; j < 64; ++j) im |= (__shfl(site, j) == site) ? (uint64_t(1) << j) : 0;
       if (valid) {
-        MaskRec rec;
-        rec.mask = im;
-        rec.epoch = epoch | kInjFlag;
-        rec.pad = 0;
-        mrec[site] = rec;
         pend[site] = epoch;
-        atomicAdd(&s_lvlcnt[L.level], 1);
+        s_lvl[L.level] = 1;
       }
     }
-    if (tid == 0) s_maxl = T.end_level;
+    if (tid == 0) {
+      s_maxl = T.end_level;
+      s_pool = 0;
+    }
     __syncthreads();
     int l_end = T.start_level;
-    for (int l = T.start_level; l <= s_maxl; ++l) {
+    bool aborted = false;
+    for (int l = T.start_level; l <= s_maxl && !aborted; ++l) {
       l_end = l;
-      if (s_lvlcnt[l] == 0) continue;
+      if (s_lvl[l] == 0) continue;
       ++n_lv;
       const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
-      for (int c0 = lo; c0 < hi; c0 += kPairChunk) {
+      for (int c0 = lo; c0 < hi; c0 += CHUNK) {
         ++n_pass;
         // A. scan and stage
         if (tid == 0) s_wcnt2 = 0;
@@ -740,20 +793,27 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
         uint64_t act = 0;
         uint32_t inf = 0;
         int4 f = make_int4(0, 0, 0, 0);
-        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
+        uint32_t fb[4] = {0, 0, 0, 0};
         if (p) {
           inf = J.info[u];
           f = J.fin[u];
-          const MaskRec own = mrec[u];
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
           if (type != G_PI) {
-            m0 = stamped_mask(mrec, f.x, epoch);
-            if (nin > 1) m1 = stamped_mask(mrec, f.y, epoch);
-            if (nin > 2) m2 = stamped_mask(mrec, f.z, epoch);
-            if (nin > 3) m3 = stamped_mask(mrec, f.w, epoch);
+            const int fi[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              if (k < int(nin)) {
+                const PairRec r = rec[fi[k]];
+                if (r.epoch == epoch) {
+                  m[k] = r.nm;
+                  fa[k] = r.act;
+                  fb[k] = r.base;
+                }
+              }
           }
-          act = m0 | m1 | m2 | m3 | (own.epoch == (epoch | kInjFlag) ? own.mask : 0);
-          bytes += 40 + (type != G_PI ? 16 * nin : 0);
+          act = m[0] | m[1] | m[2] | m[3] | inj_lanes(s_lsite, u);
+          bytes += 20 + (type != G_PI ? 24 * nin : 0);
         }
         bytes += 4;
         const uint64_t slot = wave_claim(p ? ((uint64_t(1) << 32) | uint64_t(__popcll(act))) : 0, lane, &s_cnt2);
@@ -762,10 +822,12 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
           s_u[k] = u;
           s_info[k] = inf;
           s_fin[k] = f;
-          s_m[k][0] = m0;
-          s_m[k][1] = m1;
-          s_m[k][2] = m2;
-          s_m[k][3] = m3;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            s_m[k][j] = m[j];
+            s_fa[k][j] = fa[j];
+            s_fb[k][j] = fb[j];
+          }
           s_act[k] = act;
           s_nm[k] = 0;
           s_pref[k] = int(uint32_t(slot));
@@ -773,29 +835,37 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
         __syncthreads();
         const uint64_t c2 = s_cnt2;
         const int n = int(c2 >> 32), P = int(uint32_t(c2));
+        const uint32_t pb = s_pool;
+        if (uint64_t(pb) + uint64_t(P) > uint64_t(a.pool_words)) {
+          // The pass would outgrow the slot's pool: the tile is given up and
+          // swept again in a slot that holds any tile (same value for every
+          // thread, so the whole workgroup leaves together).
+          aborted = true;
+          break;
+        }
         // B. (node, active lane) pairs, kPairsPerThread per thread at a time
-        // with their loads interleaved.
+        // with their loads interleaved; a pair that differs writes its pool word.
 #pragma unroll 1
-        for (int q0 = tid; q0 < P; q0 += kPairsPerThread * kPairChunk) {
-          int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread];
+        for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
+          int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread], rk[kPairsPerThread];
           uint32_t type[kPairsPerThread], nin[kPairsPerThread];
           int4 fv[kPairsPerThread];
           int64_t w[kPairsPerThread];
-          uint64_t gu[kPairsPerThread], r[kPairsPerThread], mk[kPairsPerThread][4];
+          uint64_t gu[kPairsPerThread], r[kPairsPerThread];
           bool ok[kPairsPerThread];
 #pragma unroll
           for (int i = 0; i < kPairsPerThread; ++i) {
-            const int q = q0 + i * kPairChunk;
-            ok[i] = q < P;
-            k[i] = find_range(s_pref, n, ok[i] ? q : q0);
-            ln[i] = nth_set_bit(s_act[k[i]], (ok[i] ? q : q0) - s_pref[k[i]]);
+            const int qq = q0 + i * CHUNK;
+            ok[i] = qq < P;
+            const int qe = ok[i] ? qq : q0;
+            k[i] = find_range(s_pref, n, qe);
+            rk[i] = qe - s_pref[k[i]];
+            ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
             v[i] = s_u[k[i]];
             const uint32_t vi = s_info[k[i]];
             type[i] = vi & 15u;
             nin[i] = (vi >> 4) & 7u;
             fv[i] = s_fin[k[i]];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) mk[i][j] = s_m[k[i]][j];
             w[i] = s_lword[ln[i]];
             gu[i] = ok[i] ? good[int64_t(v[i]) * RW + w[i]] : 0;
           }
@@ -804,16 +874,16 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
             r[i] = gu[i];
             if (ok[i] && type[i] != G_PI) {
               const int b = ln[i];
-              const uint64_t v0 =
-                  ((mk[i][0] >> b) & 1) ? rows[int64_t(fv[i].x) * 64 + b] : good[int64_t(fv[i].x) * RW + w[i]];
-              uint64_t v1 = 0, v2 = 0, v3 = 0;
-              if (nin[i] > 1)
-                v1 = ((mk[i][1] >> b) & 1) ? rows[int64_t(fv[i].y) * 64 + b] : good[int64_t(fv[i].y) * RW + w[i]];
-              if (nin[i] > 2)
-                v2 = ((mk[i][2] >> b) & 1) ? rows[int64_t(fv[i].z) * 64 + b] : good[int64_t(fv[i].z) * RW + w[i]];
-              if (nin[i] > 3)
-                v3 = ((mk[i][3] >> b) & 1) ? rows[int64_t(fv[i].w) * 64 + b] : good[int64_t(fv[i].w) * RW + w[i]];
-              r[i] = eval_gate(type[i], nin[i], v0, v1, v2, v3);
+              const uint64_t below = (uint64_t(1) << b) - 1;
+              const int fi[4] = {fv[i].x, fv[i].y, fv[i].z, fv[i].w};
+              uint64_t vals[4] = {0, 0, 0, 0};
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                if (j < int(nin[i]))
+                  vals[j] = ((s_m[k[i]][j] >> b) & 1)
+                                ? pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))]
+                                : good[int64_t(fi[j]) * RW + w[i]];
+              r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
             }
           }
 #pragma unroll
@@ -827,7 +897,7 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
             const uint64_t d = r[i] ^ gu[i];
             bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
             if (d) {
-              rows[int64_t(v[i]) * 64 + b] = r[i];
+              pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
               atomicOr(&s_nm[k[i]], 1ull << b);
               if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
               bytes += 8;
@@ -845,15 +915,17 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
           if (nm) {
             wr = true;
             const int v = s_u[tid];
-            MaskRec rec;
-            rec.mask = nm;
-            rec.epoch = epoch;
-            rec.pad = 0;
-            mrec[v] = rec;
+            PairRec r;
+            r.nm = nm;
+            r.act = s_act[tid];
+            r.epoch = epoch;
+            r.base = pb + uint32_t(s_pref[tid]);
+            r.pad = 0;
+            rec[v] = r;
             e0 = J.fo_off[v];
             deg = J.fo_off[v + 1] - e0;
             if (deg) atomicMax(&s_maxl, J.fo[e0 + deg - 1].y);
-            bytes += 32;
+            bytes += 40;
           }
         }
         const uint64_t wslot = wave_claim(wr ? ((uint64_t(1) << 32) | uint64_t(deg)) : 0, lane, &s_wcnt2);
@@ -865,13 +937,14 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
         __syncthreads();
         const uint64_t w2 = s_wcnt2;
         const int nw = int(w2 >> 32), E = int(uint32_t(w2));
-        // D. mark fanouts pending
+        // D. mark fanouts pending; the tile's pool grows by this pass's pairs
+        if (tid == 0) s_pool = pb + uint32_t(P);
 #pragma unroll 1
-        for (int e = tid; e < E; e += kPairChunk) {
+        for (int e = tid; e < E; e += CHUNK) {
           const int k = find_range(s_epref, nw, e);
           const int2 fo = J.fo[s_wfo0[k] + (e - s_epref[k])];
           pend[fo.x] = epoch;
-          atomicAdd(&s_lvlcnt[fo.y], 1);
+          s_lvl[fo.y] = 1;
           bytes += 12;
         }
         if (kStats) {
@@ -880,9 +953,19 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
         }
         __syncthreads();
       }
-      if (tid == 0) s_lvlcnt[l] = 0;
+      if (tid == 0 && !aborted) s_lvl[l] = 0;
     }
-    if (tid < T.nlanes) a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+    if (aborted) {
+      __syncthreads();  // every thread has read the pass counters
+      for (int i = tid; i < a.lvl_entries; i += CHUNK) s_lvl[i] = 0;
+      if (tid == 0) {
+        s_cnt2 = 0;
+        s_wcnt2 = 0;
+        a.ovf_list[atomicAdd(a.ovf_count, 1)] = t;
+      }
+    } else if (tid < T.nlanes) {
+      a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+    }
     unsigned long long wbytes = bytes;
     for (int o = 32; o > 0; o >>= 1) wbytes += __shfl_xor(wbytes, o);
     if (lane == 0 && wbytes) atomicAdd(a.bytes, wbytes);
@@ -902,6 +985,14 @@ __global__ void __launch_bounds__(kPairChunk) __attribute__((amdgpu_waves_per_eu
     }
     __syncthreads();
   }
+}
+
+using PairKernel = void (*)(PairArgs);
+
+// The pair sweep instance for a workgroup of `threads` (512 or 256).
+PairKernel pairs_kernel(int threads, bool stats) {
+  if (threads == 256) return stats ? k_sweep_pairs<true, 256> : k_sweep_pairs<false, 256>;
+  return stats ? k_sweep_pairs<true, 512> : k_sweep_pairs<false, 512>;
 }
 
 // ---------------------------------------------------------------- detection
@@ -1124,6 +1215,7 @@ struct CircuitRes {
   int64_t N = 0, NL = 0, P = 0, G = 0, O = 0, E = 0;
   int L = 0;
   std::vector<std::unique_ptr<GoodGraph>> graphs;  // most recent last, at most kMaxGraphs
+  bool wide = false;  // mean live nodes per level above 256 (the pair sweep then uses 512-thread groups)
   bool has_mux = false;
   std::vector<int32_t> pos2dp, dp2pos, level_dp, live_lvl, dead_lvl;
   std::vector<int32_t> head_dp;     // per position: region head (device position), -1 if dead
@@ -1186,6 +1278,12 @@ struct Engine::Impl {
   DevBuf<uint64_t> d_rows, d_masks;
   DevBuf<MaskRec> d_mrec;
   DevBuf<uint32_t> d_pend;
+  // pair sweep scratch: per-slot records and pending flags, one pool carved per launch
+  DevBuf<PairRec> d_prec;
+  DevBuf<uint32_t> d_ppend;
+  DevBuf<uint64_t> d_pool;
+  DevBuf<int> d_ovf_count, d_ovf_list, d_ovf_list2;
+  int64_t pair_slots = 0, pair_nl = 0;
   int64_t slots = 0, slot_nl = 0;
   uint32_t epoch_base = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1247,8 +1345,14 @@ struct Engine::Impl {
   void finalize(JobState &j);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
   void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
-             int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st);
+             int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st);
+  void sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
+                   unsigned long long *tstats, lsim_run_stats &st);
   void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu);
+  void ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, int64_t slots_needed, int64_t full_slots);
+  void release_slots();
+  void release_pair_scratch();
+  int64_t scratch_budget(const EngineOptions &eo, int64_t held) const;
 };
 
 Engine::Engine(int device) : impl_(new Impl) {
@@ -1331,6 +1435,11 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
       c.live_lvl[l + 1] = int32_t(c.live_lvl[l] + cl[l + 1]);
       c.dead_lvl[l + 1] = int32_t(c.dead_lvl[l] + cd[l + 1]);
     }
+  }
+  {
+    int live_levels = 0;
+    for (int l = 0; l <= c.L; ++l) live_levels += c.live_lvl[l + 1] > c.live_lvl[l] ? 1 : 0;
+    c.wide = c.NL > 256 * int64_t(std::max(live_levels, 1));
   }
   const int64_t nl1 = std::max<int64_t>(c.NL, 1);
   std::vector<uint32_t> info(N);
@@ -1694,20 +1803,80 @@ void Engine::Impl::finalize(JobState &j) {
   j.done = true;
 }
 
+int64_t Engine::Impl::scratch_budget(const EngineOptions &eo, int64_t held) const {
+  if (eo.scratch_mb > 0) return eo.scratch_mb << 20;
+  size_t fr = 0, tot = 0;
+  LSIM_HIP(hipMemGetInfo(&fr, &tot));
+  return (int64_t(fr) + held) / 10 * 7;
+}
+
+void Engine::Impl::release_slots() {
+  d_rows.release();
+  d_masks.release();
+  d_mrec.release();
+  d_pend.release();
+  slots = 0;
+  slot_nl = 0;
+}
+
+void Engine::Impl::release_pair_scratch() {
+  d_prec.release();
+  d_ppend.release();
+  d_pool.release();
+  pair_slots = 0;
+  pair_nl = 0;
+}
+
+void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, int64_t slots_needed,
+                                       int64_t full_slots) {
+  const int64_t nl = std::max<int64_t>(nl_max, 1);
+  if (d_pool.p && pair_nl >= nl && pair_slots >= slots_needed) return;
+  LSIM_HIP(hipStreamSynchronize(stream));
+  const int64_t held = int64_t(d_rows.bytes() + d_masks.bytes() + d_mrec.bytes() + d_pend.bytes() +
+                               d_prec.bytes() + d_ppend.bytes() + d_pool.bytes());
+  release_slots();
+  release_pair_scratch();
+  const int64_t budget = scratch_budget(eo, held);
+  const int64_t fixed = slots_needed * nl * int64_t(sizeof(PairRec) + sizeof(uint32_t));
+  // The pool must hold at least one tile of any size (64 words per live
+  // node); it needs no more than that for every workgroup.
+  (void)full_slots;
+  const int64_t pool_words = std::min((budget - fixed) / 8, std::max<int64_t>(slots_needed, 1) * nl * 64);
+  if (pool_words < nl * 64)
+    throw DeviceError(LSIM_E_NOMEM, "sweep scratch of " + std::to_string(budget >> 20) + " MiB cannot hold " +
+                                        std::to_string(slots_needed) + " slots of a " + std::to_string(nl) +
+                                        "-node circuit");
+  const auto t0 = std::chrono::steady_clock::now();
+  d_prec.alloc(size_t(slots_needed * nl));
+  d_ppend.alloc(size_t(slots_needed * nl));
+  d_pool.alloc(size_t(pool_words));
+  const auto t1 = std::chrono::steady_clock::now();
+  // Epoch 0 is never used by a tile: zeroed records and flags are stale.
+  LSIM_HIP(hipMemsetAsync(d_prec.p, 0, d_prec.bytes(), stream));
+  LSIM_HIP(hipMemsetAsync(d_ppend.p, 0, d_ppend.bytes(), stream));
+  if (std::getenv("LSIM_TRACE")) {
+    LSIM_HIP(hipStreamSynchronize(stream));
+    const auto t2 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[lsim] pair scratch: %lld slots x %lld nodes, pool %.1f GiB: alloc %.1f ms, clear %.1f ms\n",
+                 (long long)slots_needed, (long long)nl, double(pool_words) * 8 / double(1ull << 30),
+                 std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                 std::chrono::duration<double, std::milli>(t2 - t1).count());
+  }
+  pair_slots = slots_needed;
+  pair_nl = nl;
+}
+
 void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu) {
+  if (d_pool.p) {
+    LSIM_HIP(hipStreamSynchronize(stream));
+    release_pair_scratch();
+  }
   // Per slot: faulty rows [NL][64], and the plain form's lane masks or the
   // event form's mask records and pending flags.
   const int64_t nl = std::max<int64_t>(nl_max, 1);
   const int64_t per_slot = nl * (64 * 8 + 8 + int64_t(sizeof(MaskRec)) + 4);
   const int64_t held = int64_t(d_rows.bytes() + d_masks.bytes() + d_mrec.bytes() + d_pend.bytes());
-  int64_t budget;
-  if (eo.scratch_mb > 0) {
-    budget = eo.scratch_mb << 20;
-  } else {
-    size_t fr = 0, tot = 0;
-    LSIM_HIP(hipMemGetInfo(&fr, &tot));
-    budget = (int64_t(fr) + held) / 10 * 7;
-  }
+  const int64_t budget = scratch_budget(eo, held);
   const int64_t cap = std::max<int64_t>(1, budget / per_slot);
   // No more slots than workgroups the device keeps resident at once.
   const int64_t s =
@@ -1731,7 +1900,8 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
 }
 
 void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
-                         int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, lsim_run_stats &st) {
+                         int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense,
+                         lsim_run_stats &st) {
   const int64_t ntiles = int64_t(tiles.size());
   if (!ntiles) return;
   d_lanes.alloc(size_t(nlanes));
@@ -1739,13 +1909,34 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   h2d(d_lanes.p, lanes, size_t(nlanes) * sizeof(Lane));
   h2d(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile));
   const size_t lds = size_t(lvl_entries) * sizeof(int);
+  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
+  if (trace) {
+    d_tstats.alloc(8);
+    LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, 8 * sizeof(unsigned long long), stream));
+  }
+  auto print_stats = [&] {
+    unsigned long long ts[8];
+    d2h(ts, d_tstats.p, sizeof(ts));
+    LSIM_HIP(hipStreamSynchronize(stream));
+    const double nt = double(std::max<unsigned long long>(ts[0], 1));
+    std::fprintf(stderr,
+                 "[lsim] tiles %llu: per tile levels walked %.1f, pending levels %.1f, scan passes %.1f, staged %.1f, "
+                 "written %.1f (max %llu), active lanes per staged node %.2f\n",
+                 ts[0], ts[6] / nt, ts[1] / nt, ts[2] / nt, ts[3] / nt, ts[4] / nt, ts[5],
+                 double(ts[7]) / double(std::max<unsigned long long>(ts[3], 1)));
+  };
+  if (!plain_circuit && eo.sweep_kernel == 0) {
+    st.sweep_tiles += ntiles;
+    time_begin();
+    sweep_pairs(eo, ntiles, lvl_entries, nl_max, dense, trace ? d_tstats.p : nullptr, st);
+    st.t_sweep_ms += time_end();
+    if (trace) print_stats();
+    return;
+  }
   const int bw = std::max(1, std::min(kChunk / 64, eo.block_waves));
   int per_cu = 3;
   if (!plain_circuit) {
-    if (eo.sweep_kernel == 1)
-      LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_events<false>, 64 * bw, lds));
-    else
-      LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pairs<false>, kPairChunk, lds));
+    LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_events<false>, 64 * bw, lds));
     if (per_cu < 1) throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) +
                                                          " B of level counters)");
   }
@@ -1769,13 +1960,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   a.rec_stride = slot_nl;
   a.pend_stride = slot_nl;
   a.epoch_base = 0;
-  a.tstats = nullptr;
-  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
-  if (trace) {
-    d_tstats.alloc(8);
-    LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, 8 * sizeof(unsigned long long), stream));
-    a.tstats = d_tstats.p;
-  }
+  a.tstats = trace ? d_tstats.p : nullptr;
   st.sweep_tiles += ntiles;
   time_begin();
   if (!plain_circuit) {
@@ -1790,17 +1975,10 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
     epoch_base += uint32_t(ntiles);
     LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
     const unsigned grid = unsigned(std::min<int64_t>(std::min<int64_t>(slots, ntiles), int64_t(cus) * per_cu));
-    if (eo.sweep_kernel == 1) {
-      if (a.tstats)
-        k_sweep_events<true><<<grid, 64 * bw, lds, stream>>>(a);
-      else
-        k_sweep_events<false><<<grid, 64 * bw, lds, stream>>>(a);
-    } else {
-      if (a.tstats)
-        k_sweep_pairs<true><<<grid, kPairChunk, lds, stream>>>(a);
-      else
-        k_sweep_pairs<false><<<grid, kPairChunk, lds, stream>>>(a);
-    }
+    if (a.tstats)
+      k_sweep_events<true><<<grid, 64 * bw, lds, stream>>>(a);
+    else
+      k_sweep_events<false><<<grid, 64 * bw, lds, stream>>>(a);
     LSIM_HIP(hipGetLastError());
     st.sweep_launches++;
   } else {
@@ -1821,17 +1999,92 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
     }
   }
   st.t_sweep_ms += time_end();
-  if (trace && !plain_circuit) {
-    unsigned long long ts[8];
-    d2h(ts, d_tstats.p, sizeof(ts));
-    LSIM_HIP(hipStreamSynchronize(stream));
-    const double nt = double(std::max<unsigned long long>(ts[0], 1));
-    std::fprintf(stderr,
-                 "[lsim] tiles %llu: per tile levels walked %.1f, pending levels %.1f, scan passes %.1f, staged %.1f, "
-                 "written %.1f (max %llu), active lanes per staged node %.2f\n",
-                 ts[0], ts[6] / nt, ts[1] / nt, ts[2] / nt, ts[3] / nt, ts[4] / nt, ts[5],
-                 double(ts[7]) / double(std::max<unsigned long long>(ts[3], 1)));
+  if (trace && !plain_circuit) print_stats();
+}
+
+void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
+                               unsigned long long *tstats, lsim_run_stats &st) {
+  const size_t lds = size_t(lvl_entries);  // one flag byte per level
+  int pc256 = 0, pc512 = 0;
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false), 256, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false), 512, lds));
+  if (pc512 < 1)
+    throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
+  ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  // Dense or wide steps use 512-thread workgroups, the others 256-thread
+  // ones, twice as many tiles at once.
+  int threads = eo.pair_threads ? eo.pair_threads : (dense ? 512 : 256);
+  if (threads == 256 && pc256 < 1) threads = 512;
+  const int per_cu = threads == 256 ? pc256 : pc512;
+  const int64_t full = pair_nl * 64;  // pool words one tile can use at most
+  const int64_t pool_words = int64_t(d_pool.n);
+  // Tile epochs are unique over the engine's lifetime; before the 32-bit
+  // counter wraps, the records are cleared and numbering restarts.
+  if (uint64_t(epoch_base) + 2 * uint64_t(ntiles) + 2 >= (uint64_t(1) << 31)) {
+    LSIM_HIP(hipMemsetAsync(d_prec.p, 0, d_prec.bytes(), stream));
+    LSIM_HIP(hipMemsetAsync(d_ppend.p, 0, d_ppend.bytes(), stream));
+    epoch_base = 0;
   }
+  d_queue.alloc(1);
+  d_ovf_count.alloc(1);
+  d_ovf_list.alloc(size_t(ntiles));
+  PairArgs a;
+  a.jobs = d_jobs.p;
+  a.tiles = d_tiles.p;
+  a.tile_list = nullptr;
+  a.ntiles = int(ntiles);
+  a.lvl_entries = lvl_entries;
+  a.lanes = d_lanes.p;
+  a.out = d_dout.p;
+  a.queue = d_queue.p;
+  a.bytes = d_bytes.p;
+  a.rec = d_prec.p;
+  a.pend = d_ppend.p;
+  a.rec_stride = pair_nl;
+  a.pool = d_pool.p;
+  a.ovf_count = d_ovf_count.p;
+  a.ovf_list = d_ovf_list.p;
+  a.tstats = tstats;
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ntiles, int64_t(cus) * per_cu),
+                                                               pair_slots));
+  a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_words / grid), int64_t(UINT32_MAX));
+  a.epoch_base = epoch_base;
+  epoch_base += uint32_t(ntiles);
+  LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
+  LSIM_HIP(hipMemsetAsync(d_ovf_count.p, 0, sizeof(int), stream));
+  hipLaunchKernelGGL(pairs_kernel(threads, tstats != nullptr), dim3(unsigned(grid)), dim3(threads), lds, stream, a);
+  LSIM_HIP(hipGetLastError());
+  st.sweep_launches++;
+  if (tstats)
+    std::fprintf(stderr, "[lsim] pair sweep: %d threads, %lld workgroups (%d/%d per CU at 256/512), %lld-word pools\n",
+                 threads, (long long)grid, pc256, pc512, (long long)a.pool_words);
+  if (a.pool_words >= full) return;  // no tile can outgrow its pool
+  int count = 0;
+  d2h(&count, d_ovf_count.p, sizeof(int));
+  LSIM_HIP(hipStreamSynchronize(stream));
+  if (count <= 0) return;
+  // Tiles that outgrew their pool: again, in slots that hold any tile.
+  const int64_t g2 = std::max<int64_t>(
+      1, std::min<int64_t>(std::min<int64_t>(count, int64_t(cus) * pc512), std::min(pair_slots, pool_words / full)));
+  d_ovf_list2.alloc(size_t(count));
+  a.tile_list = d_ovf_list.p;
+  a.ntiles = count;
+  a.ovf_list = d_ovf_list2.p;
+  a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_words / g2), int64_t(UINT32_MAX));
+  a.epoch_base = epoch_base;
+  epoch_base += uint32_t(count);
+  LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
+  LSIM_HIP(hipMemsetAsync(d_ovf_count.p, 0, sizeof(int), stream));
+  hipLaunchKernelGGL(pairs_kernel(512, tstats != nullptr), dim3(unsigned(g2)), dim3(512), lds, stream, a);
+  LSIM_HIP(hipGetLastError());
+  st.sweep_launches++;
+  if (std::getenv("LSIM_TRACE"))
+    std::fprintf(stderr, "[lsim] %d of %lld tiles outgrew %lld-word pools; swept again in %lld slots\n", count,
+                 (long long)ntiles, (long long)(pool_words / grid), (long long)g2);
+  int again = 0;
+  d2h(&again, d_ovf_count.p, sizeof(int));
+  LSIM_HIP(hipStreamSynchronize(stream));
+  if (again) throw DeviceError(LSIM_E_DEVICE, "a tile outgrew a pool of 64 words per live node");
 }
 
 void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOptions &eo) {
@@ -1990,6 +2243,10 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   std::vector<Tile> tiles;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
+    // The pair sweep finds a node's injecting lanes by binary search: a job's
+    // lanes follow its units, which are in ascending position.
+    for (int64_t l = lane_off[i] + 1; l < lane_off[i + 1]; ++l)
+      if (lanes[l].site < lanes[l - 1].site) throw DeviceError(LSIM_E_DEVICE, "sweep lanes out of site order");
     for (int64_t l0 = lane_off[i]; l0 < lane_off[i + 1]; l0 += 64) {
       Tile t;
       t.lane0 = int32_t(l0);
@@ -2007,7 +2264,16 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   });
   const bool plain = eo.mode == LSIM_MODE_PLAIN;
   const auto h3 = clk::now();
-  sweep(eo, lanes, nlanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, st);
+  // 512-thread pair sweep groups for dense steps (a dropping job's first
+  // step: all faults, most lanes active at most nodes; full-matrix steps) and
+  // when most tiles walk levels wider than 256 nodes (one scan pass each).
+  bool dense = false;
+  int64_t wide_tiles = 0;
+  for (const JobState *j : act)
+    dense = dense || j->kind == JobState::MATRIX || (j->kind == JobState::DROP && j->w0 == 0);
+  for (const Tile &t : tiles) wide_tiles += act[t.job]->c->wide ? 1 : 0;
+  const bool big_groups = dense || 2 * wide_tiles > int64_t(tiles.size());
+  sweep(eo, lanes, nlanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, big_groups, st);
   const auto h4 = clk::now();
   time_begin();
   if (items) {
