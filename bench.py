@@ -211,7 +211,8 @@ def bench_serving(args) -> int:
 
     circuits = workloads.c5_circuit_specs()
     t0 = time.perf_counter()
-    srv = lsim_amd.Server(device=0, job_budget_mb=64 * 1024, scratch_mb=16 * 1024)
+    srv = lsim_amd.Server(device=0, job_budget_mb=64 * 1024)
+    srv.set_option("word_growth", args.c5_growth)
     cids = []
     for i, sp in enumerate(circuits):
         cids.append(srv.add_circuit(workloads.circuit_text(sp)))
@@ -225,6 +226,11 @@ def bench_serving(args) -> int:
     def submit(k):
         f, p = texts[k]
         return srv.submit(cids[jobs[k]["circuit"]], f, p, drop=True)
+
+    # One untimed job first: the sweep scratch is allocated on first use.
+    w = submit(0)
+    srv.wait(w, 600)
+    srv.release_job(w)
 
     # Phase 1: unloaded.
     alone, first = [], []
@@ -306,6 +312,7 @@ def main() -> int:
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c5-jobs", type=int, default=96, help="jobs per phase of the serving run")
     ap.add_argument("--c5-load", type=float, default=0.5, help="arrival rate as a fraction of the saturated rate")
+    ap.add_argument("--c5-growth", type=int, default=4, help="word growth per step of serving jobs")
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
     ap.add_argument("--sweep-kernel", default="pairs", choices=["pairs", "nodewave"],
                     help="persistent sweep kernel of the partitioned engine (without --autotune)")

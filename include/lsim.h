@@ -75,7 +75,17 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            per-tile state (default 0 = automatic).
  *  "direct"             0/1  inject every fault directly instead of stem
  *                            tracing (default 0; forced on for circuits with MUX).
- *  "block_waves"        waves per workgroup of the persistent sweep (default 8). */
+ *  "sweep_kernel"       persistent sweep: 0 flattened (node, lane) pairs (default),
+ *                            1 one node per wave.
+ *  "pair_threads"       pair sweep workgroup: 0 by step (default), 256 or 512.
+ *  "pool_cap"           pair sweep: at most this many pool words per slot
+ *                            (default 0 = as memory allows); a tile that needs
+ *                            more is swept again in a slot that holds any tile.
+ *  "block_waves"        waves per workgroup of the one-node-per-wave sweep (default 8).
+ *  "autotune"           0/1  successive runs of a job set each try one (kernel,
+ *                            word schedule) candidate, then keep the fastest.
+ *  "good_graph"         0/1  a lone job replays its circuit's captured
+ *                            good-machine graph (default 1). */
 LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
 
 /* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).
@@ -139,6 +149,7 @@ typedef struct {
   int64_t steps;           /* word steps executed */
   int64_t slots;           /* concurrent sweep tiles (scratch slots) */
   int64_t live_nodes;
+  int64_t sweep_reruns;    /* tiles swept again after outgrowing their slot's pool */
   int64_t tune_trials;     /* autotuner candidates measured so far for this job set (0: off) */
   int64_t tuned;           /* configuration used: sweep kernel | first step words << 8 |
                               word growth << 16 */
@@ -154,7 +165,8 @@ typedef struct lsim_batch lsim_batch;
 LSIM_API int lsim_batch_create(int device, lsim_batch **out);
 LSIM_API void lsim_batch_destroy(lsim_batch *b);
 LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
-/* Options as lsim_set_option: "scratch_mb", "block_waves", "mode" (0 plain, 1 persistent). */
+/* Options as lsim_set_option (engine options: "scratch_mb", "block_waves", "sweep_kernel", "pair_threads",
+ * "pool_cap", "autotune", "good_graph") and "mode" (0 plain, 1 persistent). */
 LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
 /* Makes a circuit (.ckt text) resident; returns its id. */
 LSIM_API int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id);
@@ -179,7 +191,7 @@ LSIM_API int lsim_batch_release_job(lsim_batch *b, int64_t job_id);
  * others run joins at the next iteration. */
 typedef struct lsim_server lsim_server;
 /* job_budget_mb: HBM for admitted jobs (good values, fault state, results);
- * scratch_mb: HBM for sweep tiles (0 = 8 GiB). */
+ * scratch_mb: HBM for sweep tiles (0 = 90 % of free HBM less the job budget and 4 GiB). */
 LSIM_API int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, lsim_server **out);
 LSIM_API void lsim_server_destroy(lsim_server *s);
 LSIM_API const char *lsim_server_last_error(const lsim_server *s);
@@ -196,6 +208,9 @@ LSIM_API int lsim_server_job_times(lsim_server *s, int64_t job_id, double *times
 LSIM_API int lsim_server_release_job(lsim_server *s, int64_t job_id);
 /* counters[0..3] = iterations, jobs finished, jobs failed, peak admitted jobs. */
 LSIM_API int lsim_server_counters(lsim_server *s, int64_t *counters);
+/* Engine options as lsim_set_option, and the word schedule of jobs submitted
+ * later: "first_words" (default 1) and "word_growth" (default 4). */
+LSIM_API int lsim_server_set_option(lsim_server *s, const char *key, int64_t value);
 
 /* Seeded synthetic workload in the reference formats (no reference
  * counterpart; the reference ships fsim/circuit_generator.py and

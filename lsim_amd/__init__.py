@@ -54,6 +54,7 @@ class RunStats(ctypes.Structure):
         ("steps", ctypes.c_int64),
         ("slots", ctypes.c_int64),
         ("live_nodes", ctypes.c_int64),
+        ("sweep_reruns", ctypes.c_int64),
         ("tune_trials", ctypes.c_int64),
         ("tuned", ctypes.c_int64),
     ]
@@ -156,6 +157,7 @@ def lib() -> ctypes.CDLL:
         "lsim_server_job_times": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
         "lsim_server_release_job": (ctypes.c_int, [P, ctypes.c_int64]),
         "lsim_server_counters": (ctypes.c_int, [P, i64p]),
+        "lsim_server_set_option": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int64]),
         "lsim_generate_faults": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_generate_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
@@ -520,6 +522,9 @@ class Server:
 
     def release_job(self, job: int) -> None:
         self._check(self._L.lsim_server_release_job(self._h, job))
+
+    def set_option(self, key: str, value: int) -> None:
+        self._check(self._L.lsim_server_set_option(self._h, key.encode(), int(value)))
 
     def counters(self) -> dict:
         c = (ctypes.c_int64 * 4)()

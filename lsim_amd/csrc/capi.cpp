@@ -146,6 +146,8 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "pair_threads") {
     if (value != 0 && value != 256 && value != 512) return fail(err, LSIM_E_ARG, "pair_threads must be 0, 256 or 512");
     eo.pair_threads = int(value);
+  } else if (k == "pool_cap") {
+    eo.pool_cap = value < 0 ? 0 : value;
   } else if (k == "autotune") {
     eo.autotune = value != 0;
   } else if (k == "good_graph") {
@@ -490,6 +492,7 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.sweep_node_steps = std::max(b->stats.sweep_node_steps, s.sweep_node_steps);
       b->stats.sweep_bytes = std::max(b->stats.sweep_bytes, s.sweep_bytes);
       b->stats.tune_trials = s.tune_trials;
+      b->stats.sweep_reruns = std::max(b->stats.sweep_reruns, s.sweep_reruns);
       b->stats.tuned = s.tuned;
       b->stats.slots = s.slots;
       b->stats.good_bytes += s.good_bytes;
@@ -596,6 +599,7 @@ struct lsim_server {
   std::map<int64_t, std::unique_ptr<ServerJob>> jobs;
   int64_t next_id = 0;
   int64_t iterations = 0, finished = 0, failed = 0, peak = 0;
+  int first_words = 1, word_growth = 4;
   bool stop = false;
   std::thread worker;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -715,7 +719,13 @@ int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, ls
     s->engine.reset(new lsim::Engine(device));
     s->budget = job_budget_mb << 20;
     s->eopt.mode = LSIM_MODE_PARTITIONED;
-    s->eopt.scratch_mb = scratch_mb > 0 ? scratch_mb : 8192;
+    if (scratch_mb > 0) {
+      s->eopt.scratch_mb = scratch_mb;
+    } else {
+      // Free HBM less the job budget and a 4 GiB margin for resident circuits.
+      const int64_t free_mb = s->engine->free_bytes() >> 20;
+      s->eopt.scratch_mb = std::max<int64_t>(1024, (free_mb - job_budget_mb - 4096) / 10 * 9);
+    }
     s->worker = std::thread([p = s.get()] { p->loop(); });
   });
   if (rc) {
@@ -789,6 +799,10 @@ int lsim_server_submit(lsim_server *s, int64_t circuit_id, const char *flst, siz
   j->circuit = circuit_id;
   j->opt.drop = drop != 0;
   j->opt.report_last_round = false;
+  // Fewer, larger word steps than a lone run: every iteration a job waits
+  // for is shared with the other admitted jobs.
+  j->opt.first_words = s->first_words;
+  j->opt.word_growth = s->word_growth;
   j->bytes = lsim::Engine::job_bytes(c->res, int64_t(j->site.size()), j->R, j->opt.drop);
   j->t_submit = s->now();
   j->id = s->next_id++;
@@ -854,6 +868,22 @@ int lsim_server_release_job(lsim_server *s, int64_t job_id) {
   if (it->second->state < 2) return fail(&s->err, LSIM_E_STATE, "job still queued or running");
   s->jobs.erase(it);
   return LSIM_OK;
+}
+
+int lsim_server_set_option(lsim_server *s, const char *key, int64_t value) {
+  if (!s || !key) return LSIM_E_ARG;
+  const std::string k(key);
+  std::lock_guard<std::mutex> elk(s->engine_mu);  // the worker reads eopt under this lock
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (k == "first_words" || k == "word_growth") {
+    if (value < 1 || value > 64) return fail(&s->err, LSIM_E_ARG, k + " must be 1..64");
+    (k == "first_words" ? s->first_words : s->word_growth) = int(value);
+    return LSIM_OK;
+  }
+  bool known = false;
+  int rc = set_engine_option(s->eopt, &s->err, k, value, known);
+  if (rc || known) return rc;
+  return fail(&s->err, LSIM_E_ARG, "unknown option " + k);
 }
 
 int lsim_server_counters(lsim_server *s, int64_t *counters) {

@@ -1157,6 +1157,15 @@ struct HostBuf {
   ~HostBuf() { release(); }
 };
 
+// LSIM_TRACE set to anything but "" or "0": per-iteration timings on stderr.
+inline bool trace_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("LSIM_TRACE");
+    return v && *v && std::strcmp(v, "0") != 0;
+  }();
+  return on;
+}
+
 inline unsigned blocks_for(int64_t threads, int bs) { return unsigned((threads + bs - 1) / bs); }
 
 // Runs fn(order[k]) for k in [0, n) on up to 16 host threads (items are
@@ -1283,7 +1292,7 @@ struct Engine::Impl {
   DevBuf<uint32_t> d_ppend;
   DevBuf<uint64_t> d_pool;
   DevBuf<int> d_ovf_count, d_ovf_list, d_ovf_list2;
-  int64_t pair_slots = 0, pair_nl = 0;
+  int64_t pair_slots = 0, pair_req = 0, pair_nl = 0;
   int64_t slots = 0, slot_nl = 0;
   uint32_t epoch_base = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1824,45 +1833,50 @@ void Engine::Impl::release_pair_scratch() {
   d_ppend.release();
   d_pool.release();
   pair_slots = 0;
+  pair_req = 0;
   pair_nl = 0;
 }
 
 void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, int64_t slots_needed,
                                        int64_t full_slots) {
   const int64_t nl = std::max<int64_t>(nl_max, 1);
-  if (d_pool.p && pair_nl >= nl && pair_slots >= slots_needed) return;
+  if (d_pool.p && pair_nl >= nl && pair_req >= slots_needed) return;
   LSIM_HIP(hipStreamSynchronize(stream));
   const int64_t held = int64_t(d_rows.bytes() + d_masks.bytes() + d_mrec.bytes() + d_pend.bytes() +
                                d_prec.bytes() + d_ppend.bytes() + d_pool.bytes());
   release_slots();
   release_pair_scratch();
   const int64_t budget = scratch_budget(eo, held);
-  const int64_t fixed = slots_needed * nl * int64_t(sizeof(PairRec) + sizeof(uint32_t));
-  // The pool must hold at least one tile of any size (64 words per live
-  // node); it needs no more than that for every workgroup.
   (void)full_slots;
-  const int64_t pool_words = std::min((budget - fixed) / 8, std::max<int64_t>(slots_needed, 1) * nl * 64);
-  if (pool_words < nl * 64)
-    throw DeviceError(LSIM_E_NOMEM, "sweep scratch of " + std::to_string(budget >> 20) + " MiB cannot hold " +
-                                        std::to_string(slots_needed) + " slots of a " + std::to_string(nl) +
-                                        "-node circuit");
+  // Per slot: a record and a pending flag per live node. The pool must hold
+  // at least one tile of any size (64 words per live node) and needs no more
+  // than that for every slot; a tight budget gets fewer slots.
+  const int64_t per_slot = nl * int64_t(sizeof(PairRec) + sizeof(uint32_t));
+  const int64_t one_tile = nl * 64 * 8;
+  const int64_t fit = (budget - one_tile) / per_slot;
+  if (fit < 1)
+    throw DeviceError(LSIM_E_NOMEM, "sweep scratch of " + std::to_string(budget >> 20) + " MiB cannot hold a slot of a " +
+                                        std::to_string(nl) + "-node circuit");
+  const int64_t ns = std::max<int64_t>(1, std::min(slots_needed, fit));
+  const int64_t pool_words = std::min((budget - ns * per_slot) / 8, ns * nl * 64);
   const auto t0 = std::chrono::steady_clock::now();
-  d_prec.alloc(size_t(slots_needed * nl));
-  d_ppend.alloc(size_t(slots_needed * nl));
+  d_prec.alloc(size_t(ns * nl));
+  d_ppend.alloc(size_t(ns * nl));
   d_pool.alloc(size_t(pool_words));
   const auto t1 = std::chrono::steady_clock::now();
   // Epoch 0 is never used by a tile: zeroed records and flags are stale.
   LSIM_HIP(hipMemsetAsync(d_prec.p, 0, d_prec.bytes(), stream));
   LSIM_HIP(hipMemsetAsync(d_ppend.p, 0, d_ppend.bytes(), stream));
-  if (std::getenv("LSIM_TRACE")) {
+  if (trace_on()) {
     LSIM_HIP(hipStreamSynchronize(stream));
     const auto t2 = std::chrono::steady_clock::now();
     std::fprintf(stderr, "[lsim] pair scratch: %lld slots x %lld nodes, pool %.1f GiB: alloc %.1f ms, clear %.1f ms\n",
-                 (long long)slots_needed, (long long)nl, double(pool_words) * 8 / double(1ull << 30),
+                 (long long)ns, (long long)nl, double(pool_words) * 8 / double(1ull << 30),
                  std::chrono::duration<double, std::milli>(t1 - t0).count(),
                  std::chrono::duration<double, std::milli>(t2 - t1).count());
   }
-  pair_slots = slots_needed;
+  pair_slots = ns;
+  pair_req = slots_needed;
   pair_nl = nl;
 }
 
@@ -1909,7 +1923,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   h2d(d_lanes.p, lanes, size_t(nlanes) * sizeof(Lane));
   h2d(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile));
   const size_t lds = size_t(lvl_entries) * sizeof(int);
-  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
+  const bool trace = trace_on();
   if (trace) {
     d_tstats.alloc(8);
     LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, 8 * sizeof(unsigned long long), stream));
@@ -2010,6 +2024,9 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false), 512, lds));
   if (pc512 < 1)
     throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
+  // Sized for the largest resident circuit, so later jobs on it reuse the
+  // scratch instead of reallocating it.
+  for (const auto &c : circuits) nl_max = std::max(nl_max, c->NL);
   ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
   // Dense or wide steps use 512-thread workgroups, the others 256-thread
   // ones, twice as many tiles at once.
@@ -2048,6 +2065,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ntiles, int64_t(cus) * per_cu),
                                                                pair_slots));
   a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_words / grid), int64_t(UINT32_MAX));
+  if (eo.pool_cap > 0) a.pool_words = std::min<int64_t>(a.pool_words, eo.pool_cap);
   a.epoch_base = epoch_base;
   epoch_base += uint32_t(ntiles);
   LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
@@ -2063,6 +2081,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   d2h(&count, d_ovf_count.p, sizeof(int));
   LSIM_HIP(hipStreamSynchronize(stream));
   if (count <= 0) return;
+  st.sweep_reruns += count;
   // Tiles that outgrew their pool: again, in slots that hold any tile.
   const int64_t g2 = std::max<int64_t>(
       1, std::min<int64_t>(std::min<int64_t>(count, int64_t(cus) * pc512), std::min(pair_slots, pool_words / full)));
@@ -2078,7 +2097,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   hipLaunchKernelGGL(pairs_kernel(512, tstats != nullptr), dim3(unsigned(g2)), dim3(512), lds, stream, a);
   LSIM_HIP(hipGetLastError());
   st.sweep_launches++;
-  if (std::getenv("LSIM_TRACE"))
+  if (trace_on())
     std::fprintf(stderr, "[lsim] %d of %lld tiles outgrew %lld-word pools; swept again in %lld slots\n", count,
                  (long long)ntiles, (long long)(pool_words / grid), (long long)g2);
   int again = 0;
@@ -2090,7 +2109,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
 void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOptions &eo) {
   const int nj = int(act.size());
   using clk = std::chrono::steady_clock;
-  const bool trace = std::getenv("LSIM_TRACE") != nullptr;
+  const bool trace = trace_on();
   const auto h0 = clk::now();
   std::vector<JobDev> devs(nj);
   std::vector<int64_t> item_off(nj + 1, 0);
@@ -2310,6 +2329,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     js.sweep_tiles += st.sweep_tiles;
     js.sweep_node_steps += st.sweep_node_steps;
     js.sweep_bytes += st.sweep_bytes;
+    js.sweep_reruns += st.sweep_reruns;
     switch (j.kind) {
       case JobState::MATRIX: j.w0 += j.step_Wn; break;
       case JobState::DROP: {
@@ -2424,7 +2444,7 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
     const double steps_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     double &slot = ts->ms[size_t(cand)];
     slot = slot < 0 ? steps_ms : std::min(slot, steps_ms);
-    if (std::getenv("LSIM_TRACE"))
+    if (trace_on())
       std::fprintf(stderr, "[lsim] autotune kernel=%d first=%d growth=%d steps %.2f ms (%d of %zu tried)\n",
                    e2.sweep_kernel, first, growth, steps_ms, ts->tried(), ts->cands.size());
   }

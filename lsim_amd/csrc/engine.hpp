@@ -41,6 +41,7 @@ struct EngineOptions {
   int sweep_kernel = 0;              // persistent sweep: 0 = flattened (node, lane) pairs, 1 = one node per wave
   bool good_graph = true;            // a lone starting job replays its circuit's captured good-machine graph
   int pair_threads = 0;              // workgroup size of the pair sweep: 512, 256, or 0 = by step density
+  int64_t pool_cap = 0;              // pair sweep: at most this many pool words per slot (0 = as memory allows)
   bool autotune = false;             // run(): pick sweep kernel and word schedule by timed probes (cached)
 };
 

@@ -145,3 +145,19 @@ def test_good_machine_graph_replays(c1):
         eager.run(lsim_amd.Simulator.Mode.PLAIN)  # eager launches on another engine between replays
         assert np.array_equal(eager.detect_matrix(), want_det)
     assert np.array_equal(want_det, ref["det"])
+
+
+@pytest.mark.parametrize("drop", [0, 1], ids=["matrix", "dropping"])
+def test_pool_overflow_reruns(c1, drop):
+    """Pools capped at 64 words per slot: most tiles outgrow theirs, are given
+    up and swept again in slots that hold any tile; results are unchanged."""
+    ckt, flst, ptn, ref = c1
+    s = _sim(ckt, flst, ptn, drop=drop, pool_cap=64)
+    s.run(lsim_amd.Simulator.Mode.PARTITIONED)
+    st = s.last_run_stats()
+    assert st["sweep_reruns"] > 0
+    det = ref["det"]
+    assert np.array_equal(s.first_detect_round(), np.where(det.any(1), det.argmax(1), -1))
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    if not drop:
+        assert np.array_equal(s.detect_matrix(), det)

@@ -179,3 +179,13 @@ def test_stem_regions_on_generated_circuit():
     for p in range(P, P + G):
         if not reach[p]:
             assert all(not reach[q] for q in fo[p])
+
+
+def test_trace_flag_off_values(monkeypatch):
+    """LSIM_TRACE=0 or empty leaves tracing off (the library reads it once per process)."""
+    import subprocess, sys
+    code = "import lsim_amd; print(lsim_amd.lib() is not None)"
+    for v in ("0", ""):
+        env = dict(__import__("os").environ, LSIM_TRACE=v)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0 and "[lsim]" not in r.stderr
