@@ -230,6 +230,19 @@ typedef struct {
 LSIM_API int lsim_generate(const lsim_gen_spec *spec, char **ckt, size_t *ckt_len, char **flst, size_t *flst_len, char **ptn,
                   size_t *ptn_len);
 LSIM_API void lsim_free_text(char *p);
+
+/* Loop removal by traversal from the PIs, the rule the reference's netlist
+ * producer applies before it writes a .ckt (fsim/parser.py:163-211, its
+ * 'bfs' and 'dfs' modes): nodes are taken from a queue seeded with the PIs,
+ * from the front (BFS) or the back (DFS); a node taken before is skipped;
+ * otherwise its wires into nodes taken before are dropped and its other
+ * destinations are queued. A node's wires are its first wire record. Returns
+ * the new .ckt text (free with lsim_free_text) and the number of dropped
+ * (destination, pin) entries. */
+#define LSIM_ORDER_BFS 0
+#define LSIM_ORDER_DFS 1
+LSIM_API int lsim_break_loops(const char *ckt, size_t ckt_len, int mode, char **out, size_t *out_len,
+                              int64_t *removed);
 /* The fault list or the patterns alone (same seeds and rules as lsim_generate):
  * SA0/SA1 over inner gate ids [P+O, P+O+G) thinned to `fraction`, or
  * ceil(n/64) rounds of P random words. */

@@ -158,6 +158,8 @@ def lib() -> ctypes.CDLL:
         "lsim_server_release_job": (ctypes.c_int, [P, ctypes.c_int64]),
         "lsim_server_counters": (ctypes.c_int, [P, i64p]),
         "lsim_server_set_option": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int64]),
+        "lsim_break_loops": (ctypes.c_int, [ctypes.c_char_p, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(sz), i64p]),
         "lsim_generate_faults": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                 ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_generate_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
@@ -376,6 +378,17 @@ def _take_text(rc: int, buf: ctypes.c_void_p, n: ctypes.c_size_t) -> bytes:
     out = ctypes.string_at(buf.value, n.value)
     L.lsim_free_text(buf)
     return out
+
+
+def break_loops(ckt, mode: str = "bfs"):
+    """(.ckt text, dropped entries): wires into nodes already taken by a BFS or
+    DFS traversal from the PIs are dropped (fsim/parser.py:163-211)."""
+    c = _as_bytes(ckt)
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    removed = ctypes.c_int64()
+    m = {"bfs": 0, "dfs": 1}[mode]
+    rc = lib().lsim_break_loops(c, len(c), m, ctypes.byref(b), ctypes.byref(n), ctypes.byref(removed))
+    return _take_text(rc, b, n), removed.value
 
 
 def generate_faults(seed: int, num_pis: int, num_pos: int, num_gates: int, fraction: float = 1.0) -> bytes:

@@ -15,6 +15,7 @@
 
 #include "engine.hpp"
 #include "generator.hpp"
+#include "loops.hpp"
 #include "lsim.h"
 #include "netlist.hpp"
 
@@ -959,6 +960,17 @@ int lsim_generate_faults(uint64_t seed, int64_t num_pis, int64_t num_pos, int64_
   std::string text;
   int rc = guarded(&g_create_error, [&] { text = lsim::generate_flst(seed, num_pis, num_pos, num_gates, fraction); });
   return rc ? rc : emit_text(text, flst, flst_len);
+}
+
+int lsim_break_loops(const char *ckt, size_t ckt_len, int mode, char **out, size_t *out_len, int64_t *removed) {
+  if (!out || !out_len || (!ckt && ckt_len) || (mode != LSIM_ORDER_BFS && mode != LSIM_ORDER_DFS)) return LSIM_E_ARG;
+  *out = nullptr;
+  std::string text;
+  lsim::LoopStats st;
+  int rc = guarded(&g_create_error, [&] { text = lsim::break_loops(ckt, ckt_len, mode == LSIM_ORDER_DFS, &st); });
+  if (rc) return rc;
+  if (removed) *removed = st.removed;
+  return emit_text(text, out, out_len);
 }
 
 int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn, size_t *ptn_len) {
