@@ -85,7 +85,9 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *  "autotune"           0/1  successive runs of a job set each try one (kernel,
  *                            word schedule) candidate, then keep the fastest.
  *  "good_graph"         0/1  a lone job replays its circuit's captured
- *                            good-machine graph (default 1). */
+ *                            good-machine graph (default 1).
+ *  "good_form"          good machine of a lone job: 0 one launch per level
+ *                            (default), 1 LDS-staged level windows. */
 LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
 
 /* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).

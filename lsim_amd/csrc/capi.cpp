@@ -151,6 +151,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
     eo.pool_cap = value < 0 ? 0 : value;
   } else if (k == "autotune") {
     eo.autotune = value != 0;
+  } else if (k == "good_form") {
+    if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "good_form must be 0 or 1");
+    eo.good_form = int(value);
   } else if (k == "good_graph") {
     eo.good_graph = value != 0;
   } else if (k == "sweep_kernel") {

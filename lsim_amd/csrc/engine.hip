@@ -243,6 +243,61 @@ __global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const i
 }
 
 // Masked good values of one round, gathered per device position.
+// Level-window form of the good machine. A window is kWinLevels consecutive
+// levels; its nodes are split into blocks, each closed under fanins inside
+// the window (the backward cones of the window's outputs, shared nodes
+// repeated in every block that needs them, each written by one owner). A
+// workgroup evaluates one block for 64 words level by level, keeping the
+// block's values in LDS and reading only fanins of earlier windows from HBM.
+constexpr int kWinLevels = 4;
+constexpr int kWinNodes = 128;  // a block's values for 64 words: 64 KB of LDS
+
+struct WinNode {
+  int32_t u;      // device position
+  uint32_t info;  // type | fanin count << 4 | owner << 8
+  int4 fin;       // fanin position, or ~(block index) for a fanin inside the block; PI index for a PI
+};
+
+struct WinBlock {
+  int32_t node0;               // first WinNode of the block
+  int32_t seg[kWinLevels + 1];  // seg[k]: first block node on the window's k-th level; seg[kWinLevels]: count
+};
+
+__global__ void __launch_bounds__(512) k_good_window(const JobDev *__restrict__ jd, const WinBlock *__restrict__ blocks,
+                                                     const WinNode *__restrict__ nodes, int block0, int wgroups) {
+  __shared__ uint64_t s_val[kWinNodes][64];
+  const JobDev &J = jd[0];
+  const WinBlock B = blocks[block0 + int(blockIdx.x) / wgroups];
+  const int g = int(blockIdx.x) % wgroups;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int64_t RW = J.RW;
+  const int64_t w = int64_t(g) * 64 + lane;
+  const bool wok = w < RW;
+  const uint64_t *good = J.good;
+  for (int k = 0; k < kWinLevels; ++k) {
+    for (int i = B.seg[k] + wave; i < B.seg[k + 1]; i += nwaves) {
+      const WinNode n = nodes[B.node0 + i];
+      const uint32_t type = n.info & 15u, nin = (n.info >> 4) & 7u;
+      uint64_t r = 0;
+      if (wok) {
+        if (type == G_PI) {
+          r = J.pi_words[int64_t(n.fin.x) * RW + w];
+        } else {
+          const int fi[4] = {n.fin.x, n.fin.y, n.fin.z, n.fin.w};
+          uint64_t v[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q < int(nin)) v[q] = fi[q] < 0 ? s_val[~fi[q]][lane] : good[int64_t(fi[q]) * RW + w];
+          r = eval_gate(type, nin, v[0], v[1], v[2], v[3]);
+        }
+        if ((n.info >> 8) & 1u) J.good[int64_t(n.u) * RW + w] = r;
+      }
+      s_val[i][lane] = r;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_gather_round(const uint64_t *__restrict__ good, int64_t RW, int64_t w, int N, uint64_t mask,
                                uint64_t *__restrict__ out) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1210,6 +1265,7 @@ void parallel_for(const std::vector<int> &order, Fn &&fn) {
 // replay) and per-level item offsets from `off`. Only kernel nodes.
 struct GoodGraph {
   int64_t R = 0;
+  int form = 0;  // 0: one launch per level, 1: level windows
   DevBuf<JobDev> job;
   DevBuf<int64_t> off;
   hipGraph_t graph = nullptr;
@@ -1220,9 +1276,20 @@ struct GoodGraph {
   }
 };
 
+// Level-window blocks of one circuit (built on first use).
+struct WindowPlan {
+  std::vector<int32_t> win_block;  // window k holds blocks [win_block[k], win_block[k+1])
+  DevBuf<WinBlock> blocks;
+  DevBuf<WinNode> nodes;
+  int64_t repeated = 0;  // block nodes beyond one per circuit node
+};
+
 struct CircuitRes {
   int64_t N = 0, NL = 0, P = 0, G = 0, O = 0, E = 0;
   int L = 0;
+  std::vector<uint32_t> h_info;  // per device position (kept for the window partitioner)
+  std::vector<int4> h_fin;
+  std::unique_ptr<WindowPlan> win;
   std::vector<std::unique_ptr<GoodGraph>> graphs;  // most recent last, at most kMaxGraphs
   bool wide = false;  // mean live nodes per level above 256 (the pair sweep then uses 512-thread groups)
   bool has_mux = false;
@@ -1319,7 +1386,8 @@ struct Engine::Impl {
     if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
   }
   JobDev job_dev(JobState &j) const;
-  GoodGraph *good_graph(CircuitRes &c, int64_t R);
+  GoodGraph *good_graph(CircuitRes &c, int64_t R, int form);
+  void build_window_plan(CircuitRes &c);
   void finish_start(const std::vector<JobState *> &js, double ms);
   int64_t iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo);
   // Autotuner: sweep kernel and dropping word schedule. Per job-set
@@ -1502,6 +1570,8 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   m.h2d(c.fo.p, fo.data(), fo.size() * sizeof(int2));
   m.h2d(c.is_po.p, c.head_is_po.data(), c.head_is_po.size());
   LSIM_HIP(hipStreamSynchronize(m.stream));
+  c.h_info = std::move(info);
+  c.h_fin = std::move(fin);
   m.circuits.push_back(std::move(cp));
   return m.circuits.back().get();
 }
@@ -1633,9 +1703,134 @@ int64_t good_items(const CircuitRes &c, int64_t R, int l) {
 }
 }  // namespace
 
-GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R) {
+// Splits every window of kWinLevels levels into blocks of at most kWinNodes
+// nodes closed under in-window fanins: the window's outputs (nodes with no
+// fanout inside it) are taken in position order and each adds its in-window
+// backward cone to the current block, which is closed when the cone would
+// overflow it. A cone of kWinLevels levels with at most 4 fanins per node has
+// at most 85 nodes, so it always fits one block. A node's first block owns it.
+void Engine::Impl::build_window_plan(CircuitRes &c) {
+  auto plan = std::make_unique<WindowPlan>();
+  const int64_t N = c.N;
+  auto fanins = [&](int32_t u, int32_t *f) {
+    const uint32_t type = c.h_info[size_t(u)] & 15u, nin = (c.h_info[size_t(u)] >> 4) & 7u;
+    if (type == G_PI) return 0;
+    const int4 v = c.h_fin[size_t(u)];
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+    f[3] = v.w;
+    return int(nin);
+  };
+  // A node is an output of its window when its lowest fanout level lies
+  // beyond the window (fanouts sit on higher levels than the node).
+  std::vector<int32_t> min_fo_level(size_t(N), INT32_MAX);
+  for (int32_t u = 0; u < int32_t(N); ++u) {
+    int32_t f[4];
+    const int k = fanins(u, f);
+    for (int q = 0; q < k; ++q)
+      min_fo_level[size_t(f[q])] = std::min(min_fo_level[size_t(f[q])], c.level_dp[size_t(u)]);
+  }
+  std::vector<WinBlock> blocks;
+  std::vector<WinNode> nodes;
+  std::vector<uint8_t> owned(size_t(N), 0);
+  std::vector<int32_t> in_block(size_t(N), -1);  // index in the current block, by block id stamp
+  std::vector<int32_t> stamp(size_t(N), -1), cone_stamp(size_t(N), -1);
+  std::vector<int32_t> cur, cone, stack, outs;
+  int32_t block_id = 0, cone_id = 0;
+  auto flush = [&](int l0) {
+    if (cur.empty()) return;
+    if (cur.size() > size_t(kWinNodes)) throw DeviceError(LSIM_E_DEVICE, "window block above its node capacity");
+    std::sort(cur.begin(), cur.end(), [&](int32_t a, int32_t b) {
+      const int32_t la = c.level_dp[size_t(a)], lb = c.level_dp[size_t(b)];
+      return la != lb ? la < lb : a < b;
+    });
+    WinBlock B;
+    B.node0 = int32_t(nodes.size());
+    for (size_t i = 0; i < cur.size(); ++i) in_block[size_t(cur[i])] = int32_t(i);
+    int k = 0;
+    B.seg[0] = 0;
+    for (size_t i = 0; i < cur.size(); ++i) {
+      const int32_t u = cur[i];
+      while (k < kWinLevels - 1 && c.level_dp[size_t(u)] > l0 + k) B.seg[++k] = int32_t(i);
+      WinNode n;
+      n.u = u;
+      const bool own = !owned[size_t(u)];
+      owned[size_t(u)] = 1;
+      n.info = (c.h_info[size_t(u)] & 0xffu) | (own ? (1u << 8) : 0u);
+      int32_t f[4] = {0, 0, 0, 0};
+      const int nin = fanins(u, f);
+      int e[4] = {0, 0, 0, 0};
+      if ((c.h_info[size_t(u)] & 15u) == G_PI) {
+        e[0] = c.h_fin[size_t(u)].x;
+      } else {
+        for (int q = 0; q < nin; ++q)
+          e[q] = (stamp[size_t(f[q])] == block_id) ? ~in_block[size_t(f[q])] : f[q];
+      }
+      n.fin = make_int4(e[0], e[1], e[2], e[3]);
+      nodes.push_back(n);
+    }
+    while (k < kWinLevels) B.seg[++k] = int32_t(cur.size());
+    blocks.push_back(B);
+    cur.clear();
+    ++block_id;
+  };
+  for (int l0 = 0; l0 <= c.L; l0 += kWinLevels) {
+    const int l1 = std::min(c.L, l0 + kWinLevels - 1);
+    plan->win_block.push_back(int32_t(blocks.size()));
+    outs.clear();
+    for (int l = l0; l <= l1; ++l) {
+      for (int32_t u = c.live_lvl[size_t(l)]; u < c.live_lvl[size_t(l) + 1]; ++u)
+        if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
+      for (int32_t u = c.dead_lvl[size_t(l)]; u < c.dead_lvl[size_t(l) + 1]; ++u)
+        if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
+    }
+    std::sort(outs.begin(), outs.end());
+    for (int32_t o : outs) {
+      // in-window backward cone of o
+      cone.clear();
+      stack.clear();
+      stack.push_back(o);
+      cone_stamp[size_t(o)] = cone_id;
+      while (!stack.empty()) {
+        const int32_t u = stack.back();
+        stack.pop_back();
+        cone.push_back(u);
+        int32_t f[4];
+        const int k = fanins(u, f);
+        for (int q = 0; q < k; ++q)
+          if (c.level_dp[size_t(f[q])] >= l0 && cone_stamp[size_t(f[q])] != cone_id) {
+            cone_stamp[size_t(f[q])] = cone_id;
+            stack.push_back(f[q]);
+          }
+      }
+      ++cone_id;
+      int64_t fresh = 0;
+      for (int32_t u : cone) fresh += stamp[size_t(u)] == block_id ? 0 : 1;
+      if (int64_t(cur.size()) + fresh > kWinNodes) flush(l0);
+      for (int32_t u : cone)
+        if (stamp[size_t(u)] != block_id) {
+          stamp[size_t(u)] = block_id;
+          cur.push_back(u);
+        }
+    }
+    flush(l0);
+  }
+  plan->win_block.push_back(int32_t(blocks.size()));
+  for (int64_t u = 0; u < N; ++u)
+    if (!owned[size_t(u)]) throw DeviceError(LSIM_E_DEVICE, "window partition left a node without a block");
+  plan->repeated = int64_t(nodes.size()) - N;
+  plan->blocks.alloc(std::max<size_t>(blocks.size(), 1));
+  plan->nodes.alloc(std::max<size_t>(nodes.size(), 1));
+  h2d(plan->blocks.p, blocks.data(), blocks.size() * sizeof(WinBlock));
+  h2d(plan->nodes.p, nodes.data(), nodes.size() * sizeof(WinNode));
+  LSIM_HIP(hipStreamSynchronize(stream));
+  c.win = std::move(plan);
+}
+
+GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
   for (size_t i = 0; i < c.graphs.size(); ++i)
-    if (c.graphs[i]->R == R) {
+    if (c.graphs[i]->R == R && c.graphs[i]->form == form) {
       std::unique_ptr<GoodGraph> g = std::move(c.graphs[i]);
       c.graphs.erase(c.graphs.begin() + long(i));
       c.graphs.push_back(std::move(g));
@@ -1643,17 +1838,30 @@ GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R) {
     }
   auto g = std::make_unique<GoodGraph>();
   g->R = R;
+  g->form = form;
   g->job.alloc(1);
   std::vector<int64_t> off(size_t(c.L + 1) * 2, 0);
   for (int l = 0; l <= c.L; ++l) off[size_t(l) * 2 + 1] = good_items(c, R, l);
   g->off.alloc(off.size());
   h2d(g->off.p, off.data(), off.size() * sizeof(int64_t));
+  if (form == 1 && !c.win) build_window_plan(c);
   LSIM_HIP(hipStreamSynchronize(stream));
   LSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  for (int l = 0; l <= c.L; ++l) {
-    const int64_t total = off[size_t(l) * 2 + 1];
-    if (!total) continue;
-    k_good_level<<<blocks_for(total, 256), 256, 0, stream>>>(g->job.p, 1, g->off.p + size_t(l) * 2, l);
+  if (form == 1) {
+    const int wg = int((R + 63) / 64);
+    const WindowPlan &wp = *c.win;
+    for (size_t k = 0; k + 1 < wp.win_block.size(); ++k) {
+      const int nb = wp.win_block[k + 1] - wp.win_block[k];
+      if (!nb) continue;
+      k_good_window<<<unsigned(int64_t(nb) * wg), 512, 0, stream>>>(g->job.p, wp.blocks.p, wp.nodes.p,
+                                                                   wp.win_block[k], wg);
+    }
+  } else {
+    for (int l = 0; l <= c.L; ++l) {
+      const int64_t total = off[size_t(l) * 2 + 1];
+      if (!total) continue;
+      k_good_level<<<blocks_for(total, 256), 256, 0, stream>>>(g->job.p, 1, g->off.p + size_t(l) * 2, l);
+    }
   }
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(stream, &graph);
@@ -1678,7 +1886,7 @@ void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo)
   if (js.size() == 1 && eo.good_graph && js[0]->R > 0) {
     // One job: replay its circuit's captured good-machine graph.
     JobState &j = *js[0];
-    GoodGraph *g = m.good_graph(*j.c, j.R);
+    GoodGraph *g = m.good_graph(*j.c, j.R, eo.good_form);
     const JobDev d = m.job_dev(j);
     m.time_begin();
     m.h2d(g->job.p, &d, sizeof(JobDev));

@@ -161,3 +161,17 @@ def test_pool_overflow_reruns(c1, drop):
     assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
     if not drop:
         assert np.array_equal(s.detect_matrix(), det)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["per_level", "windows"])
+def test_good_machine_forms_agree(c2, form):
+    """The good machine of a lone job, one launch per level or in LDS-staged
+    level windows: identical last-round good values and detection on C2
+    (100k gates, 120 levels, 128 rounds)."""
+    ckt, flst, ptn = c2
+    s = _sim(ckt, flst, ptn, good_form=form)
+    s.run(lsim_amd.Simulator.Mode.PARTITIONED)
+    other = _sim(ckt, flst, ptn, good_form=1 - form, good_graph=0)
+    other.run(lsim_amd.Simulator.Mode.PARTITIONED)
+    assert all(np.array_equal(a, b) for a, b in zip(s.good_results(), other.good_results()))
+    assert np.array_equal(s.detect_matrix(), other.detect_matrix())
