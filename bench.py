@@ -107,9 +107,15 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
     t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
     bytes_launch = agg["sweep_bytes"] / launches
     achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
-    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(config, kernel),
-            "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
+    traffic = profiled_traffic(config, kernel)
+    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
+    if traffic and t_launch_s > 0:
+        # rocprof-measured HBM bytes per launch over this run's launch time
+        out["traffic_gbs"] = traffic / t_launch_s / 1e9
+        out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+    return out
 
 
 def _kernel_name(args, tuned=None) -> str:
