@@ -283,6 +283,10 @@ def bench_serving(args) -> int:
         exact = exact and np.array_equal(srv.first_detect_round(jid), first[k])
     cnt = srv.counters()
     srv.close()
+    slowest = sorted(range(len(jobs)), key=lambda k: -lat[k])[:5]
+    tail = [{"latency_ms": round(lat[k] * 1e3, 1), "unloaded_ms": round(alone[k] * 1e3, 1),
+             "faults": F[k], "patterns": jobs[k]["num_patterns"],
+             "gates": circuits[jobs[k]["circuit"]]["num_gates"]} for k in slowest]
     p99_alone = _pct(alone, 99)
     out = {
         "metric": "p99 latency at fixed QPS",
@@ -304,6 +308,7 @@ def bench_serving(args) -> int:
         "throughput": {"jobs_per_s": len(jobs) / t_run, "evals_per_s": evals / t_run,
                        "saturated_jobs_per_s": rate_sat, "saturated_evals_per_s": evals / t_sat},
         "bit_exact_alone_vs_mix": bool(exact),
+        "slowest_jobs": tail,
         "server": cnt,
     }
     print(json.dumps(out), flush=True)

@@ -188,9 +188,13 @@ LSIM_API int lsim_batch_release_job(lsim_batch *b, int64_t job_id);
 /* ------------------------------------------------------------------------
  * Serving (no reference counterpart): circuits stay resident; jobs are
  * submitted from any thread and admitted in arrival order while their HBM
- * footprint fits the budget; a worker thread advances all admitted jobs one
- * word step per iteration (continuous batching), so a job that arrives while
- * others run joins at the next iteration. */
+ * footprint fits the budget. Two lanes, each an engine with its own stream
+ * and worker thread, run concurrently on the device: small jobs are batched
+ * continuously (every admitted job advances one word step per iteration, so
+ * a job that arrives while others run joins at the next one); large jobs run
+ * one at a time in arrival order. While both lanes have work the small-job
+ * lane's sweeps take a quarter of the workgroup slots and the other lane the
+ * rest. */
 typedef struct lsim_server lsim_server;
 /* job_budget_mb: HBM for admitted jobs (good values, fault state, results);
  * scratch_mb: HBM for sweep tiles (0 = 90 % of free HBM less the job budget and 4 GiB). */
@@ -210,8 +214,12 @@ LSIM_API int lsim_server_job_times(lsim_server *s, int64_t job_id, double *times
 LSIM_API int lsim_server_release_job(lsim_server *s, int64_t job_id);
 /* counters[0..3] = iterations, jobs finished, jobs failed, peak admitted jobs. */
 LSIM_API int lsim_server_counters(lsim_server *s, int64_t *counters);
-/* Engine options as lsim_set_option, and the word schedule of jobs submitted
- * later: "first_words" (default 1) and "word_growth" (default 4). */
+/* Engine options as lsim_set_option; the word schedule of jobs submitted
+ * later: "first_words" (default 1) and "word_growth" (default 4); and
+ * "big_evals" and "big_node_words": jobs with at least that many fault x
+ * pattern evaluations (default 4e9), or circuit nodes x 64-pattern rounds
+ * (default 5e7), run in the large-job lane, one at a time in arrival order
+ * ("big_evals" 0 puts every job in the batching lane). */
 LSIM_API int lsim_server_set_option(lsim_server *s, const char *key, int64_t value);
 
 /* Seeded synthetic workload in the reference formats (no reference

@@ -481,8 +481,11 @@ class Batch:
 
 
 class Server:
-    """Resident circuits and a worker thread that admits submitted jobs by HBM
-    footprint and advances all admitted jobs together (continuous batching)."""
+    """Resident circuits and two lanes, each an engine with its own stream and
+    worker thread: small jobs are batched continuously (a job joins at the
+    next word step), large ones (fault x pattern evaluations from the
+    `big_evals` option, default 4e9) run one at a time in arrival order.
+    Admission in arrival order by HBM footprint."""
 
     def __init__(self, device: int = 0, job_budget_mb: int = 64 * 1024, scratch_mb: int = 0):
         self._L = lib()

@@ -85,6 +85,7 @@ struct ResidentCircuit {
   lsim::Netlist nl;
   lsim::FaultStructure fs;
   lsim::CircuitRes *res = nullptr;
+  lsim::CircuitRes *res2 = nullptr;  // the server's second lane
 };
 
 }  // namespace
@@ -576,6 +577,7 @@ int lsim_batch_release_job(lsim_batch *b, int64_t job_id) {
 struct ServerJob {
   int64_t id = 0;
   int64_t circuit = -1;
+  int lane = 0;
   std::vector<int32_t> site;
   std::vector<uint8_t> sa;
   int64_t n = 0, R = 0;
@@ -590,65 +592,76 @@ struct ServerJob {
   lsim::RunResults res;
 };
 
-struct lsim_server {
+// A lane: an engine on its own HIP stream with its own worker thread. Lane 0
+// batches small jobs continuously; lane 1 runs large jobs one at a time in
+// arrival order. The lanes run concurrently on the device, each sweep taking
+// a share of the workgroup slots while the other lane has work.
+struct ServerLane {
   std::unique_ptr<lsim::Engine> engine;
   lsim::EngineOptions eopt;
-  int64_t budget = 0, used = 0;
-  std::mutex mu;  // queue, job table, counters
-  std::condition_variable cv_work, cv_done;
-  std::mutex engine_mu;  // every Engine call
-  std::vector<std::unique_ptr<ResidentCircuit>> circuits;
+  std::mutex engine_mu;  // every call on this lane's engine
+  std::condition_variable cv_work;
   std::deque<ServerJob *> queue;
   std::vector<ServerJob *> running;
+  int max_running = 0;  // 0: no limit
+  double busy_share = 1.0;  // slot share while the other lane has work
+  std::thread worker;
+};
+
+struct lsim_server {
+  ServerLane lane[2];
+  int64_t budget = 0, used = 0;
+  std::mutex mu;  // queues, job table, counters, budget
+  std::condition_variable cv_done;
+  std::vector<std::unique_ptr<ResidentCircuit>> circuits;
   std::map<int64_t, std::unique_ptr<ServerJob>> jobs;
   int64_t next_id = 0;
   int64_t iterations = 0, finished = 0, failed = 0, peak = 0;
   int first_words = 1, word_growth = 4;
+  double big_evals = 4e9;       // fault x pattern evaluations from which a job is large (<= 0: one lane)
+  double big_node_words = 5e7;  // circuit nodes x rounds from which a job is large
   bool stop = false;
-  std::thread worker;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   std::string err;
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
-  void loop();
+  bool lane_has_work(int l) const { return !lane[l].queue.empty() || !lane[l].running.empty(); }
+  void loop(int l);
 };
 
-void lsim_server::loop() {
+void lsim_server::loop(int l) {
+  ServerLane &L = lane[l];
   for (;;) {
     std::vector<ServerJob *> admit;
+    lsim::EngineOptions eo;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+      L.cv_work.wait(lk, [&] {
+        if (stop || !L.running.empty()) return true;
+        return !L.queue.empty() && used + L.queue.front()->bytes <= budget;
+      });
       if (stop) return;
-      // Admission in arrival order while the footprint fits the budget; a job
-      // that cannot fit even on an idle device fails instead of blocking.
-      while (!queue.empty()) {
-        ServerJob *j = queue.front();
-        if (used + j->bytes <= budget) {
-          used += j->bytes;
-          admit.push_back(j);
-          queue.pop_front();
-        } else if (running.empty() && admit.empty()) {
-          j->state = 3;
-          j->code = LSIM_E_NOMEM;
-          j->error = "job needs " + std::to_string(j->bytes) + " bytes, above the server's job budget";
-          j->t_finish = now();
-          ++failed;
-          queue.pop_front();
-          cv_done.notify_all();
-        } else {
-          break;
-        }
+      // Admission in arrival order while the footprint fits the budget (a
+      // job larger than the whole budget was failed at submit).
+      while (!L.queue.empty() && (L.max_running == 0 || int(L.running.size() + admit.size()) < L.max_running)) {
+        ServerJob *j = L.queue.front();
+        if (used + j->bytes > budget) break;
+        used += j->bytes;
+        admit.push_back(j);
+        L.queue.pop_front();
       }
+      eo = L.eopt;
+      eo.slot_share = lane_has_work(1 - l) ? L.busy_share : 1.0;
     }
     std::vector<ServerJob *> done_now;
     std::vector<ServerJob *> failed_now;
     {
-      std::lock_guard<std::mutex> elk(engine_mu);
+      std::lock_guard<std::mutex> elk(L.engine_mu);
       std::vector<ServerJob *> fresh;
       for (ServerJob *j : admit) {
+        const ResidentCircuit &c = *circuits[size_t(j->circuit)];
         int rc = guarded(&j->error, [&] {
-          j->js = engine->add_job(circuits[size_t(j->circuit)]->res, std::move(j->site), std::move(j->sa), j->n, j->R,
-                                  j->words, j->opt);
+          j->js = L.engine->add_job(l == 0 ? c.res : c.res2, std::move(j->site), std::move(j->sa), j->n, j->R,
+                                    j->words, j->opt);
         });
         j->words.clear();
         j->words.shrink_to_fit();
@@ -662,15 +675,15 @@ void lsim_server::loop() {
       }
       std::vector<lsim::JobState *> js;
       std::vector<ServerJob *> live;
-      for (ServerJob *j : running) live.push_back(j);
+      for (ServerJob *j : L.running) live.push_back(j);
       for (ServerJob *j : fresh) live.push_back(j);
       for (ServerJob *j : live) js.push_back(j->js);
       std::string msg;
       int rc = guarded(&msg, [&] {
         std::vector<lsim::JobState *> fs;
         for (ServerJob *j : fresh) fs.push_back(j->js);
-        engine->start(fs, eopt);
-        engine->iterate(js, eopt);
+        L.engine->start(fs, eo);
+        L.engine->iterate(js, eo);
       });
       std::vector<ServerJob *> still;
       for (ServerJob *j : live) {
@@ -678,12 +691,12 @@ void lsim_server::loop() {
           // The iteration failed: its jobs fail, the server keeps serving.
           j->code = rc;
           j->error = msg;
-          engine->release_job(j->js);
+          L.engine->release_job(j->js);
           j->js = nullptr;
           failed_now.push_back(j);
-        } else if (engine->done(j->js)) {
-          j->res = std::move(engine->results(j->js));
-          engine->release_job(j->js);
+        } else if (L.engine->done(j->js)) {
+          j->res = std::move(L.engine->results(j->js));
+          L.engine->release_job(j->js);
           j->js = nullptr;
           done_now.push_back(j);
         } else {
@@ -691,8 +704,8 @@ void lsim_server::loop() {
         }
       }
       std::lock_guard<std::mutex> lk(mu);
-      running.swap(still);
-      peak = std::max<int64_t>(peak, int64_t(running.size() + done_now.size()));
+      L.running.swap(still);
+      peak = std::max<int64_t>(peak, int64_t(lane[0].running.size() + lane[1].running.size() + done_now.size()));
       ++iterations;
       const double t = now();
       for (ServerJob *j : done_now) {
@@ -708,7 +721,10 @@ void lsim_server::loop() {
         ++failed;
       }
     }
-    if (!done_now.empty() || !failed_now.empty()) cv_done.notify_all();
+    if (!done_now.empty() || !failed_now.empty()) {
+      cv_done.notify_all();
+      lane[1 - l].cv_work.notify_all();  // budget freed for the other lane
+    }
   }
 }
 
@@ -720,20 +736,39 @@ int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, ls
   std::unique_ptr<lsim_server> s(new (std::nothrow) lsim_server);
   if (!s) return LSIM_E_NOMEM;
   int rc = guarded(&s->err, [&] {
-    s->engine.reset(new lsim::Engine(device));
+    for (ServerLane &L : s->lane) L.engine.reset(new lsim::Engine(device));
     s->budget = job_budget_mb << 20;
-    s->eopt.mode = LSIM_MODE_PARTITIONED;
-    if (scratch_mb > 0) {
-      s->eopt.scratch_mb = scratch_mb;
-    } else {
+    int64_t scratch = scratch_mb;
+    if (scratch <= 0) {
       // Free HBM less the job budget and a 4 GiB margin for resident circuits.
-      const int64_t free_mb = s->engine->free_bytes() >> 20;
-      s->eopt.scratch_mb = std::max<int64_t>(1024, (free_mb - job_budget_mb - 4096) / 10 * 9);
+      const int64_t free_mb = s->lane[0].engine->free_bytes() >> 20;
+      scratch = std::max<int64_t>(2048, (free_mb - job_budget_mb - 4096) / 10 * 9);
     }
-    s->worker = std::thread([p = s.get()] { p->loop(); });
+    // Small jobs: 30 % of the scratch and a quarter of the slots while large
+    // jobs run; large jobs: 70 % and the other three quarters.
+    s->lane[0].eopt.scratch_mb = std::max<int64_t>(512, scratch * 3 / 10);
+    s->lane[1].eopt.scratch_mb = std::max<int64_t>(512, scratch - s->lane[0].eopt.scratch_mb);
+    s->lane[0].busy_share = 0.25;
+    s->lane[1].busy_share = 0.75;
+    s->lane[1].max_running = 1;
+    for (ServerLane &L : s->lane) {
+      L.eopt.mode = LSIM_MODE_PARTITIONED;
+      // Serving jobs rarely repeat a circuit's round count, so a captured
+      // good-machine graph would seldom be replayed.
+      L.eopt.good_graph = false;
+    }
+    for (int l = 0; l < 2; ++l) s->lane[l].worker = std::thread([p = s.get(), l] { p->loop(l); });
   });
   if (rc) {
     g_create_error = s->err;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    for (ServerLane &L : s->lane) {
+      L.cv_work.notify_all();
+      if (L.worker.joinable()) L.worker.join();
+    }
     return rc;
   }
   *out = s.release();
@@ -746,13 +781,17 @@ void lsim_server_destroy(lsim_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
   }
-  s->cv_work.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  for (ServerLane &L : s->lane) {
+    L.cv_work.notify_all();
+    if (L.worker.joinable()) L.worker.join();
+  }
   try {
-    std::lock_guard<std::mutex> elk(s->engine_mu);
-    for (auto &kv : s->jobs)
-      if (kv.second->js) s->engine->release_job(kv.second->js);
-    s->engine.reset();
+    for (int l = 0; l < 2; ++l) {
+      std::lock_guard<std::mutex> elk(s->lane[l].engine_mu);
+      for (auto &kv : s->jobs)
+        if (kv.second->js && kv.second->lane == l) s->lane[l].engine->release_job(kv.second->js);
+      s->lane[l].engine.reset();
+    }
   } catch (...) {
   }
   delete s;
@@ -767,8 +806,12 @@ int lsim_server_add_circuit(lsim_server *s, const char *ckt, size_t ckt_len, int
   int rc = guarded(&msg, [&] {
     lsim::read_circuit(rc_ptr->nl, ckt, ckt_len);
     lsim::build_fault_structure(rc_ptr->nl, rc_ptr->fs);
-    std::lock_guard<std::mutex> elk(s->engine_mu);
-    rc_ptr->res = s->engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
+    {
+      std::lock_guard<std::mutex> elk(s->lane[0].engine_mu);
+      rc_ptr->res = s->lane[0].engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
+    }
+    std::lock_guard<std::mutex> elk(s->lane[1].engine_mu);
+    rc_ptr->res2 = s->lane[1].engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
   });
   std::lock_guard<std::mutex> lk(s->mu);
   if (rc) {
@@ -811,9 +854,26 @@ int lsim_server_submit(lsim_server *s, int64_t circuit_id, const char *flst, siz
   j->t_submit = s->now();
   j->id = s->next_id++;
   *job_id = j->id;
-  s->queue.push_back(j.get());
+  // Large: many fault x pattern evaluations, or a large good machine (nodes
+  // x rounds), which also sets how far every sweep tile walks.
+  const double evals = double(j->site.size()) * double(j->n);
+  const double node_words = double(c->nl.N) * double(j->R);
+  j->lane = (s->big_evals > 0 && (evals >= s->big_evals || node_words >= s->big_node_words)) ? 1 : 0;
+  ServerJob *jp = j.get();
   s->jobs[j->id] = std::move(j);
-  s->cv_work.notify_all();
+  if (jp->bytes > s->budget) {
+    // Larger than the whole job budget: it could never be admitted.
+    jp->state = 3;
+    jp->code = LSIM_E_NOMEM;
+    jp->error = "job needs " + std::to_string(jp->bytes) + " bytes, above the server's job budget";
+    jp->t_finish = s->now();
+    ++s->failed;
+    s->cv_done.notify_all();
+    return LSIM_OK;
+  }
+  s->lane[jp->lane].queue.push_back(jp);
+  s->lane[jp->lane].cv_work.notify_all();
+  s->lane[1 - jp->lane].cv_work.notify_all();  // the other lane's slot share changes
   return LSIM_OK;
 }
 
@@ -877,16 +937,31 @@ int lsim_server_release_job(lsim_server *s, int64_t job_id) {
 int lsim_server_set_option(lsim_server *s, const char *key, int64_t value) {
   if (!s || !key) return LSIM_E_ARG;
   const std::string k(key);
-  std::lock_guard<std::mutex> elk(s->engine_mu);  // the worker reads eopt under this lock
+  std::lock_guard<std::mutex> elk0(s->lane[0].engine_mu);  // the workers read eopt under these locks
+  std::lock_guard<std::mutex> elk1(s->lane[1].engine_mu);
   std::lock_guard<std::mutex> lk(s->mu);
   if (k == "first_words" || k == "word_growth") {
     if (value < 1 || value > 64) return fail(&s->err, LSIM_E_ARG, k + " must be 1..64");
     (k == "first_words" ? s->first_words : s->word_growth) = int(value);
     return LSIM_OK;
   }
+  if (k == "big_evals") {
+    s->big_evals = double(value);
+    return LSIM_OK;
+  }
+  if (k == "big_node_words") {
+    s->big_node_words = double(value);
+    return LSIM_OK;
+  }
   bool known = false;
-  int rc = set_engine_option(s->eopt, &s->err, k, value, known);
-  if (rc || known) return rc;
+  int rc = 0;
+  for (ServerLane &L : s->lane) {
+    const int64_t keep = L.eopt.scratch_mb;
+    rc = set_engine_option(L.eopt, &s->err, k, value, known);
+    if (k == "scratch_mb") L.eopt.scratch_mb = keep;  // the server splits its scratch at creation
+    if (rc) return rc;
+  }
+  if (known) return LSIM_OK;
   return fail(&s->err, LSIM_E_ARG, "unknown option " + k);
 }
 

@@ -1166,21 +1166,56 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Stream on which device buffers allocated by the calling thread are
+// ordered: an engine's stream while one of its calls runs. Stream-ordered
+// allocation and release never wait for other streams (a plain hipFree waits
+// for the whole device, so one engine's release would wait for another
+// engine's kernels).
+thread_local hipStream_t tl_alloc_stream = nullptr;
+
+struct AllocStream {
+  hipStream_t prev;
+  explicit AllocStream(hipStream_t s) : prev(tl_alloc_stream) { tl_alloc_stream = s; }
+  ~AllocStream() { tl_alloc_stream = prev; }
+  AllocStream(const AllocStream &) = delete;
+  AllocStream &operator=(const AllocStream &) = delete;
+};
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  hipStream_t s = nullptr;  // stream of a stream-ordered allocation, null for a plain one
   void alloc(size_t count) {
     if (count <= n && p) return;
     release();
     if (count == 0) count = 1;
+    if (tl_alloc_stream) {
+      LSIM_HIP(hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), tl_alloc_stream));
+      s = tl_alloc_stream;
+    } else {
+      LSIM_HIP(hipMalloc(&p, count * sizeof(T)));
+      s = nullptr;
+    }
+    n = count;
+  }
+  // Plain allocation, for the large sweep scratch that lives as long as the engine.
+  void alloc_direct(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
     LSIM_HIP(hipMalloc(&p, count * sizeof(T)));
+    s = nullptr;
     n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (s) (void)hipFreeAsync(p, s);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    s = nullptr;
   }
   size_t bytes() const { return n * sizeof(T); }
   DevBuf() = default;
@@ -1330,7 +1365,20 @@ struct JobState {
   RunResults res;
 };
 
+// Owns the engine's stream; declared first in Engine::Impl so that it is
+// destroyed last, after every buffer ordered on the stream is released.
+struct StreamHolder {
+  hipStream_t s = nullptr;
+  ~StreamHolder() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
 struct Engine::Impl {
+  StreamHolder holder;
   int device = 0;
   hipStream_t stream = nullptr;
   int cus = 0;
@@ -1369,7 +1417,7 @@ struct Engine::Impl {
     circuits.clear();
     if (ev[0]) (void)hipEventDestroy(ev[0]);
     if (ev[1]) (void)hipEventDestroy(ev[1]);
-    if (stream) (void)hipStreamDestroy(stream);
+    // The stream itself goes with `holder`, after the member buffers.
   }
   void time_begin() { LSIM_HIP(hipEventRecord(ev[0], stream)); }
   double time_end() {
@@ -1447,6 +1495,14 @@ Engine::Engine(int device) : impl_(new Impl) {
     LSIM_HIP(hipGetDeviceProperties(&prop, device));
     impl_->cus = prop.multiProcessorCount;
     LSIM_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
+    impl_->holder.s = impl_->stream;
+    // Stream-ordered releases stay in the device's pool up to 8 GiB instead of
+    // going back at every synchronisation.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = uint64_t(8) << 30;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     LSIM_HIP(hipEventCreate(&impl_->ev[0]));
     LSIM_HIP(hipEventCreate(&impl_->ev[1]));
     impl_->d_bytes.alloc(1);
@@ -1461,6 +1517,7 @@ Engine::Engine(int device) : impl_(new Impl) {
 Engine::~Engine() {
   if (impl_) {
     (void)hipSetDevice(impl_->device);
+    AllocStream alloc_on(impl_->stream);
     delete impl_;
   }
 }
@@ -1477,6 +1534,7 @@ int64_t Engine::free_bytes() const {
 CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   auto cp = std::make_unique<CircuitRes>();
   CircuitRes &c = *cp;
   c.N = nl.N;
@@ -1579,6 +1637,7 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
 void Engine::remove_circuit(CircuitRes *c) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   LSIM_HIP(hipStreamSynchronize(m.stream));
   for (auto it = m.circuits.begin(); it != m.circuits.end(); ++it)
     if (it->get() == c) {
@@ -1600,6 +1659,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
                           const JobOptions &opt) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   auto jp = std::make_unique<JobState>();
   JobState &j = *jp;
   j.c = c;
@@ -1654,6 +1714,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
 void Engine::release_job(JobState *j) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   LSIM_HIP(hipStreamSynchronize(m.stream));
   for (auto it = m.jobs.begin(); it != m.jobs.end(); ++it)
     if (it->get() == j) {
@@ -1879,6 +1940,7 @@ GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
 void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   std::vector<JobState *> js;
   for (JobState *j : jobs)
     if (!j->started) js.push_back(j);
@@ -2068,9 +2130,9 @@ void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, 
   const int64_t ns = std::max<int64_t>(1, std::min(slots_needed, fit));
   const int64_t pool_words = std::min((budget - ns * per_slot) / 8, ns * nl * 64);
   const auto t0 = std::chrono::steady_clock::now();
-  d_prec.alloc(size_t(ns * nl));
-  d_ppend.alloc(size_t(ns * nl));
-  d_pool.alloc(size_t(pool_words));
+  d_prec.alloc_direct(size_t(ns * nl));
+  d_ppend.alloc_direct(size_t(ns * nl));
+  d_pool.alloc_direct(size_t(pool_words));
   const auto t1 = std::chrono::steady_clock::now();
   // Epoch 0 is never used by a tile: zeroed records and flags are stale.
   LSIM_HIP(hipMemsetAsync(d_prec.p, 0, d_prec.bytes(), stream));
@@ -2109,10 +2171,10 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
     d_masks.release();
     d_mrec.release();
     d_pend.release();
-    d_rows.alloc(size_t(ns) * nl * 64);
-    d_masks.alloc(size_t(ns) * nl);
-    d_mrec.alloc(size_t(ns) * nl);
-    d_pend.alloc(size_t(ns) * nl);
+    d_rows.alloc_direct(size_t(ns) * nl * 64);
+    d_masks.alloc_direct(size_t(ns) * nl);
+    d_mrec.alloc_direct(size_t(ns) * nl);
+    d_pend.alloc_direct(size_t(ns) * nl);
     // Epoch 0 is never used by a tile, so zeroed records read as "no lane differs".
     LSIM_HIP(hipMemsetAsync(d_mrec.p, 0, d_mrec.bytes(), stream));
     LSIM_HIP(hipMemsetAsync(d_pend.p, 0, d_pend.bytes(), stream));
@@ -2270,8 +2332,11 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   a.ovf_count = d_ovf_count.p;
   a.ovf_list = d_ovf_list.p;
   a.tstats = tstats;
-  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ntiles, int64_t(cus) * per_cu),
-                                                               pair_slots));
+  // A share of the device's resident workgroups (another engine on the same
+  // device may hold the rest).
+  const double share = std::min(1.0, std::max(0.0, eo.slot_share));
+  const int64_t resident = std::max<int64_t>(1, int64_t(double(int64_t(cus) * per_cu) * share));
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(ntiles, resident), pair_slots));
   a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_words / grid), int64_t(UINT32_MAX));
   if (eo.pool_cap > 0) a.pool_words = std::min<int64_t>(a.pool_words, eo.pool_cap);
   a.epoch_base = epoch_base;
@@ -2292,7 +2357,8 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   st.sweep_reruns += count;
   // Tiles that outgrew their pool: again, in slots that hold any tile.
   const int64_t g2 = std::max<int64_t>(
-      1, std::min<int64_t>(std::min<int64_t>(count, int64_t(cus) * pc512), std::min(pair_slots, pool_words / full)));
+      1, std::min<int64_t>(std::min<int64_t>(count, std::max<int64_t>(1, int64_t(double(int64_t(cus) * pc512) * share))),
+                           std::min(pair_slots, pool_words / full)));
   d_ovf_list2.alloc(size_t(count));
   a.tile_list = d_ovf_list.p;
   a.ntiles = count;
@@ -2583,6 +2649,7 @@ int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const En
 int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
   return m.iterate_jobs(jobs, eo);
 }
 
