@@ -180,3 +180,37 @@ def test_autotuned_batch_matches_single_runs():
             assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid)
             b.release_job(jid)
     assert len(seen) == 8  # 2 kernels x 4 word schedules
+
+
+def test_server_two_lanes_match_single_runs():
+    """Jobs split between the batching lane and the one-at-a-time lane (the
+    larger circuits' jobs by node-words), submitted together from several
+    threads: every result equals the single-simulator run."""
+    srv = lsim_amd.Server(device=0, job_budget_mb=16 * 1024, scratch_mb=4096)
+    # jobs on the two largest circuits (nodes x rounds >= 720,000) take the large-job lane
+    srv.set_option("big_node_words", 12000 * 60)
+    circuits = []
+    for spec in SPECS:
+        ckt, flst, ptn = _texts(spec)
+        circuits.append((srv.add_circuit(ckt), ckt, spec))
+    submitted = []
+    lock = threading.Lock()
+
+    def client(k):
+        for c, ck, sp in circuits:
+            _, f2, p2 = _texts(sp, pattern_seed=sp[0] + 11 * k + 3, fraction=0.5)
+            jid = srv.submit(c, f2, p2, drop=True)
+            with lock:
+                submitted.append((jid, ck, f2, p2))
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for jid, ck, f2, p2 in submitted:
+        srv.wait(jid, 300)
+        assert np.array_equal(srv.first_detect_round(jid), _single(ck, f2, p2, True)["first"]), jid
+    cnt = srv.counters()
+    assert cnt["finished"] == len(submitted) and cnt["failed"] == 0
+    srv.close()
