@@ -144,6 +144,8 @@ def bench_batch(args) -> int:
     batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0,
                            sweep_kernel=0 if args.sweep_kernel == "pairs" else 1, autotune=args.autotune,
                            pair_threads=args.pair_threads)
+    if args.pair_dense >= 0:
+        batch.set_option("pair_dense", args.pair_dense)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
@@ -329,6 +331,9 @@ def main() -> int:
                     help="persistent sweep kernel of the partitioned engine (without --autotune)")
     ap.add_argument("--pair-threads", type=int, default=0, choices=[0, 256, 512],
                     help="workgroup size of the pair sweep (0: 512 for dense steps, 256 otherwise)")
+    ap.add_argument("--pair-dense", type=int, default=-1,
+                    help="pair sweep: passes whose nodes average this many active lanes go one node per wave "
+                         "(0: never; -1: the engine default)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
@@ -370,6 +375,8 @@ def main() -> int:
     sim.set_option("sweep_kernel", 0 if args.sweep_kernel == "pairs" else 1)
     sim.set_option("autotune", args.autotune)
     sim.set_option("pair_threads", args.pair_threads)
+    if args.pair_dense >= 0:
+        sim.set_option("pair_dense", args.pair_dense)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")

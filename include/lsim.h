@@ -78,6 +78,9 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *  "sweep_kernel"       persistent sweep: 0 flattened (node, lane) pairs (default),
  *                            1 one node per wave.
  *  "pair_threads"       pair sweep workgroup: 0 by step (default), 256 or 512.
+ *  "pair_dense"         pair sweep: a level pass whose staged nodes average at
+ *                            least this many active lanes evaluates one node per
+ *                            wave (default 32; 0 = always flattened pairs).
  *  "pool_cap"           pair sweep: at most this many pool words per slot
  *                            (default 0 = as memory allows); a tile that needs
  *                            more is swept again in a slot that holds any tile.
@@ -168,7 +171,7 @@ LSIM_API int lsim_batch_create(int device, lsim_batch **out);
 LSIM_API void lsim_batch_destroy(lsim_batch *b);
 LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
 /* Options as lsim_set_option (engine options: "scratch_mb", "block_waves", "sweep_kernel", "pair_threads",
- * "pool_cap", "autotune", "good_graph") and "mode" (0 plain, 1 persistent). */
+ * "pair_dense", "pool_cap", "autotune", "good_graph", "good_form") and "mode" (0 plain, 1 persistent). */
 LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
 /* Makes a circuit (.ckt text) resident; returns its id. */
 LSIM_API int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id);

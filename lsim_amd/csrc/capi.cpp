@@ -150,6 +150,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
     eo.pair_threads = int(value);
   } else if (k == "pool_cap") {
     eo.pool_cap = value < 0 ? 0 : value;
+  } else if (k == "pair_dense") {
+    if (value < 0 || value > 64) return fail(err, LSIM_E_ARG, "pair_dense must be 0..64");
+    eo.pair_dense = int(value);
   } else if (k == "autotune") {
     eo.autotune = value != 0;
   } else if (k == "good_form") {

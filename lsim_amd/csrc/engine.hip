@@ -662,20 +662,26 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 }
 
 // Event form with flattened work. Per pass over a level chunk the workgroup
-//  A. scans the chunk's pending flags and stages each pending node (record,
-//     fanin masks, active lanes = lanes with a differing fanin or an
-//     injection), numbering nodes and their (node, active lane) pairs in one
-//     LDS counter;
+//  A. scans the chunk's pending flags, loading each node's static data with
+//     its flag, and stages each pending node (fanin masks, active lanes =
+//     lanes with a differing fanin or an injection, fanout range), numbering
+//     nodes and their (node, active lane) pairs in one LDS counter;
 //  B. evaluates every (node, active lane) pair, two per thread: lanes where
 //     no fanin differs keep the good value and are never touched;
-//  C. for the nodes that differ somewhere, writes the mask record and numbers
-//     their (node, live fanout) pairs;
-//  D. marks every such fanout pending, one per thread.
-// A level thus costs a few dependent memory round trips however many nodes
-// and lanes it has. A node's injecting lanes are found by a binary search
+//  C. for the nodes that differ somewhere, writes the mask record;
+//  D. marks their live fanouts pending, each wave those of its own nodes.
+// A level pass thus costs four dependent memory round trips (flags, fanin
+// records, words, fanout lists) and three barriers, however many nodes and
+// lanes it has. A node's injecting lanes are found by a binary search
 // over the tile's lane sites in LDS. Pending flags and mask records carry
 // the tile's epoch, so nothing is cleared between tiles.
 constexpr int kPairsPerThread = 2;
+constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
+
+// Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
 
 // Position of the r-th set bit (r < popcount) of m.
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
@@ -770,6 +776,7 @@ struct PairArgs {
   int *ovf_count;    // tiles that outgrew their pool, listed in ovf_list
   int *ovf_list;
   uint32_t epoch_base;
+  int dense_lanes;             // a pass whose nodes average this many active lanes goes one node per wave (0: never)
   unsigned long long *tstats;  // as SweepArgs::tstats, or null
 };
 
@@ -781,15 +788,15 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ uint32_t s_info[CHUNK];
   __shared__ int4 s_fin[CHUNK];
   __shared__ unsigned long long s_m[CHUNK][4], s_fa[CHUNK][4];  // fanin differing / active lanes
-  __shared__ uint32_t s_fb[CHUNK][4];                            // fanin pool bases; phases C-D reuse it
+  __shared__ uint32_t s_fb[CHUNK][4];                            // fanin pool bases
   __shared__ unsigned long long s_act[CHUNK], s_nm[CHUNK];
   __shared__ int s_pref[CHUNK];
+  __shared__ int s_fo0[CHUNK], s_deg[CHUNK];  // live fanout range
   __shared__ int s_tile, s_maxl;
   __shared__ uint32_t s_pool;  // next free word of the slot's pool for this tile
-  __shared__ unsigned long long s_cnt2, s_wcnt2;  // (nodes << 32 | pairs), (written << 32 | fanouts)
+  __shared__ unsigned long long s_cnt2;  // nodes << 32 | pairs
+  __shared__ unsigned long long s_nwt;   // statistics: nodes the tile wrote
   extern __shared__ uint8_t s_lvl[];  // per level: some node of the level is pending
-  int *s_wfo0 = reinterpret_cast<int *>(&s_fb[0][0]);  // phases C-D: written node fanout start
-  int *s_epref = s_wfo0 + CHUNK;                       // phases C-D: fanout prefix
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   PairRec *rec = a.rec + int64_t(blockIdx.x) * a.rec_stride;
@@ -798,7 +805,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
   for (int i = tid; i < a.lvl_entries; i += CHUNK) s_lvl[i] = 0;
   if (tid == 0) {
     s_cnt2 = 0;
-    s_wcnt2 = 0;
+    s_nwt = 0;
   }
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(a.queue, 1);
@@ -841,18 +848,26 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += CHUNK) {
         ++n_pass;
-        // A. scan and stage
-        if (tid == 0) s_wcnt2 = 0;
+        // A. scan and stage. The node's static data (type, fanins, fanout
+        // range) is loaded together with its pending flag, so a pending node
+        // costs two dependent loads: these, then its fanins' records.
         const int u = c0 + tid;
-        const bool p = u < hi && pend[u] == epoch;
-        uint64_t act = 0;
-        uint32_t inf = 0;
+        const bool in = u < hi;
+        uint32_t pf = 0, inf = 0;
         int4 f = make_int4(0, 0, 0, 0);
+        int fo0 = 0, fo1 = 0;
+        if (in) {
+          pf = pend[u];
+          inf = J.info[u];
+          f = J.fin[u];
+          fo0 = J.fo_off[u];
+          fo1 = J.fo_off[u + 1];
+        }
+        const bool p = in && pf == epoch;
+        uint64_t act = 0;
         uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
         uint32_t fb[4] = {0, 0, 0, 0};
         if (p) {
-          inf = J.info[u];
-          f = J.fin[u];
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
           if (type != G_PI) {
             const int fi[4] = {f.x, f.y, f.z, f.w};
@@ -886,6 +901,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           s_act[k] = act;
           s_nm[k] = 0;
           s_pref[k] = int(uint32_t(slot));
+          s_fo0[k] = fo0;
+          s_deg[k] = fo1 - fo0;
         }
         __syncthreads();
         const uint64_t c2 = s_cnt2;
@@ -898,77 +915,144 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           aborted = true;
           break;
         }
-        // B. (node, active lane) pairs, kPairsPerThread per thread at a time
-        // with their loads interleaved; a pair that differs writes its pool word.
+        if (a.dense_lanes > 0 && int64_t(P) >= int64_t(a.dense_lanes) * n) {
+          // B'. Dense pass: one node per wave at a time, two in flight, the
+          // wave's lanes being the tile's lanes. A lane's pool word within a
+          // node is the count of the node's active lanes below it (mbcnt), so
+          // no pair has to be located by search.
+          const int wave = tid >> 6;
+          const int64_t w_l = s_lword[lane];
+          const int site_l = s_lsite[lane];
+          const uint32_t kind_l = uint32_t(s_lkind[lane]);
+          const uint64_t mask_l = s_lmask[lane];
 #pragma unroll 1
-        for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
-          int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread], rk[kPairsPerThread];
-          uint32_t type[kPairsPerThread], nin[kPairsPerThread];
-          int4 fv[kPairsPerThread];
-          int64_t w[kPairsPerThread];
-          uint64_t gu[kPairsPerThread], r[kPairsPerThread];
-          bool ok[kPairsPerThread];
+          for (int k0 = wave; k0 < n; k0 += 2 * (CHUNK / 64)) {
+            int kk[2], v[2];
+            bool has[2], on[2];
+            uint32_t type[2], nin[2];
+            uint64_t act[2], gu[2], r[2];
 #pragma unroll
-          for (int i = 0; i < kPairsPerThread; ++i) {
-            const int qq = q0 + i * CHUNK;
-            ok[i] = qq < P;
-            const int qe = ok[i] ? qq : q0;
-            k[i] = find_range(s_pref, n, qe);
-            rk[i] = qe - s_pref[k[i]];
-            ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
-            v[i] = s_u[k[i]];
-            const uint32_t vi = s_info[k[i]];
-            type[i] = vi & 15u;
-            nin[i] = (vi >> 4) & 7u;
-            fv[i] = s_fin[k[i]];
-            w[i] = s_lword[ln[i]];
-            gu[i] = ok[i] ? good[int64_t(v[i]) * RW + w[i]] : 0;
+            for (int i = 0; i < 2; ++i) {
+              const int ki = k0 + i * (CHUNK / 64);
+              has[i] = ki < n;
+              kk[i] = has[i] ? ki : k0;
+              v[i] = __builtin_amdgcn_readfirstlane(s_u[kk[i]]);
+              const uint32_t vi = __builtin_amdgcn_readfirstlane(s_info[kk[i]]);
+              type[i] = vi & 15u;
+              nin[i] = (vi >> 4) & 7u;
+              act[i] = s_act[kk[i]];
+              on[i] = has[i] && ((act[i] >> lane) & 1);
+              gu[i] = on[i] ? good[int64_t(v[i]) * RW + w_l] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+              r[i] = gu[i];
+              if (on[i] && type[i] != G_PI) {
+                const int4 fv = s_fin[kk[i]];
+                const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
+                uint64_t vals[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                  if (j < int(nin[i])) {
+                    const uint64_t fa = s_fa[kk[i]][j];
+                    vals[j] = ((s_m[kk[i]][j] >> lane) & 1)
+                                  ? pool[s_fb[kk[i]][j] + lanes_below(fa)]
+                                  : good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
+                  }
+                r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+              if (!has[i]) continue;
+              if (on[i] && site_l == v[i])
+                r[i] = kind_l == K_FLIP ? (gu[i] ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+              const uint64_t d = on[i] ? (r[i] ^ gu[i]) : 0;
+              const uint64_t nm = __ballot(d != 0);
+              if (on[i]) {
+                bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
+                if (kStats) ++n_pairs;
+              }
+              if (d) {
+                pool[pb + uint32_t(s_pref[kk[i]]) + lanes_below(act[i])] = r[i];
+                if (type[i] == G_PO) atomicOr(&s_acc[lane], (unsigned long long)d);
+                bytes += 8;
+              }
+              if (lane == 0) s_nm[kk[i]] = nm;
+            }
           }
-#pragma unroll
-          for (int i = 0; i < kPairsPerThread; ++i) {
-            r[i] = gu[i];
-            if (ok[i] && type[i] != G_PI) {
+        } else {
+          // B. (node, active lane) pairs, kPairsPerThread per thread at a time
+          // with their loads interleaved; a pair that differs writes its pool word.
+  #pragma unroll 1
+          for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
+            int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread], rk[kPairsPerThread];
+            uint32_t type[kPairsPerThread], nin[kPairsPerThread];
+            int4 fv[kPairsPerThread];
+            int64_t w[kPairsPerThread];
+            uint64_t gu[kPairsPerThread], r[kPairsPerThread];
+            bool ok[kPairsPerThread];
+  #pragma unroll
+            for (int i = 0; i < kPairsPerThread; ++i) {
+              const int qq = q0 + i * CHUNK;
+              ok[i] = qq < P;
+              const int qe = ok[i] ? qq : q0;
+              k[i] = find_range(s_pref, n, qe);
+              rk[i] = qe - s_pref[k[i]];
+              ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
+              v[i] = s_u[k[i]];
+              const uint32_t vi = s_info[k[i]];
+              type[i] = vi & 15u;
+              nin[i] = (vi >> 4) & 7u;
+              fv[i] = s_fin[k[i]];
+              w[i] = s_lword[ln[i]];
+              gu[i] = ok[i] ? good[int64_t(v[i]) * RW + w[i]] : 0;
+            }
+  #pragma unroll
+            for (int i = 0; i < kPairsPerThread; ++i) {
+              r[i] = gu[i];
+              if (ok[i] && type[i] != G_PI) {
+                const int b = ln[i];
+                const uint64_t below = (uint64_t(1) << b) - 1;
+                const int fi[4] = {fv[i].x, fv[i].y, fv[i].z, fv[i].w};
+                uint64_t vals[4] = {0, 0, 0, 0};
+  #pragma unroll
+                for (int j = 0; j < 4; ++j)
+                  if (j < int(nin[i]))
+                    vals[j] = ((s_m[k[i]][j] >> b) & 1)
+                                  ? pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))]
+                                  : good[int64_t(fi[j]) * RW + w[i]];
+                r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
+              }
+            }
+  #pragma unroll
+            for (int i = 0; i < kPairsPerThread; ++i) {
+              if (!ok[i]) continue;
               const int b = ln[i];
-              const uint64_t below = (uint64_t(1) << b) - 1;
-              const int fi[4] = {fv[i].x, fv[i].y, fv[i].z, fv[i].w};
-              uint64_t vals[4] = {0, 0, 0, 0};
-#pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if (j < int(nin[i]))
-                  vals[j] = ((s_m[k[i]][j] >> b) & 1)
-                                ? pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))]
-                                : good[int64_t(fi[j]) * RW + w[i]];
-              r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
+              if (s_lsite[b] == v[i]) {
+                const uint32_t kind = uint32_t(s_lkind[b]);
+                r[i] = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+              }
+              const uint64_t d = r[i] ^ gu[i];
+              bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
+              if (d) {
+                pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
+                atomicOr(&s_nm[k[i]], 1ull << b);
+                if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
+                bytes += 8;
+              }
+              if (kStats) ++n_pairs;
             }
-          }
-#pragma unroll
-          for (int i = 0; i < kPairsPerThread; ++i) {
-            if (!ok[i]) continue;
-            const int b = ln[i];
-            if (s_lsite[b] == v[i]) {
-              const uint32_t kind = uint32_t(s_lkind[b]);
-              r[i] = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
-            }
-            const uint64_t d = r[i] ^ gu[i];
-            bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
-            if (d) {
-              pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
-              atomicOr(&s_nm[k[i]], 1ull << b);
-              if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
-              bytes += 8;
-            }
-            if (kStats) ++n_pairs;
           }
         }
         __syncthreads();
-        // C. nodes that differ somewhere: record, deepest fanout level, fanout pairs
+        // C. nodes that differ somewhere write their record; each wave then
+        // marks the live fanouts of its own written nodes pending.
         if (tid == 0) s_cnt2 = 0;
         int e0 = 0, deg = 0;
-        bool wr = false;
         if (tid < n) {
           const uint64_t nm = s_nm[tid];
           if (nm) {
-            wr = true;
             const int v = s_u[tid];
             PairRec r;
             r.nm = nm;
@@ -977,35 +1061,48 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             r.base = pb + uint32_t(s_pref[tid]);
             r.pad = 0;
             rec[v] = r;
-            e0 = J.fo_off[v];
-            deg = J.fo_off[v + 1] - e0;
-            if (deg) atomicMax(&s_maxl, J.fo[e0 + deg - 1].y);
+            e0 = s_fo0[tid];
+            deg = s_deg[tid];
+            if (deg) atomicMax(&s_maxl, int(s_info[tid] >> 8));
             bytes += 40;
+            if (kStats) ++n_written;
           }
         }
-        const uint64_t wslot = wave_claim(wr ? ((uint64_t(1) << 32) | uint64_t(deg)) : 0, lane, &s_wcnt2);
-        if (wr) {
-          const int k = int(wslot >> 32);
-          s_wfo0[k] = e0;
-          s_epref[k] = int(uint32_t(wslot));
+        int incl = deg;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o);
+          if (lane >= o) incl += y;
         }
-        __syncthreads();
-        const uint64_t w2 = s_wcnt2;
-        const int nw = int(w2 >> 32), E = int(uint32_t(w2));
-        // D. mark fanouts pending; the tile's pool grows by this pass's pairs
+        const int excl = incl - deg;
+        const int Ew = __shfl(incl, 63);
+        // D. kFanoutsPerLane fanouts per lane at a time, loads interleaved.
+        // The fanout e belongs to the last lane whose prefix is <= e.
         if (tid == 0) s_pool = pb + uint32_t(P);
 #pragma unroll 1
-        for (int e = tid; e < E; e += CHUNK) {
-          const int k = find_range(s_epref, nw, e);
-          const int2 fo = J.fo[s_wfo0[k] + (e - s_epref[k])];
-          pend[fo.x] = epoch;
-          s_lvl[fo.y] = 1;
-          bytes += 12;
+        for (int eb = 0; eb < Ew; eb += kFanoutsPerLane * 64) {
+          int2 fo[kFanoutsPerLane];
+          bool ok[kFanoutsPerLane];
+#pragma unroll
+          for (int i = 0; i < kFanoutsPerLane; ++i) {
+            const int e = eb + i * 64 + lane;
+            int k = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
+              if (__shfl(excl, k + step) <= e) k += step;
+            const int src0 = __shfl(e0, k), srcx = __shfl(excl, k);
+            ok[i] = e < Ew;
+            fo[i] = ok[i] ? J.fo[src0 + (e - srcx)] : make_int2(0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < kFanoutsPerLane; ++i)
+            if (ok[i]) {
+              pend[fo[i].x] = epoch;
+              s_lvl[fo[i].y] = 1;
+              bytes += 12;
+            }
         }
-        if (kStats) {
-          n_staged += n;
-          n_written += nw;
-        }
+        if (kStats) n_staged += n;
         __syncthreads();
       }
       if (tid == 0 && !aborted) s_lvl[l] = 0;
@@ -1015,7 +1112,6 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       for (int i = tid; i < a.lvl_entries; i += CHUNK) s_lvl[i] = 0;
       if (tid == 0) {
         s_cnt2 = 0;
-        s_wcnt2 = 0;
         a.ovf_list[atomicAdd(a.ovf_count, 1)] = t;
       }
     } else if (tid < T.nlanes) {
@@ -1025,17 +1121,25 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     for (int o = 32; o > 0; o >>= 1) wbytes += __shfl_xor(wbytes, o);
     if (lane == 0 && wbytes) atomicAdd(a.bytes, wbytes);
     if (kStats) {
-      unsigned long long np = n_pairs;
-      for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o);
-      if (lane == 0) atomicAdd(&a.tstats[7], np);
+      unsigned long long np = n_pairs, nw = n_written;
+      for (int o = 32; o > 0; o >>= 1) {
+        np += __shfl_xor(np, o);
+        nw += __shfl_xor(nw, o);
+      }
+      if (lane == 0) {
+        atomicAdd(&a.tstats[7], np);
+        atomicAdd(&s_nwt, nw);
+      }
+      __syncthreads();
       if (tid == 0) {
         atomicAdd(&a.tstats[0], 1ull);
         atomicAdd(&a.tstats[1], (unsigned long long)n_lv);
         atomicAdd(&a.tstats[2], (unsigned long long)n_pass);
         atomicAdd(&a.tstats[3], (unsigned long long)n_staged);
-        atomicAdd(&a.tstats[4], (unsigned long long)n_written);
-        atomicMax(&a.tstats[5], (unsigned long long)n_written);
+        atomicAdd(&a.tstats[4], s_nwt);
+        atomicMax(&a.tstats[5], s_nwt);
         atomicAdd(&a.tstats[6], (unsigned long long)(l_end - T.start_level + 1));
+        s_nwt = 0;
       }
     }
     __syncthreads();
@@ -1603,6 +1707,11 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
     for (int32_t v : outs) fo.push_back(make_int2(v, c.level_dp[v]));
     fo_off[u + 1] = int32_t(fo.size());
   }
+  // info bits 8..31: level of the node's deepest live fanout (the last one,
+  // as positions ascend with level), which bounds how far a sweep must walk.
+  if (c.L >= (1 << 24)) throw DeviceError(LSIM_E_INPUT, "circuit too deep (levels must fit in 24 bits)");
+  for (int64_t u = 0; u < c.NL; ++u)
+    if (fo_off[u + 1] > fo_off[u]) info[u] |= uint32_t(fo[fo_off[u + 1] - 1].y) << 8;
   if (fo.empty()) fo.push_back(make_int2(0, 0));
   c.head_is_po.assign(nl1, 0);
   for (int64_t p = 0; p < N; ++p)
@@ -2332,6 +2441,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   a.ovf_count = d_ovf_count.p;
   a.ovf_list = d_ovf_list.p;
   a.tstats = tstats;
+  a.dense_lanes = eo.pair_dense;
   // A share of the device's resident workgroups (another engine on the same
   // device may hold the rest).
   const double share = std::min(1.0, std::max(0.0, eo.slot_share));

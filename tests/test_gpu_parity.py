@@ -14,10 +14,15 @@ from lsim_amd import workloads
 pytestmark = pytest.mark.gpu
 
 # Engines: the per-level form, and the persistent event form with each of its
-# two sweep kernels (flattened (node, lane) pairs; one node per wave).
+# two sweep kernels (flattened (node, lane) pairs; one node per wave). The pair
+# sweep evaluates dense level passes one node per wave: by default where the
+# nodes average 32 active lanes, and forced never (pairs_flat) or always
+# (pairs_dense).
 MODES = [(lsim_amd.Simulator.Mode.PLAIN, {}), (lsim_amd.Simulator.Mode.PARTITIONED, {}),
-         (lsim_amd.Simulator.Mode.PARTITIONED, {"sweep_kernel": 1})]
-MODE_IDS = ["plain", "pairs", "nodewave"]
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"sweep_kernel": 1}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 0}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 1})]
+MODE_IDS = ["plain", "pairs", "nodewave", "pairs_flat", "pairs_dense"]
 
 
 def _sim(ckt, flst, ptn, mode=None, **opts):
