@@ -28,17 +28,26 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(config: str, kernel: str):
-    """HBM bytes per launch of `kernel` on `config` from the committed rocprofv3
-    summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), or None."""
+def profiled_value(config: str, kernel: str, key: str = "traffic_bytes_per_launch"):
+    """A per-launch figure of `kernel` on `config` from its committed rocprofv3
+    summary: `traffic_bytes_per_launch` (HBM bytes, FETCH_SIZE and WRITE_SIZE
+    passes, copy-calibrated) or `transactions_per_launch` (TCC_EA0_RDREQ +
+    TCC_EA0_WRREQ requests); None when absent."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{config}.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def probe_random(nbytes: int = 16 << 30) -> dict:
+    """Random 8-byte reads per second the device serves (lsim_probe_random_reads)."""
+    import lsim_amd
+
+    return lsim_amd.probe_random_reads(0, nbytes)
 
 
 def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: float) -> dict:
@@ -107,7 +116,7 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
     t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
     bytes_launch = agg["sweep_bytes"] / launches
     achieved = bytes_launch / t_launch_s / 1e9 if t_launch_s > 0 else 0.0
-    traffic = profiled_traffic(config, kernel)
+    traffic = profiled_value(config, kernel)
     out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
            "bytes_per_launch": bytes_launch, "avg_launch_ms": t_launch_s * 1000.0}
@@ -115,6 +124,23 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
         # rocprof-measured HBM bytes per launch over this run's launch time
         out["traffic_gbs"] = traffic / t_launch_s / 1e9
         out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+    # The sweeps' accesses are small records at random node positions: their
+    # memory requests per second against the rate the device serves random
+    # 8-byte reads (measured live on this device).
+    tx = profiled_value(config, kernel, "transactions_per_launch")
+    try:
+        pr = probe_random()
+    except Exception as e:  # reported, never fatal for the GPU figure
+        pr = {"error": str(e)}
+    ra = {"probe_reads_per_s": pr.get("reads_per_s"), "probe_latency_ns": pr.get("latency_ns"),
+          "transactions_per_launch": tx}
+    if "error" in pr:
+        ra["probe_error"] = pr["error"]
+    if tx and t_launch_s > 0:
+        ra["transactions_per_s"] = tx / t_launch_s
+        if pr.get("reads_per_s"):
+            ra["frac"] = ra["transactions_per_s"] / pr["reads_per_s"]
+    out["random_access"] = ra
     return out
 
 
@@ -339,9 +365,14 @@ def main() -> int:
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe-random", type=int, default=0, metavar="MIB",
+                    help="only run the random-read probe over a MIB MiB buffer and print its figures")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.probe_random:
+        print(json.dumps(probe_random(args.probe_random << 20)), flush=True)
+        return 0
     if args.calibrate_copy:
         import torch
 

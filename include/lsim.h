@@ -264,6 +264,12 @@ LSIM_API int lsim_generate_faults(uint64_t seed, int64_t num_pis, int64_t num_po
 LSIM_API int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn,
                                     size_t *ptn_len);
 
+/* Measurement (no reference counterpart): independent random 8-byte reads
+ * served per second over a `bytes`-sized device buffer, and the latency of one
+ * read in a dependent chain. The fault sweeps' memory traffic is random small
+ * records, so bench.py sets their traffic against this rate. */
+LSIM_API int lsim_probe_random_reads(int device, int64_t bytes, double *reads_per_s, double *latency_ns);
+
 #ifdef __cplusplus
 }
 #endif

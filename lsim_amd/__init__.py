@@ -164,6 +164,8 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_generate_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
                                                   ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
+        "lsim_probe_random_reads": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                                   ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -403,6 +405,16 @@ def generate_patterns(seed: int, num_pis: int, num_patterns: int) -> bytes:
     b, n = ctypes.c_void_p(), ctypes.c_size_t()
     rc = lib().lsim_generate_patterns(seed, num_pis, num_patterns, ctypes.byref(b), ctypes.byref(n))
     return _take_text(rc, b, n)
+
+
+def probe_random_reads(device: int = 0, nbytes: int = 16 << 30) -> dict:
+    """Independent random 8-byte reads the device serves per second over an
+    `nbytes` buffer, and the latency of one read in a dependent chain (ns)."""
+    rate, lat = ctypes.c_double(), ctypes.c_double()
+    rc = lib().lsim_probe_random_reads(device, int(nbytes), ctypes.byref(rate), ctypes.byref(lat))
+    if rc != LSIM_OK:
+        raise LsimError(rc, lib().lsim_last_error(None).decode())
+    return {"reads_per_s": rate.value, "latency_ns": lat.value, "buffer_bytes": int(nbytes)}
 
 
 class Batch:

@@ -16,6 +16,7 @@
 #include "engine.hpp"
 #include "generator.hpp"
 #include "loops.hpp"
+#include "probe.hpp"
 #include "lsim.h"
 #include "netlist.hpp"
 
@@ -1060,6 +1061,19 @@ int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns,
   std::string text;
   int rc = guarded(&g_create_error, [&] { text = lsim::generate_ptn(seed, num_pis, num_patterns); });
   return rc ? rc : emit_text(text, ptn, ptn_len);
+}
+
+int lsim_probe_random_reads(int device, int64_t bytes, double *reads_per_s, double *latency_ns) {
+  if (!reads_per_s || !latency_ns) return LSIM_E_ARG;
+  try {
+    const lsim::ProbeResult r = lsim::probe_random_reads(device, bytes);
+    *reads_per_s = r.reads_per_s;
+    *latency_ns = r.latency_ns;
+    return LSIM_OK;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    return LSIM_E_DEVICE;
+  }
 }
 
 }  // extern "C"

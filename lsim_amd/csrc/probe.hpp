@@ -1,0 +1,22 @@
+// Random-access probe of the device's memory system.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+
+namespace lsim {
+
+struct ProbeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct ProbeResult {
+  double reads_per_s = 0;  // independent random 8-byte reads served per second
+  double latency_ns = 0;   // one read of a dependent chain
+  double reads = 0;        // reads timed
+  double ms = 0;           // their time
+};
+
+// Allocates `bytes` of device memory for the duration of the probe.
+ProbeResult probe_random_reads(int device, int64_t bytes);
+
+}  // namespace lsim
