@@ -1290,8 +1290,9 @@ struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
   hipStream_t s = nullptr;  // stream of a stream-ordered allocation, null for a plain one
+  bool own = true;          // false: a view into memory another buffer owns
   void alloc(size_t count) {
-    if (count <= n && p) return;
+    if (count <= n && p && own) return;
     release();
     if (count == 0) count = 1;
     if (tl_alloc_stream) {
@@ -1305,7 +1306,7 @@ struct DevBuf {
   }
   // Plain allocation, for the large sweep scratch that lives as long as the engine.
   void alloc_direct(size_t count) {
-    if (count <= n && p) return;
+    if (count <= n && p && own) return;
     release();
     if (count == 0) count = 1;
     LSIM_HIP(hipMalloc(&p, count * sizeof(T)));
@@ -1313,13 +1314,21 @@ struct DevBuf {
     n = count;
   }
   void release() {
-    if (p) {
+    if (p && own) {
       if (s) (void)hipFreeAsync(p, s);
       else (void)hipFree(p);
     }
     p = nullptr;
     n = 0;
     s = nullptr;
+    own = true;
+  }
+  // Points at `count` elements of memory owned elsewhere (never freed here).
+  void view(T *ptr, size_t count) {
+    release();
+    p = ptr;
+    n = count;
+    own = false;
   }
   size_t bytes() const { return n * sizeof(T); }
   DevBuf() = default;
@@ -1454,6 +1463,7 @@ struct JobState {
   std::vector<int32_t> site, head;  // per fault: device position, region head (-1: dead site)
   std::vector<uint8_t> sa;
   std::vector<int32_t> live, alive;
+  DevBuf<uint8_t> arena;  // one allocation holding every per-job device array below
   DevBuf<uint64_t> good, pi_words, detbits;
   DevBuf<int32_t> d_site, d_head, first, head_slot;
   DevBuf<uint8_t> d_sa, last;
@@ -1481,11 +1491,70 @@ struct StreamHolder {
   }
 };
 
+// Device memory of released jobs, kept for the engine's later jobs (up to
+// kCap bytes): a stream-ordered free or allocation costs a fraction of a
+// millisecond, a reuse nothing. Blocks are used on the engine's one stream
+// only, so a reuse is ordered after the released job's work.
+struct ArenaCache {
+  struct Block {
+    uint8_t *p;
+    size_t n;
+    hipStream_t s;
+  };
+  static constexpr size_t kCap = size_t(16) << 30;
+  std::vector<Block> blocks;
+  size_t bytes = 0;
+  // The smallest block of at least `want` bytes and at most twice that.
+  bool take(size_t want, DevBuf<uint8_t> &out) {
+    int best = -1;
+    for (int i = 0; i < int(blocks.size()); ++i)
+      if (blocks[i].n >= want && blocks[i].n <= 2 * want && (best < 0 || blocks[i].n < blocks[size_t(best)].n))
+        best = i;
+    if (best < 0) return false;
+    const Block b = blocks[size_t(best)];
+    blocks.erase(blocks.begin() + best);
+    bytes -= b.n;
+    out.release();
+    out.p = b.p;
+    out.n = b.n;
+    out.s = b.s;
+    out.own = true;
+    return true;
+  }
+  // Takes the buffer's memory (a stream-ordered allocation) into the cache.
+  void put(DevBuf<uint8_t> &b) {
+    if (!b.p || !b.own || !b.s || b.n > kCap) {
+      b.release();
+      return;
+    }
+    blocks.push_back({b.p, b.n, b.s});
+    bytes += b.n;
+    b.p = nullptr;
+    b.n = 0;
+    b.s = nullptr;
+    while (bytes > kCap) {
+      (void)hipFreeAsync(blocks.front().p, blocks.front().s);
+      bytes -= blocks.front().n;
+      blocks.erase(blocks.begin());
+    }
+  }
+  void clear() {
+    for (const Block &b : blocks) (void)hipFreeAsync(b.p, b.s);
+    blocks.clear();
+    bytes = 0;
+  }
+  ArenaCache() = default;
+  ArenaCache(const ArenaCache &) = delete;
+  ArenaCache &operator=(const ArenaCache &) = delete;
+  ~ArenaCache() { clear(); }
+};
+
 struct Engine::Impl {
   StreamHolder holder;
   int device = 0;
   hipStream_t stream = nullptr;
   int cus = 0;
+  ArenaCache arenas;  // destroyed after the jobs, before the stream
   std::vector<std::unique_ptr<CircuitRes>> circuits;
   std::vector<std::unique_ptr<JobState>> jobs;
   // per iteration
@@ -1600,11 +1669,12 @@ Engine::Engine(int device) : impl_(new Impl) {
     impl_->cus = prop.multiProcessorCount;
     LSIM_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
     impl_->holder.s = impl_->stream;
-    // Stream-ordered releases stay in the device's pool up to 8 GiB instead of
-    // going back at every synchronisation.
+    // Stream-ordered releases stay in the device's pool up to 32 GiB instead
+    // of going back at every synchronisation: jobs of successive batches and
+    // serving iterations reuse them.
     hipMemPool_t pool = nullptr;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
-      uint64_t keep = uint64_t(8) << 30;
+      uint64_t keep = uint64_t(32) << 30;
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     LSIM_HIP(hipEventCreate(&impl_->ev[0]));
@@ -1792,15 +1862,45 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   for (int64_t r = 0; r < R; ++r)
     for (int64_t i = 0; i < c->P; ++i) piw[i * R + r] = patterns[r * c->P + i];
   const int64_t F1 = std::max<int64_t>(j.F, 1);
-  j.good.alloc(size_t(std::max<int64_t>(c->N * R, 1)));
-  j.pi_words.alloc(piw.size());
-  j.d_site.alloc(F1);
-  j.d_head.alloc(F1);
-  j.d_sa.alloc(F1);
-  j.first.alloc(F1);
-  j.head_slot.alloc(std::max<int64_t>(c->NL, 1));
-  if (!opt.drop) j.detbits.alloc(size_t(std::max<int64_t>(j.F * j.RB, 1)));
-  if (opt.drop && opt.report_last_round) j.last.alloc(F1);
+  // Every per-job array in one allocation (one release per job).
+  {
+    size_t total = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = total;
+      total += (bytes + 255) & ~size_t(255);
+      return at;
+    };
+    const size_t n_good = size_t(std::max<int64_t>(c->N * R, 1)), n_hs = size_t(std::max<int64_t>(c->NL, 1));
+    const size_t n_det = opt.drop ? 0 : size_t(std::max<int64_t>(j.F * j.RB, 1));
+    const bool with_last = opt.drop && opt.report_last_round;
+    const size_t o_good = take(n_good * 8), o_piw = take(piw.size() * 8), o_site = take(size_t(F1) * 4),
+                 o_head = take(size_t(F1) * 4), o_sa = take(size_t(F1)), o_first = take(size_t(F1) * 4),
+                 o_hs = take(n_hs * 4), o_det = take(n_det * 8), o_last = take(with_last ? size_t(F1) : 0);
+    if (!m.arenas.take(total, j.arena)) {
+      try {
+        j.arena.alloc(total);
+      } catch (const DeviceError &e) {
+        // Out of memory with released jobs' memory cached: free it and retry.
+        if (e.code != LSIM_E_NOMEM || m.arenas.blocks.empty()) throw;
+        (void)hipGetLastError();
+        m.arenas.clear();
+        LSIM_HIP(hipStreamSynchronize(m.stream));
+        hipMemPool_t mp = nullptr;
+        if (hipDeviceGetDefaultMemPool(&mp, m.device) == hipSuccess && mp) (void)hipMemPoolTrimTo(mp, 0);
+        j.arena.alloc(total);
+      }
+    }
+    uint8_t *b = j.arena.p;
+    j.good.view(reinterpret_cast<uint64_t *>(b + o_good), n_good);
+    j.pi_words.view(reinterpret_cast<uint64_t *>(b + o_piw), piw.size());
+    j.d_site.view(reinterpret_cast<int32_t *>(b + o_site), size_t(F1));
+    j.d_head.view(reinterpret_cast<int32_t *>(b + o_head), size_t(F1));
+    j.d_sa.view(b + o_sa, size_t(F1));
+    j.first.view(reinterpret_cast<int32_t *>(b + o_first), size_t(F1));
+    j.head_slot.view(reinterpret_cast<int32_t *>(b + o_hs), n_hs);
+    if (n_det) j.detbits.view(reinterpret_cast<uint64_t *>(b + o_det), n_det);
+    if (with_last) j.last.view(b + o_last, size_t(F1));
+  }
   m.h2d(j.pi_words.p, piw.data(), piw.size() * sizeof(uint64_t));
   m.h2d(j.d_site.p, j.site.data(), j.F * sizeof(int32_t));
   m.h2d(j.d_head.p, j.head.data(), j.F * sizeof(int32_t));
@@ -1823,10 +1923,13 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
 void Engine::release_job(JobState *j) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
+  // The job's device memory is one stream-ordered allocation: it goes to the
+  // engine's cache, where only work queued later on the same stream reuses
+  // it, so no wait.
   AllocStream alloc_on(m.stream);
-  LSIM_HIP(hipStreamSynchronize(m.stream));
   for (auto it = m.jobs.begin(); it != m.jobs.end(); ++it)
     if (it->get() == j) {
+      m.arenas.put(j->arena);
       m.jobs.erase(it);
       return;
     }
@@ -2225,6 +2328,14 @@ void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, 
                                d_prec.bytes() + d_ppend.bytes() + d_pool.bytes());
   release_slots();
   release_pair_scratch();
+  // Cached job memory and what the stream-ordered pool keeps count as free
+  // for the scratch budget.
+  arenas.clear();
+  LSIM_HIP(hipStreamSynchronize(stream));
+  {
+    hipMemPool_t mp = nullptr;
+    if (hipDeviceGetDefaultMemPool(&mp, device) == hipSuccess && mp) (void)hipMemPoolTrimTo(mp, 0);
+  }
   const int64_t budget = scratch_budget(eo, held);
   (void)full_slots;
   // Per slot: a record and a pending flag per live node. The pool must hold
@@ -2608,27 +2719,56 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   st.t_detect_ms += time_end();
   const auto h2 = clk::now();
   // Lanes: one per (unit, word) with needed bits, per job in unit order
-  // (units are level-sorted), counted then filled in parallel.
-  std::vector<int64_t> lane_off(nj + 1, 0);
-  parallel_for(order, [&](int i) {
+  // (units are level-sorted), counted then filled in parallel over chunks of
+  // a job's units.
+  constexpr int64_t kUnitChunk = 4096;
+  struct LaneChunk {
+    int job;
+    int64_t k0, k1, off;
+  };
+  std::vector<LaneChunk> chunks;
+  for (int i = 0; i < nj; ++i) {
+    const int64_t nu = int64_t(act[i]->units.size());
+    for (int64_t k0 = 0; k0 < nu; k0 += kUnitChunk) chunks.push_back({i, k0, std::min(nu, k0 + kUnitChunk), 0});
+  }
+  std::vector<int> chunk_ids(chunks.size());
+  std::iota(chunk_ids.begin(), chunk_ids.end(), 0);
+  parallel_for(chunk_ids, [&](int ci) {
+    LaneChunk &ch = chunks[size_t(ci)];
+    const int64_t Wn = act[ch.job]->step_Wn;
     int64_t n = 0;
-    for (int64_t o = word0[i]; o < word0[i + 1]; ++o) n += need[o] != 0;
-    lane_off[i + 1] = n;
+    for (int64_t o = word0[ch.job] + ch.k0 * Wn, e = word0[ch.job] + ch.k1 * Wn; o < e; ++o) n += need[o] != 0;
+    ch.off = n;
   });
-  for (int i = 0; i < nj; ++i) lane_off[i + 1] += lane_off[i];
+  std::vector<int64_t> lane_off(nj + 1, 0);
+  {
+    int64_t run = 0;
+    size_t c = 0;
+    for (int i = 0; i < nj; ++i) {
+      lane_off[i] = run;
+      for (; c < chunks.size() && chunks[c].job == i; ++c) {
+        const int64_t n = chunks[c].off;
+        chunks[c].off = run;
+        run += n;
+      }
+    }
+    lane_off[nj] = run;
+  }
   const int64_t nlanes = lane_off[nj];
   h_lanes.alloc(size_t(std::max<int64_t>(nlanes, 1)));
   Lane *lanes = h_lanes.p;
-  parallel_for(order, [&](int i) {
+  parallel_for(chunk_ids, [&](int ci) {
+    const LaneChunk &ch = chunks[size_t(ci)];
+    const int i = ch.job;
     const JobState &j = *act[i];
     const int Wn = j.step_Wn;
-    Lane *out = lanes + lane_off[i];
-    for (size_t k = 0; k < j.units.size(); ++k) {
-      const int32_t u = j.units[k];
-      const int32_t kind = j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP;
+    Lane *out = lanes + ch.off;
+    for (int64_t k = ch.k0; k < ch.k1; ++k) {
+      const int32_t u = j.units[size_t(k)];
+      const int32_t kind = j.direct ? (j.sa[j.step_faults[size_t(k)]] ? K_SA1 : K_SA0) : K_FLIP;
       const int32_t level = j.c->level_dp[u];
       for (int jw = 0; jw < Wn; ++jw) {
-        const int64_t o = word0[i] + int64_t(k) * Wn + jw;
+        const int64_t o = word0[i] + k * Wn + jw;
         const uint64_t msk = need[o];
         if (!msk) continue;
         Lane ln;
