@@ -1640,7 +1640,9 @@ struct Engine::Impl {
   std::string tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo, bool &any_drop) const;
   TuneState &tune_state(const std::string &key, const EngineOptions &eo, bool any_drop);
   bool plan(JobState &j);
-  void finalize(JobState &j);
+  std::vector<JobState *> to_finalize;  // jobs plan() found finished
+  HostBuf<uint8_t> h_final;
+  void finalize_jobs(const std::vector<JobState *> &js);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
   void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
              int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st);
@@ -2227,16 +2229,17 @@ void Engine::Impl::finish_start(const std::vector<JobState *> &js, double ms) {
   }
 }
 
-// Decides the job's next step; false (and the job finalised) when none is left.
+// Decides the job's next step; false (and the job queued for finalize_jobs)
+// when none is left.
 bool Engine::Impl::plan(JobState &j) {
   if (j.done) return false;
   if (j.R == 0 || j.live.empty()) {
-    finalize(j);
+    to_finalize.push_back(&j);
     return false;
   }
   if (!j.opt.drop) {
     if (j.w0 >= j.R) {
-      finalize(j);
+      to_finalize.push_back(&j);
       return false;
     }
     j.kind = JobState::MATRIX;
@@ -2261,37 +2264,69 @@ bool Engine::Impl::plan(JobState &j) {
     j.step_Wn = 1;
     return true;
   }
-  finalize(j);
+  to_finalize.push_back(&j);
   return false;
 }
 
-void Engine::Impl::finalize(JobState &j) {
-  RunResults &res = j.res;
-  const int64_t F = j.F, R = j.R;
-  res.first_round.assign(F, INT32_MAX);
-  if (F) d2h(res.first_round.data(), j.first.p, F * sizeof(int32_t));
-  res.RB = j.RB;
-  if (!j.opt.drop) {
-    res.det_bits.assign(size_t(F * j.RB), 0);
-    if (F && j.detbits.p) d2h(res.det_bits.data(), j.detbits.p, size_t(F * j.RB) * sizeof(uint64_t));
+// Results of finished jobs: every job's device arrays copied into one
+// page-locked block, one wait, then unpacked.
+void Engine::Impl::finalize_jobs(const std::vector<JobState *> &js) {
+  if (js.empty()) return;
+  std::vector<size_t> at(js.size() + 1, 0);
+  for (size_t i = 0; i < js.size(); ++i) {
+    const JobState &j = *js[i];
+    size_t b = size_t(j.F) * sizeof(int32_t);
+    if (!j.opt.drop && j.detbits.p) b += size_t(j.F * j.RB) * sizeof(uint64_t);
+    if (j.opt.drop && j.last.p && j.last_done) b += size_t(j.F);
+    at[i + 1] = at[i] + ((b + 63) & ~size_t(63));
   }
-  res.last_round.assign(F, 0);
-  if (j.opt.drop && j.last.p && j.last_done && F) d2h(res.last_round.data(), j.last.p, F);
+  h_final.alloc(std::max<size_t>(at.back(), 1));
+  for (size_t i = 0; i < js.size(); ++i) {
+    const JobState &j = *js[i];
+    uint8_t *o = h_final.p + at[i];
+    const size_t nf = size_t(j.F) * sizeof(int32_t);
+    if (j.F) d2h(o, j.first.p, nf);
+    o += nf;
+    if (!j.opt.drop && j.detbits.p && j.F) {
+      d2h(o, j.detbits.p, size_t(j.F * j.RB) * sizeof(uint64_t));
+      o += size_t(j.F * j.RB) * sizeof(uint64_t);
+    }
+    if (j.opt.drop && j.last.p && j.last_done && j.F) d2h(o, j.last.p, size_t(j.F));
+  }
   LSIM_HIP(hipStreamSynchronize(stream));
-  if (!j.opt.drop) {
-    res.has_matrix = true;
-    res.has_last = true;
-    if (R > 0)
-      for (int64_t f = 0; f < F; ++f)
-        res.last_round[f] = uint8_t((res.det_bits[f * j.RB + ((R - 1) >> 6)] >> ((R - 1) & 63)) & 1);
-  } else {
-    res.has_matrix = false;
-    res.has_last = j.opt.report_last_round;
+  for (size_t i = 0; i < js.size(); ++i) {
+    JobState &j = *js[i];
+    RunResults &res = j.res;
+    const int64_t F = j.F, R = j.R;
+    const uint8_t *o = h_final.p + at[i];
+    res.first_round.assign(size_t(F), INT32_MAX);
+    if (F) std::memcpy(res.first_round.data(), o, size_t(F) * sizeof(int32_t));
+    o += size_t(F) * sizeof(int32_t);
+    res.RB = j.RB;
+    if (!j.opt.drop) {
+      res.det_bits.assign(size_t(F * j.RB), 0);
+      if (F && j.detbits.p) {
+        std::memcpy(res.det_bits.data(), o, size_t(F * j.RB) * sizeof(uint64_t));
+        o += size_t(F * j.RB) * sizeof(uint64_t);
+      }
+    }
+    res.last_round.assign(size_t(F), 0);
+    if (j.opt.drop && j.last.p && j.last_done && F) std::memcpy(res.last_round.data(), o, size_t(F));
+    if (!j.opt.drop) {
+      res.has_matrix = true;
+      res.has_last = true;
+      if (R > 0)
+        for (int64_t f = 0; f < F; ++f)
+          res.last_round[size_t(f)] = uint8_t((res.det_bits[size_t(f * j.RB + ((R - 1) >> 6))] >> ((R - 1) & 63)) & 1);
+    } else {
+      res.has_matrix = false;
+      res.has_last = j.opt.report_last_round;
+    }
+    for (auto &v : res.first_round)
+      if (v == INT32_MAX) v = -1;
+    res.stats.slots = slots;
+    j.done = true;
   }
-  for (auto &v : res.first_round)
-    if (v == INT32_MAX) v = -1;
-  res.stats.slots = slots;
-  j.done = true;
 }
 
 int64_t Engine::Impl::scratch_budget(const EngineOptions &eo, int64_t held) const {
@@ -2842,8 +2877,9 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   st.t_detect_ms += time_end();
   st.sweep_bytes = int64_t(h_bytes.p[0]);
   const auto h5 = clk::now();
-  // Advance every job.
-  for (int i = 0; i < nj; ++i) {
+  // Advance every job (in parallel: a dropping step rebuilds each job's list
+  // of faults still undetected).
+  parallel_for(order, [&](int i) {
     JobState &j = *act[i];
     lsim_run_stats &js = j.res.stats;
     js.steps++;
@@ -2869,7 +2905,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       }
       case JobState::LAST: j.last_done = true; break;
     }
-  }
+  });
   if (trace) {
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::fprintf(stderr,
@@ -2882,8 +2918,11 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
 
 int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   std::vector<JobState *> act;
+  to_finalize.clear();
   for (JobState *j : jobs)
     if (j->started && !j->done && plan(*j)) act.push_back(j);
+  finalize_jobs(to_finalize);
+  to_finalize.clear();
   if (!act.empty()) {
     if (eo.mode == LSIM_MODE_PLAIN) {
       for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
