@@ -33,6 +33,7 @@
 //        differing fanin, staged per level in LDS, ending at the deepest
 //        pending level.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -101,7 +102,7 @@ struct MaskRec {
 // Everything a kernel needs about one job and its circuit in this iteration.
 struct JobDev {
   // circuit (device positions: live nodes first, then dead nodes)
-  const uint32_t *info;      // type | fanin count << 4
+  const uint32_t *info;      // type | fanin count << 4 | deepest live fanout level << 8
   const int4 *fin;           // fanin positions (PI: PI index in .x)
   const int32_t *live_lvl;   // first live position of each level, L+2 entries
   const int32_t *dead_lvl;   // first dead position of each level, L+2 entries
@@ -133,6 +134,7 @@ struct JobDev {
   int direct;
   int64_t item0;             // first detection item of the job
   int64_t word0;             // first need/dout word of the job's units
+  int64_t lane_unit0;        // first entry of the job's units in the step's {position, kind} list
 };
 
 // The fields a sweep reads, copied out of the job table once per tile so
@@ -1265,6 +1267,106 @@ __global__ void k_set_slots(const JobDev *__restrict__ jobs, const int4 *__restr
   if (i < n) jobs[units[i].x].head_slot[units[i].y] = clear ? -1 : units[i].z;
 }
 
+// ---------------------------------------------------------------- lanes and tiles on the device
+//
+// Word o of a step's need array belongs to (job, unit k, word jw); it becomes
+// sweep lane pos[o] (pos = exclusive prefix count of words with needed bits),
+// so lanes keep the (job, unit, word) order, and a job's lanes are cut into
+// tiles of 64.
+
+struct NeedNonzero {
+  __host__ __device__ int operator()(const uint64_t &x) const { return x != 0 ? 1 : 0; }
+};
+
+// Largest job index i with jobs[i].word0 <= o (the job holding word o).
+__device__ __forceinline__ int job_of_word(const JobDev *__restrict__ jobs, int nj, int64_t o) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].word0 <= o) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Level of live position u: the largest l with live_lvl[l] <= u.
+__device__ __forceinline__ int level_of(const int32_t *__restrict__ live_lvl, int L, int u) {
+  int lo = 0, hi = L;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (live_lvl[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// lane_off[i] = first lane of job i (i < nj), lane_off[nj] = all lanes.
+__global__ void k_lane_bounds(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
+                              const int32_t *__restrict__ pos, int64_t words, int64_t *lane_off) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nj) return;
+  if (words == 0) {
+    lane_off[t] = 0;
+    return;
+  }
+  const int64_t o = t < nj ? jobs[t].word0 : words;
+  lane_off[t] = o < words ? int64_t(pos[o]) : int64_t(pos[words - 1]) + (need[words - 1] != 0 ? 1 : 0);
+}
+
+// units[lane_unit0 + k] = {position, injection kind} of unit k of a job.
+__global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
+                            const int32_t *__restrict__ pos, const int2 *__restrict__ units, int64_t words,
+                            Lane *__restrict__ lanes) {
+  const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (o >= words) return;
+  const uint64_t m = need[o];
+  if (!m) return;
+  const JobDev &J = jobs[job_of_word(jobs, nj, o)];
+  const int64_t rel = o - J.word0;
+  const int64_t k = rel / J.Wn;
+  const int jw = int(rel - k * J.Wn);
+  const int2 uk = units[J.lane_unit0 + k];
+  Lane ln;
+  ln.mask = m;
+  ln.site = uk.x;
+  ln.word = int32_t(J.w0 + jw);
+  ln.out = int32_t(o);
+  ln.kind = uint32_t(uk.y);
+  ln.level = level_of(J.live_lvl, J.max_level, uk.x);
+  ln.pad = 0;
+  lanes[pos[o]] = ln;
+}
+
+// Tile t of the step: job by tile_off (largest i with tile_off[i] <= t),
+// lanes [lane_off[i] + 64 (t - tile_off[i]), +64), and its sort key (levels
+// the tile may walk).
+__global__ void k_make_tiles(const JobDev *__restrict__ jobs, int nj, const Lane *__restrict__ lanes,
+                             const int64_t *__restrict__ lane_off, const int64_t *__restrict__ tile_off, int64_t ntiles,
+                             Tile *__restrict__ tiles, int *__restrict__ keys, int *__restrict__ idx) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_off[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const int64_t l0 = lane_off[lo] + 64 * (t - tile_off[lo]);
+  const int n = int(min(int64_t(64), lane_off[lo + 1] - l0));
+  Tile T;
+  T.lane0 = int32_t(l0);
+  T.nlanes = n;
+  T.start_level = lanes[l0].level;
+  T.end_level = lanes[l0 + n - 1].level;
+  T.job = lo;
+  tiles[t] = T;
+  keys[t] = jobs[lo].max_level - T.start_level;
+  idx[t] = int(t);
+}
+
+__global__ void k_gather_tiles(const Tile *__restrict__ in, const int *__restrict__ idx, int64_t n,
+                               Tile *__restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = in[idx[t]];
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1564,6 +1666,15 @@ struct Engine::Impl {
   HostBuf<int32_t> h_flist, h_first_out;
   HostBuf<uint64_t> h_need;
   HostBuf<Lane> h_lanes;
+  // device-built lanes and tiles (persistent sweeps)
+  HostBuf<int2> h_lane_units;
+  DevBuf<int2> d_lane_units;
+  DevBuf<int32_t> d_pos;
+  DevBuf<int64_t> d_lane_off, d_tile_off;
+  HostBuf<int64_t> h_lane_off;
+  DevBuf<Tile> d_tiles_raw;
+  DevBuf<int> d_tile_key, d_tile_idx;
+  DevBuf<uint8_t> d_cub;
   HostBuf<unsigned long long> h_bytes;
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
@@ -1644,7 +1755,7 @@ struct Engine::Impl {
   HostBuf<uint8_t> h_final;
   void finalize_jobs(const std::vector<JobState *> &js);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
-  void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
+  void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles, const std::vector<Tile> *tiles,
              int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st);
   void sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
                    unsigned long long *tstats, lsim_run_stats &st);
@@ -2438,15 +2549,20 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
   }
 }
 
-void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, const std::vector<Tile> &tiles,
-                         int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense,
-                         lsim_run_stats &st) {
-  const int64_t ntiles = int64_t(tiles.size());
+// Lanes and tiles are in d_lanes / d_tiles already, or given on the host
+// (`lanes` and `tiles` not null) and copied there first.
+void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles,
+                         const std::vector<Tile> *tiles, int lvl_entries, int64_t nl_max,
+                         const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st) {
   if (!ntiles) return;
-  d_lanes.alloc(size_t(nlanes));
-  d_tiles.alloc(tiles.size());
-  h2d(d_lanes.p, lanes, size_t(nlanes) * sizeof(Lane));
-  h2d(d_tiles.p, tiles.data(), tiles.size() * sizeof(Tile));
+  if (lanes && tiles) {
+    d_lanes.alloc(size_t(nlanes));
+    d_tiles.alloc(tiles->size());
+    h2d(d_lanes.p, lanes, size_t(nlanes) * sizeof(Lane));
+    h2d(d_tiles.p, tiles->data(), tiles->size() * sizeof(Tile));
+  } else if (plain_circuit) {
+    throw DeviceError(LSIM_E_DEVICE, "the per-level sweep needs its tiles on the host");
+  }
   const size_t lds = size_t(lvl_entries) * sizeof(int);
   const bool trace = trace_on();
   if (trace) {
@@ -2524,8 +2640,9 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
     const CircuitRes &c = *plain_circuit;
     for (int64_t t0 = 0; t0 < ntiles; t0 += slots) {
       const int nb = int(std::min<int64_t>(slots, ntiles - t0));
-      int lmin = tiles[t0].start_level;
-      for (int64_t t = t0; t < t0 + nb; ++t) lmin = std::min(lmin, tiles[t].start_level);
+      const std::vector<Tile> &tv = *tiles;
+      int lmin = tv[size_t(t0)].start_level;
+      for (int64_t t = t0; t < t0 + nb; ++t) lmin = std::min(lmin, tv[size_t(t)].start_level);
       for (int l = lmin; l <= c.L; ++l) {
         const int64_t n = c.live_lvl[l + 1] - c.live_lvl[l];
         if (n <= 0) continue;
@@ -2534,7 +2651,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
         LSIM_HIP(hipGetLastError());
         st.sweep_launches++;
       }
-      for (int64_t t = t0; t < t0 + nb; ++t) st.sweep_node_steps += c.NL - c.live_lvl[tiles[t].start_level];
+      for (int64_t t = t0; t < t0 + nb; ++t) st.sweep_node_steps += c.NL - c.live_lvl[tv[size_t(t)].start_level];
     }
   }
   st.t_sweep_ms += time_end();
@@ -2680,7 +2797,9 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
         if (mark[u]) j.units.push_back(int32_t(u));
     }
   });
-  std::vector<int64_t> unit_off(nj + 1, 0);  // stem units of job i in units_all
+  std::vector<int64_t> unit_off(nj + 1, 0);       // stem units of job i in units_all
+  std::vector<int64_t> lane_unit_off(nj + 1, 0);  // units of job i in the {position, kind} list
+  const bool plain = eo.mode == LSIM_MODE_PLAIN;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
     const CircuitRes &c = *j.c;
@@ -2697,6 +2816,8 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     d.last_flag = (j.kind == JobState::LAST) ? j.last.p : nullptr;
     d.item0 = item_off[i];
     d.word0 = word0[i];
+    d.lane_unit0 = lane_unit_off[i];
+    lane_unit_off[i + 1] = lane_unit_off[i] + int64_t(j.units.size());
     item_off[i + 1] = item_off[i] + nf * j.step_Wn;
     word0[i + 1] = word0[i] + int64_t(j.units.size()) * j.step_Wn;
     devs[i] = d;
@@ -2709,13 +2830,23 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   h_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
   d_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
   units_all.resize(size_t(unit_off[nj]));
+  if (!plain) h_lane_units.alloc(size_t(std::max<int64_t>(lane_unit_off[nj], 1)));
   parallel_for(order, [&](int i) {
     const JobState &j = *act[i];
     std::memcpy(h_flist.p + flist_off[i], j.step_faults.data(), j.step_faults.size() * sizeof(int32_t));
     if (!j.direct)
       for (size_t k = 0; k < j.units.size(); ++k)
         units_all[size_t(unit_off[i]) + k] = make_int4(i, j.units[k], int(k), 0);
+    if (!plain) {
+      int2 *lu = h_lane_units.p + lane_unit_off[i];
+      for (size_t k = 0; k < j.units.size(); ++k)
+        lu[k] = make_int2(j.units[k], int(j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP));
+    }
   });
+  if (!plain && lane_unit_off[nj]) {
+    d_lane_units.alloc(size_t(lane_unit_off[nj]));
+    h2d(d_lane_units.p, h_lane_units.p, size_t(lane_unit_off[nj]) * sizeof(int2));
+  }
   for (int i = 0; i < nj; ++i) devs[i].flist = d_flist.p + flist_off[i];
   h2d(d_flist.p, h_flist.p, size_t(nf_all) * sizeof(int32_t));
   d_jobs.alloc(nj);
@@ -2747,111 +2878,172 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     k_local<<<blocks_for(items, 256), 256, 0, stream>>>(ia);
     LSIM_HIP(hipGetLastError());
   }
-  // The needed bits decide which (unit, word) pairs become sweep lanes.
-  h_need.alloc(size_t(words));
-  const uint64_t *need = h_need.p;
-  d2h(h_need.p, d_need.p, size_t(word0[nj]) * sizeof(uint64_t));
-  st.t_detect_ms += time_end();
-  const auto h2 = clk::now();
-  // Lanes: one per (unit, word) with needed bits, per job in unit order
-  // (units are level-sorted), counted then filled in parallel over chunks of
-  // a job's units.
-  constexpr int64_t kUnitChunk = 4096;
-  struct LaneChunk {
-    int job;
-    int64_t k0, k1, off;
-  };
-  std::vector<LaneChunk> chunks;
-  for (int i = 0; i < nj; ++i) {
-    const int64_t nu = int64_t(act[i]->units.size());
-    for (int64_t k0 = 0; k0 < nu; k0 += kUnitChunk) chunks.push_back({i, k0, std::min(nu, k0 + kUnitChunk), 0});
-  }
-  std::vector<int> chunk_ids(chunks.size());
-  std::iota(chunk_ids.begin(), chunk_ids.end(), 0);
-  parallel_for(chunk_ids, [&](int ci) {
-    LaneChunk &ch = chunks[size_t(ci)];
-    const int64_t Wn = act[ch.job]->step_Wn;
-    int64_t n = 0;
-    for (int64_t o = word0[ch.job] + ch.k0 * Wn, e = word0[ch.job] + ch.k1 * Wn; o < e; ++o) n += need[o] != 0;
-    ch.off = n;
-  });
-  std::vector<int64_t> lane_off(nj + 1, 0);
-  {
-    int64_t run = 0;
-    size_t c = 0;
-    for (int i = 0; i < nj; ++i) {
-      lane_off[i] = run;
-      for (; c < chunks.size() && chunks[c].job == i; ++c) {
-        const int64_t n = chunks[c].off;
-        chunks[c].off = run;
-        run += n;
-      }
-    }
-    lane_off[nj] = run;
-  }
-  const int64_t nlanes = lane_off[nj];
-  h_lanes.alloc(size_t(std::max<int64_t>(nlanes, 1)));
-  Lane *lanes = h_lanes.p;
-  parallel_for(chunk_ids, [&](int ci) {
-    const LaneChunk &ch = chunks[size_t(ci)];
-    const int i = ch.job;
-    const JobState &j = *act[i];
-    const int Wn = j.step_Wn;
-    Lane *out = lanes + ch.off;
-    for (int64_t k = ch.k0; k < ch.k1; ++k) {
-      const int32_t u = j.units[size_t(k)];
-      const int32_t kind = j.direct ? (j.sa[j.step_faults[size_t(k)]] ? K_SA1 : K_SA0) : K_FLIP;
-      const int32_t level = j.c->level_dp[u];
-      for (int jw = 0; jw < Wn; ++jw) {
-        const int64_t o = word0[i] + k * Wn + jw;
-        const uint64_t msk = need[o];
-        if (!msk) continue;
-        Lane ln;
-        ln.mask = msk;
-        ln.site = u;
-        ln.word = int32_t(j.step_w0 + jw);
-        ln.out = int32_t(o);
-        ln.kind = kind;
-        ln.level = level;
-        ln.pad = 0;
-        *out++ = ln;
-      }
-    }
-  });
-  std::vector<Tile> tiles;
-  for (int i = 0; i < nj; ++i) {
-    JobState &j = *act[i];
-    // The pair sweep finds a node's injecting lanes by binary search: a job's
-    // lanes follow its units, which are in ascending position.
-    for (int64_t l = lane_off[i] + 1; l < lane_off[i + 1]; ++l)
-      if (lanes[l].site < lanes[l - 1].site) throw DeviceError(LSIM_E_DEVICE, "sweep lanes out of site order");
-    for (int64_t l0 = lane_off[i]; l0 < lane_off[i + 1]; l0 += 64) {
-      Tile t;
-      t.lane0 = int32_t(l0);
-      t.nlanes = int32_t(std::min<int64_t>(64, lane_off[i + 1] - l0));
-      t.start_level = lanes[l0].level;
-      t.end_level = lanes[l0 + t.nlanes - 1].level;
-      t.job = i;
-      tiles.push_back(t);
-    }
-    j.res.stats.stems += j.direct ? 0 : int64_t(j.units.size());
-  }
-  // Deep tiles first: the queue then ends with short tiles, balancing the tail.
-  std::stable_sort(tiles.begin(), tiles.end(), [&](const Tile &x, const Tile &y) {
-    return (devs[x.job].max_level - x.start_level) > (devs[y.job].max_level - y.start_level);
-  });
-  const bool plain = eo.mode == LSIM_MODE_PLAIN;
-  const auto h3 = clk::now();
   // 512-thread pair sweep groups for dense steps (a dropping job's first
   // step: all faults, most lanes active at most nodes; full-matrix steps) and
   // when most tiles walk levels wider than 256 nodes (one scan pass each).
   bool dense = false;
-  int64_t wide_tiles = 0;
   for (const JobState *j : act)
     dense = dense || j->kind == JobState::MATRIX || (j->kind == JobState::DROP && j->w0 == 0);
-  for (const Tile &t : tiles) wide_tiles += act[t.job]->c->wide ? 1 : 0;
-  const bool big_groups = dense || 2 * wide_tiles > int64_t(tiles.size());
-  sweep(eo, lanes, nlanes, tiles, lvl_entries, nl_max, plain ? act[0]->c : nullptr, big_groups, st);
+  int64_t wide_tiles = 0, ntiles = 0, nlanes = 0;
+  clk::time_point h2, h3;
+  if (!plain) {
+    // Lanes and tiles built on the device: the needed bits never leave it.
+    const int64_t nw = word0[nj];
+    d_pos.alloc(size_t(words));
+    d_lanes.alloc(size_t(words));
+    d_lane_off.alloc(size_t(nj + 1));
+    h_lane_off.alloc(size_t(nj + 1));
+    if (nw >= (int64_t(1) << 31)) throw DeviceError(LSIM_E_DEVICE, "too many unit words in one step");
+    if (nw) {
+      hipcub::TransformInputIterator<int, NeedNonzero, const uint64_t *> flags(d_need.p, NeedNonzero());
+      size_t tmp = 0;
+      LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flags, d_pos.p, int(nw), stream));
+      d_cub.alloc(tmp);
+      LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(d_cub.p, tmp, flags, d_pos.p, int(nw), stream));
+      k_lane_fill<<<blocks_for(nw, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_lane_units.p, nw,
+                                                          d_lanes.p);
+      LSIM_HIP(hipGetLastError());
+    }
+    k_lane_bounds<<<blocks_for(nj + 1, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, nw, d_lane_off.p);
+    LSIM_HIP(hipGetLastError());
+    d2h(h_lane_off.p, d_lane_off.p, size_t(nj + 1) * sizeof(int64_t));
+    st.t_detect_ms += time_end();
+    h2 = clk::now();
+    nlanes = h_lane_off.p[nj];
+    std::vector<int64_t> tile_off(nj + 1, 0);
+    for (int i = 0; i < nj; ++i) {
+      const int64_t nt = (h_lane_off.p[i + 1] - h_lane_off.p[i] + 63) / 64;
+      tile_off[i + 1] = tile_off[i] + nt;
+      if (act[i]->c->wide) wide_tiles += nt;
+      act[i]->res.stats.stems += act[i]->direct ? 0 : int64_t(act[i]->units.size());
+    }
+    ntiles = tile_off[nj];
+    if (ntiles) {
+      if (ntiles >= (int64_t(1) << 31)) throw DeviceError(LSIM_E_DEVICE, "too many sweep tiles in one step");
+      d_tile_off.alloc(size_t(nj + 1));
+      h2d(d_tile_off.p, tile_off.data(), size_t(nj + 1) * sizeof(int64_t));
+      d_tiles_raw.alloc(size_t(ntiles));
+      d_tiles.alloc(size_t(ntiles));
+      d_tile_key.alloc(size_t(ntiles) * 2);
+      d_tile_idx.alloc(size_t(ntiles) * 2);
+      int *keys = d_tile_key.p, *keys2 = d_tile_key.p + ntiles, *idx = d_tile_idx.p, *idx2 = d_tile_idx.p + ntiles;
+      k_make_tiles<<<blocks_for(ntiles, 256), 256, 0, stream>>>(d_jobs.p, nj, d_lanes.p, d_lane_off.p, d_tile_off.p,
+                                                               ntiles, d_tiles_raw.p, keys, idx);
+      LSIM_HIP(hipGetLastError());
+      // Deep tiles first (stable): the queue then ends with short tiles,
+      // balancing the tail.
+      size_t tmp = 0;
+      LSIM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, keys, keys2, idx, idx2, int(ntiles), 0,
+                                                            int(sizeof(int) * 8), stream));
+      d_cub.alloc(tmp);
+      LSIM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(d_cub.p, tmp, keys, keys2, idx, idx2, int(ntiles), 0,
+                                                            int(sizeof(int) * 8), stream));
+      k_gather_tiles<<<blocks_for(ntiles, 256), 256, 0, stream>>>(d_tiles_raw.p, idx2, ntiles, d_tiles.p);
+      LSIM_HIP(hipGetLastError());
+    }
+    h3 = clk::now();
+    const bool big_groups = dense || 2 * wide_tiles > ntiles;
+    sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, st);
+  } else {
+    // The needed bits decide which (unit, word) pairs become sweep lanes.
+    h_need.alloc(size_t(words));
+    const uint64_t *need = h_need.p;
+    d2h(h_need.p, d_need.p, size_t(word0[nj]) * sizeof(uint64_t));
+    st.t_detect_ms += time_end();
+    h2 = clk::now();
+    // Lanes: one per (unit, word) with needed bits, per job in unit order
+    // (units are level-sorted), counted then filled in parallel over chunks of
+    // a job's units.
+    constexpr int64_t kUnitChunk = 4096;
+    struct LaneChunk {
+      int job;
+      int64_t k0, k1, off;
+    };
+    std::vector<LaneChunk> chunks;
+    for (int i = 0; i < nj; ++i) {
+      const int64_t nu = int64_t(act[i]->units.size());
+      for (int64_t k0 = 0; k0 < nu; k0 += kUnitChunk) chunks.push_back({i, k0, std::min(nu, k0 + kUnitChunk), 0});
+    }
+    std::vector<int> chunk_ids(chunks.size());
+    std::iota(chunk_ids.begin(), chunk_ids.end(), 0);
+    parallel_for(chunk_ids, [&](int ci) {
+      LaneChunk &ch = chunks[size_t(ci)];
+      const int64_t Wn = act[ch.job]->step_Wn;
+      int64_t n = 0;
+      for (int64_t o = word0[ch.job] + ch.k0 * Wn, e = word0[ch.job] + ch.k1 * Wn; o < e; ++o) n += need[o] != 0;
+      ch.off = n;
+    });
+    std::vector<int64_t> lane_off(nj + 1, 0);
+    {
+      int64_t run = 0;
+      size_t c = 0;
+      for (int i = 0; i < nj; ++i) {
+        lane_off[i] = run;
+        for (; c < chunks.size() && chunks[c].job == i; ++c) {
+          const int64_t n = chunks[c].off;
+          chunks[c].off = run;
+          run += n;
+        }
+      }
+      lane_off[nj] = run;
+    }
+    nlanes = lane_off[nj];
+    h_lanes.alloc(size_t(std::max<int64_t>(nlanes, 1)));
+    Lane *lanes = h_lanes.p;
+    parallel_for(chunk_ids, [&](int ci) {
+      const LaneChunk &ch = chunks[size_t(ci)];
+      const int i = ch.job;
+      const JobState &j = *act[i];
+      const int Wn = j.step_Wn;
+      Lane *out = lanes + ch.off;
+      for (int64_t k = ch.k0; k < ch.k1; ++k) {
+        const int32_t u = j.units[size_t(k)];
+        const int32_t kind = j.direct ? (j.sa[j.step_faults[size_t(k)]] ? K_SA1 : K_SA0) : K_FLIP;
+        const int32_t level = j.c->level_dp[u];
+        for (int jw = 0; jw < Wn; ++jw) {
+          const int64_t o = word0[i] + k * Wn + jw;
+          const uint64_t msk = need[o];
+          if (!msk) continue;
+          Lane ln;
+          ln.mask = msk;
+          ln.site = u;
+          ln.word = int32_t(j.step_w0 + jw);
+          ln.out = int32_t(o);
+          ln.kind = kind;
+          ln.level = level;
+          ln.pad = 0;
+          *out++ = ln;
+        }
+      }
+    });
+    std::vector<Tile> tiles;
+    for (int i = 0; i < nj; ++i) {
+      JobState &j = *act[i];
+      // The pair sweep finds a node's injecting lanes by binary search: a job's
+      // lanes follow its units, which are in ascending position.
+      for (int64_t l = lane_off[i] + 1; l < lane_off[i + 1]; ++l)
+        if (lanes[l].site < lanes[l - 1].site) throw DeviceError(LSIM_E_DEVICE, "sweep lanes out of site order");
+      for (int64_t l0 = lane_off[i]; l0 < lane_off[i + 1]; l0 += 64) {
+        Tile t;
+        t.lane0 = int32_t(l0);
+        t.nlanes = int32_t(std::min<int64_t>(64, lane_off[i + 1] - l0));
+        t.start_level = lanes[l0].level;
+        t.end_level = lanes[l0 + t.nlanes - 1].level;
+        t.job = i;
+        tiles.push_back(t);
+      }
+      j.res.stats.stems += j.direct ? 0 : int64_t(j.units.size());
+    }
+    // Deep tiles first: the queue then ends with short tiles, balancing the tail.
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const Tile &x, const Tile &y) {
+      return (devs[x.job].max_level - x.start_level) > (devs[y.job].max_level - y.start_level);
+    });
+    h3 = clk::now();
+    for (const Tile &t : tiles) wide_tiles += act[t.job]->c->wide ? 1 : 0;
+    ntiles = int64_t(tiles.size());
+    const bool big_groups = dense || 2 * wide_tiles > ntiles;
+    sweep(eo, lanes, nlanes, ntiles, &tiles, lvl_entries, nl_max, act[0]->c, big_groups, st);
+  }
   const auto h4 = clk::now();
   time_begin();
   if (items) {
@@ -2911,7 +3103,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     std::fprintf(stderr,
                  "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld | prep %.2f local %.2f "
                  "lanes %.2f sweep %.2f (dev %.2f) detect %.2f advance %.2f ms\n",
-                 nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)tiles.size(),
+                 nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)ntiles,
                  ms(h0, h1), ms(h1, h2), ms(h2, h3), ms(h3, h4), st.t_sweep_ms, ms(h4, h5), ms(h5, clk::now()));
   }
 }
