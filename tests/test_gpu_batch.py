@@ -69,19 +69,23 @@ def test_batch_matches_single_runs(kernel):
 
 
 def test_batch_reference_made_vectors(ref_vectors):
-    """Jobs of different circuits, with and without MUX, in one batch."""
+    """Jobs of different circuits, with and without MUX, with an empty fault
+    list and with no patterns, in one batch: each case once with the full
+    d(f, r) matrix and once with fault dropping."""
     b = lsim_amd.Batch(device=0)
     ids = []
     for c in ref_vectors:
         cid = b.add_circuit(c["ckt"])
-        ids.append((b.add_job(cid, c["flst"], c["ptn"], drop=False), c))
+        ids.append((b.add_job(cid, c["flst"], c["ptn"], drop=False), b.add_job(cid, c["flst"], c["ptn"], drop=True), c))
     b.run()
-    for jid, c in ids:
+    for jid, jdrop, c in ids:
         F = b._L.lsim_batch_job_faults(b._h, jid)
         R = b._L.lsim_batch_job_rounds(b._h, jid)
         det = np.array(c["det"], dtype=np.uint8).reshape(F, R)
         assert np.array_equal(b.detect_matrix(jid), det), c["name"]
         assert list(b.found_fault_to_pattern(jid)) == c["table"], c["name"]
+        want_first = np.where(det.any(1), det.argmax(1), -1) if R else -np.ones(F, dtype=np.int64)
+        assert np.array_equal(b.first_detect_round(jdrop), want_first), c["name"]
 
 
 def test_batch_plain_mode_matches():
