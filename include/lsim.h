@@ -116,6 +116,14 @@ LSIM_API int lsim_num_gates_per_level(const lsim_sim *s, uint64_t *out, size_t n
 /* Levels of pi(i), gate(i), po(i) in topological order (ElementBase::level(),
  * fsim/base_ppg.hpp:49); n = P+G+O. */
 LSIM_API int lsim_levels(const lsim_sim *s, uint64_t *out, size_t n);
+/* Simulator::visualization_graph(path) (fsim/simulation.hpp:93,
+ * fsim/simulation.cpp:1157-1225): writes the circuit as a Graphviz digraph,
+ * one edge per fanin of every gate and PO in topological order. LSIM_E_IO
+ * when the file cannot be opened (the reference prints the message and
+ * returns). lsim_visualization_text returns the same text in a buffer the
+ * caller frees with lsim_free_text. */
+LSIM_API int lsim_visualization_graph(const lsim_sim *s, const char *path);
+LSIM_API int lsim_visualization_text(const lsim_sim *s, char **text, size_t *len);
 /* Simulator::faults() (fsim/simulation.hpp:70): site position and stuck value
  * as the reference stores it (0 or all ones); n = number of faults. */
 LSIM_API int lsim_faults(const lsim_sim *s, uint64_t *site, uint64_t *value, size_t n);

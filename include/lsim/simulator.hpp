@@ -6,6 +6,7 @@
 // does (fsim/simulation.cpp:24-32).
 #pragma once
 #include <cstdint>
+#include <iostream>
 #include <istream>
 #include <iterator>
 #include <stdexcept>
@@ -97,6 +98,16 @@ class Simulator {
   const std::vector<size_t> &gate_idx_to_tpg_order_idx() const { return order_; }
   const std::vector<size_t> &num_gates_per_level() const { return per_level_; }
   const std::vector<size_t> &found_fault_to_pattern() const { return table_; }
+  // fsim/simulation.cpp:1157-1225: on a file that cannot be opened the
+  // reference prints the message and returns; so does this.
+  void visualization_graph(const std::string &outputFile) {
+    const int rc = lsim_visualization_graph(h_, outputFile.c_str());
+    if (rc == LSIM_E_IO) {
+      std::cerr << lsim_last_error(h_) << std::endl;
+      return;
+    }
+    check(rc);
+  }
 
   // No reference counterpart: first detecting round per fault (-1: never).
   std::vector<int64_t> first_detect_round() const {

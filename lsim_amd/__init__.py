@@ -109,6 +109,8 @@ def lib() -> ctypes.CDLL:
         "lsim_gate_idx_to_tpg_order_idx": (ctypes.c_int, [P, u64p, sz]),
         "lsim_num_gates_per_level": (ctypes.c_int, [P, u64p, sz]),
         "lsim_levels": (ctypes.c_int, [P, u64p, sz]),
+        "lsim_visualization_graph": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "lsim_visualization_text": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_faults": (ctypes.c_int, [P, u64p, u64p, sz]),
         "lsim_patterns": (ctypes.c_int, [P, u64p, sz]),
         "lsim_good_results": (ctypes.c_int, [P, u64p, sz, u64p, sz, u64p, sz]),
@@ -289,6 +291,18 @@ class Simulator:
         """Levels of pi(i), gate(i), po(i) concatenated (topological order)."""
         n = self.num_PIs() + self.num_POs() + self.num_gates()
         return self._u64(self._L.lsim_levels, n)
+
+    def visualization_graph(self, path: str) -> None:
+        """Graphviz digraph of the circuit written to `path`
+        (fsim/simulation.cpp:1157-1225)."""
+        self._check(self._L.lsim_visualization_graph(self._h, path.encode()))
+
+    def visualization_text(self) -> str:
+        b, n = ctypes.c_void_p(), ctypes.c_size_t()
+        self._check(self._L.lsim_visualization_text(self._h, ctypes.byref(b), ctypes.byref(n)))
+        out = ctypes.string_at(b.value, n.value)
+        self._L.lsim_free_text(b)
+        return out.decode()
 
     def faults(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.num_faults()

@@ -2,6 +2,7 @@
 // exceptions and turns them into LSIM_E_* codes plus a message.
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <fstream>
@@ -1032,6 +1033,57 @@ static int emit_text(const std::string &x, char **o, size_t *n) {
   std::memcpy(*o, x.data(), x.size());
   (*o)[x.size()] = 0;
   *n = x.size();
+  return LSIM_OK;
+}
+
+// Graphviz text of the circuit in the reference's layout
+// (fsim/simulation.cpp:1157-1225): gates, then POs, one edge per fanin in pin
+// order; PIs are named by position, gates by position among the gates and
+// their type, POs by position among the POs.
+static std::string dot_text(const lsim::Netlist &nl) {
+  static const char *kNames[] = {"INV", "AND", "OR", "XOR", "NAND", "NOR", "XNOR", "MUX", "CLKBUF", "PI", "PO"};
+  auto tname = [](uint8_t t) { return t < 11 ? kNames[t] : "UNKNOWN"; };
+  const int64_t P = nl.P, G = nl.G;
+  std::string o = "digraph G {\n\tsubgraph cluster_0 {\n\t\tcolor=black;\n\t\tnode [style=filled,color=lightgrey];\n";
+  auto source = [&](int32_t q) {
+    if (nl.type[size_t(q)] == lsim::G_PI) return "PI_" + std::to_string(q) + " -> ";
+    return "Gate_" + std::to_string(int64_t(q) - P) + "_" + tname(nl.type[size_t(q)]) + " -> ";
+  };
+  for (int64_t p = P; p < P + G; ++p) {
+    const std::string now = "Gate_" + std::to_string(p - P) + "_" + tname(nl.type[size_t(p)]) + ";";
+    for (int32_t e = nl.fin_off[size_t(p)]; e < nl.fin_off[size_t(p) + 1]; ++e)
+      o += "\t\t" + source(nl.fin[size_t(e)]) + now + "\n";
+  }
+  for (int64_t p = P + G; p < nl.N; ++p) {
+    const std::string now = "PO_" + std::to_string(p - P - G) + ";";
+    for (int32_t e = nl.fin_off[size_t(p)]; e < nl.fin_off[size_t(p) + 1]; ++e)
+      o += "\t\t" + source(nl.fin[size_t(e)]) + now + "\n";
+  }
+  o += "\t}\n\n";
+  for (int64_t i = 0; i < P; ++i) o += "\tPI_" + std::to_string(i) + "  [shape=Msquare];\n";
+  for (int64_t i = 0; i < nl.O; ++i) o += "\tPO_" + std::to_string(i) + "  [shape=Msquare];\n";
+  o += "}\n\n";
+  return o;
+}
+
+int lsim_visualization_text(const lsim_sim *s, char **text, size_t *len) {
+  if (!text || !len) return LSIM_E_ARG;
+  *text = nullptr;
+  int rc = check_loaded(s);
+  if (rc) return rc;
+  return emit_text(dot_text(s->nl), text, len);
+}
+
+int lsim_visualization_graph(const lsim_sim *s, const char *path) {
+  int rc = check_loaded(s);
+  if (rc) return rc;
+  if (!path) return LSIM_E_ARG;
+  std::string *err = &const_cast<lsim_sim *>(s)->err;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(err, LSIM_E_IO, std::string("Error opening output file: ") + path);
+  const std::string t = dot_text(s->nl);
+  const bool ok = std::fwrite(t.data(), 1, t.size(), f) == t.size();
+  if (std::fclose(f) != 0 || !ok) return fail(err, LSIM_E_IO, std::string("cannot write ") + path);
   return LSIM_OK;
 }
 

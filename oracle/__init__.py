@@ -166,6 +166,22 @@ def ref_available() -> bool:
     return os.path.exists(REF_BIN)
 
 
+def reference_dot(ckt, flst, ptn) -> str:
+    """The reference's Simulator::visualization_graph text for the three texts
+    (fsim/simulation.cpp:1157-1225), from the compiled reference."""
+    with tempfile.TemporaryDirectory() as d:
+        paths = []
+        for name, data in (("c.ckt", ckt), ("f.flst", flst), ("p.ptn", ptn)):
+            path = os.path.join(d, name)
+            with open(path, "wb") as fh:
+                fh.write(_b(data))
+            paths.append(path)
+        dot = os.path.join(d, "g.dot")
+        subprocess.run([REF_BIN] + paths + ["0", "1", "-", dot], check=True, capture_output=True, text=True)
+        with open(dot) as fh:
+            return fh.read()
+
+
 def run_reference(ckt, flst, ptn, mode: int = 0, threads: int = 1, want_output: bool = True):
     """Run the reference binary on the three texts. Returns (dict, seconds) where
     dict holds counts, order map, gates per level, levels, table and good values."""

@@ -5,8 +5,9 @@
 //
 // TEST INFRASTRUCTURE ONLY (checker and CPU baseline).
 //
-// usage: fsim_ref <ckt> <flst> <ptn> <mode 0|1|2> <threads> <out.txt|-> [rounds]
-// prints "read_s <t> run_s <t>" on stdout.
+// usage: fsim_ref <ckt> <flst> <ptn> <mode 0|1|2> <threads> <out.txt|-> [graph.dot]
+// prints "read_s <t> run_s <t>" on stdout; with graph.dot, also writes the
+// reference's Simulator::visualization_graph there.
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -30,6 +31,7 @@ int main(int argc, char **argv) {
   auto t2 = std::chrono::steady_clock::now();
   std::printf("read_s %.6f run_s %.6f\n", std::chrono::duration<double>(t1 - t0).count(),
               std::chrono::duration<double>(t2 - t1).count());
+  if (argc > 7 && std::string(argv[7]) != "-") sim.visualization_graph(std::string(argv[7]));
   const std::string out = argv[6];
   if (out == "-") return 0;
   std::ofstream o(out);

@@ -3,7 +3,8 @@
 // then print every accessor as "name v0 v1 ..." lines for the Python test to
 // compare against the golden vectors.
 //
-// usage: facade_check <device> <ckt> <flst> <ptn> <stream|path> <run 0|1>
+// usage: facade_check <device> <ckt> <flst> <ptn> <stream|path> <run 0|1> [graph.dot]
+// (with graph.dot, also Simulator::visualization_graph as main/simple.cpp:62-65)
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -18,8 +19,8 @@ static void line(const char *name, const V &v) {
 }
 
 int main(int argc, char **argv) {
-  if (argc != 7) {
-    std::cerr << "usage: facade_check device ckt flst ptn stream|path run\n";
+  if (argc != 7 && argc != 8) {
+    std::cerr << "usage: facade_check device ckt flst ptn stream|path run [graph.dot]\n";
     return 2;
   }
   try {
@@ -42,6 +43,7 @@ int main(int argc, char **argv) {
     for (const auto &f : sim.faults()) std::cout << ' ' << f.gate_with_fault() << ' ' << (f.gate_SA_fault_val() ? 1 : 0);
     std::cout << '\n';
     if (!sim.patterns().empty()) line("ptn0", sim.patterns()[0].output_value());
+    if (argc == 8) sim.visualization_graph(argv[7]);
     if (std::stoi(argv[6])) {
       sim.run(lsim::Simulator::Mode::SEQUENTIAL, 1);
       line("pi", sim.g_pi_results());

@@ -10,14 +10,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "_bin", "facade_check")
 
 
-def _run(case, tmp_path, device, how, run):
+def _run(case, tmp_path, device, how, run, dot=None):
     paths = []
     for key, ext in (("ckt", "ckt"), ("flst", "flst"), ("ptn", "ptn")):
         p = tmp_path / f"case.{ext}"
         p.write_text(case[key])
         paths.append(str(p))
-    r = subprocess.run([BIN, str(device)] + paths + [how, str(int(run))], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([BIN, str(device)] + paths + [how, str(int(run))] + ([dot] if dot else []),
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = {}
     for line in r.stdout.splitlines():
@@ -44,6 +44,25 @@ def test_facade_structure(unit_vectors, tmp_path, how):
         pytest.fail("tests/cpp/_bin/facade_check is not built (lsim_amd/build.py builds it)")
     for c in unit_vectors:
         _check_structure(c, _run(c, tmp_path, -1, how, False))
+
+
+def test_facade_visualization_graph(unit_vectors, tmp_path):
+    """Simulator::visualization_graph through the facade writes the library's text."""
+    import lsim_amd
+
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/_bin/facade_check is not built (lsim_amd/build.py builds it)")
+    for c in unit_vectors:
+        dot = tmp_path / "g.dot"
+        _run(c, tmp_path, -1, "path", False, str(dot))
+        s = lsim_amd.Simulator(device=-1)
+        s.read_text(c["ckt"], c["flst"], c["ptn"])
+        assert dot.read_text() == s.visualization_text(), c["name"]
+    # an unopenable path: the message on stderr, no failure (fsim/simulation.cpp:1167-1170)
+    paths = [str(tmp_path / f"case.{e}") for e in ("ckt", "flst", "ptn")]
+    r = subprocess.run([BIN, "-1"] + paths + ["path", "0", str(tmp_path / "missing" / "g.dot")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "Error opening output file" in r.stderr
 
 
 @pytest.mark.gpu
