@@ -17,6 +17,9 @@
 
 namespace lsim {
 
+// Gate type codes of the .ckt format (fsim/base_ppg.hpp:19-32).
+enum class GateType { INV = 0, AND, OR, XOR, NAND, NOR, XNOR, MUX, CLKBUF, PI, PO, MAX_GATE_TYPE };
+
 class Element {
  public:
   Element(size_t idx, size_t level) : idx_(idx), level_(level) {}
@@ -98,6 +101,12 @@ class Simulator {
   const std::vector<size_t> &gate_idx_to_tpg_order_idx() const { return order_; }
   const std::vector<size_t> &num_gates_per_level() const { return per_level_; }
   const std::vector<size_t> &found_fault_to_pattern() const { return table_; }
+  // fsim/simulation.cpp:1126-1154.
+  std::string gateTypeToString(GateType type) {
+    static const char *kNames[] = {"INV", "AND", "OR", "XOR", "NAND", "NOR", "XNOR", "MUX", "CLKBUF", "PI", "PO"};
+    const int t = static_cast<int>(type);
+    return t >= 0 && t < 11 ? kNames[t] : "UNKNOWN";
+  }
   // fsim/simulation.cpp:1157-1225: on a file that cannot be opened the
   // reference prints the message and returns; so does this.
   void visualization_graph(const std::string &outputFile) {

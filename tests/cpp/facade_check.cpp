@@ -44,6 +44,10 @@ int main(int argc, char **argv) {
     std::cout << '\n';
     if (!sim.patterns().empty()) line("ptn0", sim.patterns()[0].output_value());
     if (argc == 8) sim.visualization_graph(argv[7]);
+    std::cout << "type_names";
+    for (int t = 0; t <= static_cast<int>(lsim::GateType::MAX_GATE_TYPE); ++t)
+      std::cout << ' ' << sim.gateTypeToString(static_cast<lsim::GateType>(t));
+    std::cout << '\n';
     if (std::stoi(argv[6])) {
       sim.run(lsim::Simulator::Mode::SEQUENTIAL, 1);
       line("pi", sim.g_pi_results());

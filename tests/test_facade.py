@@ -22,7 +22,7 @@ def _run(case, tmp_path, device, how, run, dot=None):
     out = {}
     for line in r.stdout.splitlines():
         name, *vals = line.split()
-        out[name] = [int(v) for v in vals]
+        out[name] = vals if name == "type_names" else [int(v) for v in vals]
     return out
 
 
@@ -36,6 +36,9 @@ def _check_structure(c, out):
         want += [c["tpg"][c["faults"][2 * i]], c["faults"][2 * i + 1]]
     assert out["faults"] == want
     assert out["ptn0"] == c["ptn_val"]
+    # fsim/simulation.cpp:1126-1154
+    assert out["type_names"] == ["INV", "AND", "OR", "XOR", "NAND", "NOR", "XNOR", "MUX", "CLKBUF", "PI", "PO",
+                                 "UNKNOWN"]
 
 
 @pytest.mark.parametrize("how", ["stream", "path"])
