@@ -264,6 +264,24 @@ LSIM_API void lsim_free_text(char *p);
 #define LSIM_ORDER_DFS 1
 LSIM_API int lsim_break_loops(const char *ckt, size_t ckt_len, int mode, char **out, size_t *out_len,
                               int64_t *removed);
+/* OpenTimer timing-graph dump -> .ckt (fsim/parser.py:1-227): sections by their
+ * marker lines, cell types to gate codes by name prefix (AOI/OAI/DFF become
+ * NAND/NOR), pins through the converter's per-cell table, node ids PIs, POs,
+ * cells in file order, loops removed by a BFS or DFS traversal from the PIs
+ * (mode LSIM_ORDER_BFS / LSIM_ORDER_DFS). A source's destinations keep the
+ * dump's order (the converter's Python set has no fixed order). Returns the
+ * .ckt text (free with lsim_free_text); `stats` may be null. */
+typedef struct lsim_ingest_stats {
+  int64_t pis, pos, cells, wires;
+  int64_t connections;   /* "a -> b" lines read */
+  int64_t records;       /* wire records written */
+  int64_t removed;       /* destinations dropped by the loop removal */
+  int64_t unknown_cells; /* cell types mapped to AND */
+  int64_t unknown_pins;  /* pins mapped to pin 0 */
+  int64_t unreached;     /* nodes reached that drive nothing (POs excluded) */
+} lsim_ingest_stats;
+LSIM_API int lsim_convert_timing_dump(const char *dump, size_t dump_len, int mode, char **ckt, size_t *ckt_len,
+                                      lsim_ingest_stats *stats);
 /* The fault list or the patterns alone (same seeds and rules as lsim_generate):
  * SA0/SA1 over inner gate ids [P+O, P+O+G) thinned to `fraction`, or
  * ceil(n/64) rounds of P random words. */

@@ -166,6 +166,8 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_generate_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
                                                   ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
+        "lsim_convert_timing_dump": (ctypes.c_int, [ctypes.c_char_p, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                                    ctypes.POINTER(sz), ctypes.POINTER(IngestStats)]),
         "lsim_probe_random_reads": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                                    ctypes.POINTER(ctypes.c_double)]),
     }
@@ -419,6 +421,25 @@ def generate_patterns(seed: int, num_pis: int, num_patterns: int) -> bytes:
     b, n = ctypes.c_void_p(), ctypes.c_size_t()
     rc = lib().lsim_generate_patterns(seed, num_pis, num_patterns, ctypes.byref(b), ctypes.byref(n))
     return _take_text(rc, b, n)
+
+
+class IngestStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("pis", "pos", "cells", "wires", "connections", "records", "removed",
+                                              "unknown_cells", "unknown_pins", "unreached")]
+
+
+def convert_timing_dump(dump, mode: str = "bfs"):
+    """(.ckt text, stats) of an OpenTimer timing-graph dump (fsim/parser.py):
+    cell and pin mapping, node ids, and loop removal by a BFS or DFS traversal
+    from the PIs."""
+    d = _as_bytes(dump)
+    if mode not in ("bfs", "dfs"):
+        raise ValueError("mode must be 'bfs' or 'dfs'")
+    b, n, st = ctypes.c_void_p(), ctypes.c_size_t(), IngestStats()
+    rc = lib().lsim_convert_timing_dump(d, len(d), 0 if mode == "bfs" else 1, ctypes.byref(b), ctypes.byref(n),
+                                        ctypes.byref(st))
+    text = _take_text(rc, b, n)
+    return text, {name: getattr(st, name) for name, _ in IngestStats._fields_}
 
 
 def probe_random_reads(device: int = 0, nbytes: int = 16 << 30) -> dict:

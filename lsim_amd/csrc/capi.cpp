@@ -16,6 +16,7 @@
 
 #include "engine.hpp"
 #include "generator.hpp"
+#include "ingest.hpp"
 #include "loops.hpp"
 #include "probe.hpp"
 #include "lsim.h"
@@ -1105,6 +1106,31 @@ int lsim_break_loops(const char *ckt, size_t ckt_len, int mode, char **out, size
   if (rc) return rc;
   if (removed) *removed = st.removed;
   return emit_text(text, out, out_len);
+}
+
+int lsim_convert_timing_dump(const char *dump, size_t dump_len, int mode, char **ckt, size_t *ckt_len,
+                             lsim_ingest_stats *stats) {
+  if (!ckt || !ckt_len || (!dump && dump_len) || (mode != LSIM_ORDER_BFS && mode != LSIM_ORDER_DFS))
+    return LSIM_E_ARG;
+  *ckt = nullptr;
+  std::string text;
+  lsim::IngestStats st;
+  int rc = guarded(&g_create_error,
+                   [&] { text = lsim::convert_timing_dump(dump, dump_len, mode == LSIM_ORDER_DFS, &st); });
+  if (rc) return rc;
+  if (stats) {
+    stats->pis = st.pis;
+    stats->pos = st.pos;
+    stats->cells = st.cells;
+    stats->wires = st.wires;
+    stats->connections = st.connections;
+    stats->records = st.records;
+    stats->removed = st.removed;
+    stats->unknown_cells = st.unknown_cells;
+    stats->unknown_pins = st.unknown_pins;
+    stats->unreached = st.unreached;
+  }
+  return emit_text(text, ckt, ckt_len);
 }
 
 int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn, size_t *ptn_len) {
