@@ -282,6 +282,23 @@ typedef struct lsim_ingest_stats {
 } lsim_ingest_stats;
 LSIM_API int lsim_convert_timing_dump(const char *dump, size_t dump_len, int mode, char **ckt, size_t *ckt_len,
                                       lsim_ingest_stats *stats);
+/* Producers with the conventions of fsim/generate_fault_patterns.py (its
+ * random draws replaced by the library's seeded stream):
+ *  - lsim_produce_faults: F distinct (wire id, value) pairs, ids uniform over
+ *    [0, N) less the PO ids [P, P+O), values 0/1 (:7-30); LSIM_E_ARG when
+ *    F > 2 (N - O).
+ *  - lsim_pack_patterns: explicit patterns (bits[k*P + i] = PI i of pattern k)
+ *    in the "new" writer's layout (:56-94): rounds of bits_per_round patterns,
+ *    one decimal word per PI with the round's first pattern most significant,
+ *    then the same chunks in binary. bits_per_round 64 is what the simulator
+ *    reads; 32 is the layout of the reference's 32-bit GPU path.
+ *  - lsim_produce_patterns: n distinct random patterns (n <= 2^P), packed so. */
+LSIM_API int lsim_produce_faults(uint64_t seed, int64_t num_nodes, int64_t num_faults, int64_t num_pis,
+                                 int64_t num_pos, char **flst, size_t *flst_len);
+LSIM_API int lsim_pack_patterns(int64_t num_pis, int64_t num_patterns, int bits_per_round, const uint8_t *bits,
+                                char **ptn, size_t *ptn_len);
+LSIM_API int lsim_produce_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, int bits_per_round,
+                                   char **ptn, size_t *ptn_len);
 /* The fault list or the patterns alone (same seeds and rules as lsim_generate):
  * SA0/SA1 over inner gate ids [P+O, P+O+G) thinned to `fraction`, or
  * ceil(n/64) rounds of P random words. */

@@ -168,6 +168,12 @@ def lib() -> ctypes.CDLL:
                                                   ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_convert_timing_dump": (ctypes.c_int, [ctypes.c_char_p, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                                     ctypes.POINTER(sz), ctypes.POINTER(IngestStats)]),
+        "lsim_produce_faults": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
+        "lsim_pack_patterns": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
+        "lsim_produce_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_probe_random_reads": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                                    ctypes.POINTER(ctypes.c_double)]),
     }
@@ -420,6 +426,33 @@ def generate_patterns(seed: int, num_pis: int, num_patterns: int) -> bytes:
     """Seeded .ptn text with `num_patterns` random patterns."""
     b, n = ctypes.c_void_p(), ctypes.c_size_t()
     rc = lib().lsim_generate_patterns(seed, num_pis, num_patterns, ctypes.byref(b), ctypes.byref(n))
+    return _take_text(rc, b, n)
+
+
+def produce_faults(seed: int, num_nodes: int, num_faults: int, num_pis: int, num_pos: int) -> bytes:
+    """.flst of distinct random (wire, value) pairs, PO ids excluded
+    (fsim/generate_fault_patterns.py:7-30)."""
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    rc = lib().lsim_produce_faults(seed, num_nodes, num_faults, num_pis, num_pos, ctypes.byref(b), ctypes.byref(n))
+    return _take_text(rc, b, n)
+
+
+def pack_patterns(patterns, bits_per_round: int = 64) -> bytes:
+    """.ptn of explicit patterns (array [n][P] of 0/1) in the layout of
+    fsim/generate_fault_patterns.py:56-94 (first pattern most significant)."""
+    a = np.ascontiguousarray(np.asarray(patterns, dtype=np.uint8))
+    if a.ndim != 2:
+        raise ValueError("patterns must be a 2-D array [n][P]")
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    rc = lib().lsim_pack_patterns(a.shape[1], a.shape[0], bits_per_round, a.ctypes.data_as(ctypes.c_void_p),
+                                  ctypes.byref(b), ctypes.byref(n))
+    return _take_text(rc, b, n)
+
+
+def produce_patterns(seed: int, num_pis: int, num_patterns: int, bits_per_round: int = 64) -> bytes:
+    """.ptn of distinct random patterns packed as pack_patterns does."""
+    b, n = ctypes.c_void_p(), ctypes.c_size_t()
+    rc = lib().lsim_produce_patterns(seed, num_pis, num_patterns, bits_per_round, ctypes.byref(b), ctypes.byref(n))
     return _take_text(rc, b, n)
 
 

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "liblsim.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["engine.hip", "probe.hip", "netlist.cpp", "generator.cpp", "loops.cpp", "ingest.cpp", "capi.cpp"]
+SOURCES = ["engine.hip", "probe.hip", "netlist.cpp", "generator.cpp", "loops.cpp", "ingest.cpp", "producers.cpp", "capi.cpp"]
 
 COMMON = [
     "-O3",

@@ -18,6 +18,7 @@
 #include "generator.hpp"
 #include "ingest.hpp"
 #include "loops.hpp"
+#include "producers.hpp"
 #include "probe.hpp"
 #include "lsim.h"
 #include "netlist.hpp"
@@ -1131,6 +1132,36 @@ int lsim_convert_timing_dump(const char *dump, size_t dump_len, int mode, char *
     stats->unreached = st.unreached;
   }
   return emit_text(text, ckt, ckt_len);
+}
+
+int lsim_produce_faults(uint64_t seed, int64_t num_nodes, int64_t num_faults, int64_t num_pis, int64_t num_pos,
+                        char **flst, size_t *flst_len) {
+  if (!flst || !flst_len) return LSIM_E_ARG;
+  *flst = nullptr;
+  std::string text;
+  int rc = guarded(&g_create_error,
+                   [&] { text = lsim::produce_faults(seed, num_nodes, num_faults, num_pis, num_pos); });
+  return rc ? rc : emit_text(text, flst, flst_len);
+}
+
+int lsim_pack_patterns(int64_t num_pis, int64_t num_patterns, int bits_per_round, const uint8_t *bits, char **ptn,
+                       size_t *ptn_len) {
+  if (!ptn || !ptn_len || (!bits && num_pis > 0 && num_patterns > 0)) return LSIM_E_ARG;
+  *ptn = nullptr;
+  std::string text;
+  int rc = guarded(&g_create_error,
+                   [&] { text = lsim::pack_patterns(num_pis, num_patterns, bits_per_round, bits); });
+  return rc ? rc : emit_text(text, ptn, ptn_len);
+}
+
+int lsim_produce_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, int bits_per_round, char **ptn,
+                          size_t *ptn_len) {
+  if (!ptn || !ptn_len) return LSIM_E_ARG;
+  *ptn = nullptr;
+  std::string text;
+  int rc = guarded(&g_create_error,
+                   [&] { text = lsim::produce_patterns(seed, num_pis, num_patterns, bits_per_round); });
+  return rc ? rc : emit_text(text, ptn, ptn_len);
 }
 
 int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn, size_t *ptn_len) {
