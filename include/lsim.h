@@ -124,6 +124,16 @@ LSIM_API int lsim_levels(const lsim_sim *s, uint64_t *out, size_t n);
  * caller frees with lsim_free_text. */
 LSIM_API int lsim_visualization_graph(const lsim_sim *s, const char *path);
 LSIM_API int lsim_visualization_text(const lsim_sim *s, char **text, size_t *len);
+/* Structural diagnostics per level, over groups of `wave` consecutive nodes of
+ * the level in topological order (the idea of the reference's
+ * fsim/cuda_MA_partition/cuda_MA_partition.cu:227-346, at wave64 by default):
+ * nodes[l]; avg_dist[l] = mean over the level's nodes with fanins of the mean
+ * level distance to their fanins; max_span[l] = largest such distance;
+ * avg_range[l] = mean over groups of (largest - smallest fanin position);
+ * avg_unique[l] = mean over groups of the number of distinct fanin positions.
+ * Arrays hold n_levels = max level + 1 entries. Needs no device. */
+LSIM_API int lsim_level_diagnostics(const lsim_sim *s, int wave, int64_t *nodes, double *avg_dist, int64_t *max_span,
+                                    double *avg_range, double *avg_unique, size_t n_levels);
 /* Simulator::faults() (fsim/simulation.hpp:70): site position and stuck value
  * as the reference stores it (0 or all ones); n = number of faults. */
 LSIM_API int lsim_faults(const lsim_sim *s, uint64_t *site, uint64_t *value, size_t n);

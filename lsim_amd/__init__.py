@@ -110,6 +110,9 @@ def lib() -> ctypes.CDLL:
         "lsim_num_gates_per_level": (ctypes.c_int, [P, u64p, sz]),
         "lsim_levels": (ctypes.c_int, [P, u64p, sz]),
         "lsim_visualization_graph": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "lsim_level_diagnostics": (ctypes.c_int, [P, ctypes.c_int, i64p, ctypes.POINTER(ctypes.c_double), i64p,
+                                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                  sz]),
         "lsim_visualization_text": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_faults": (ctypes.c_int, [P, u64p, u64p, sz]),
         "lsim_patterns": (ctypes.c_int, [P, u64p, sz]),
@@ -304,6 +307,19 @@ class Simulator:
         """Graphviz digraph of the circuit written to `path`
         (fsim/simulation.cpp:1157-1225)."""
         self._check(self._L.lsim_visualization_graph(self._h, path.encode()))
+
+    def level_diagnostics(self, wave: int = 64) -> dict:
+        """Per level: node count, mean and largest fanin level distance, and per
+        group of `wave` consecutive nodes the fanin position range and the
+        number of distinct fanins (means over the level's groups)."""
+        L = int(self.levels().max()) + 1 if self.num_PIs() + self.num_gates() + self.num_POs() else 0
+        nodes = np.zeros(L, dtype=np.int64)
+        span = np.zeros(L, dtype=np.int64)
+        dist, rng_, uniq = (np.zeros(L, dtype=np.float64) for _ in range(3))
+        self._check(self._L.lsim_level_diagnostics(self._h, wave, _ptr(nodes, ctypes.c_int64), _ptr(dist, ctypes.c_double),
+                                                   _ptr(span, ctypes.c_int64), _ptr(rng_, ctypes.c_double),
+                                                   _ptr(uniq, ctypes.c_double), L))
+        return {"nodes": nodes, "avg_dist": dist, "max_span": span, "avg_range": rng_, "avg_unique": uniq}
 
     def visualization_text(self) -> str:
         b, n = ctypes.c_void_p(), ctypes.c_size_t()

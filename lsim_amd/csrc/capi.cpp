@@ -7,6 +7,7 @@
 #include <deque>
 #include <fstream>
 #include <map>
+#include <unordered_set>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -1066,6 +1067,59 @@ static std::string dot_text(const lsim::Netlist &nl) {
   for (int64_t i = 0; i < nl.O; ++i) o += "\tPO_" + std::to_string(i) + "  [shape=Msquare];\n";
   o += "}\n\n";
   return o;
+}
+
+int lsim_level_diagnostics(const lsim_sim *s, int wave, int64_t *nodes, double *avg_dist, int64_t *max_span,
+                           double *avg_range, double *avg_unique, size_t n_levels) {
+  int rc = check_loaded(s);
+  if (rc) return rc;
+  std::string *err = &const_cast<lsim_sim *>(s)->err;
+  const lsim::Netlist &nl = s->nl;
+  const size_t L = size_t(nl.max_level) + 1;
+  if (wave < 1) return fail(err, LSIM_E_ARG, "wave must be positive");
+  if (!nodes || !avg_dist || !max_span || !avg_range || !avg_unique || n_levels != L)
+    return fail(err, LSIM_E_ARG, "level arrays must hold " + std::to_string(L) + " entries");
+  std::vector<std::vector<int32_t>> by_level(L);
+  for (int64_t p = 0; p < nl.N; ++p) by_level[size_t(nl.level[size_t(p)])].push_back(int32_t(p));
+  std::unordered_set<int32_t> uniq;
+  for (size_t l = 0; l < L; ++l) {
+    const std::vector<int32_t> &v = by_level[l];
+    nodes[l] = int64_t(v.size());
+    double dsum = 0;
+    int64_t span = 0;
+    for (int32_t p : v) {
+      const int32_t a = nl.fin_off[size_t(p)], b = nl.fin_off[size_t(p) + 1];
+      if (a == b) continue;
+      double d = 0;
+      for (int32_t e = a; e < b; ++e) {
+        const int64_t dist = int64_t(l) - nl.level[size_t(nl.fin[size_t(e)])];
+        d += double(dist);
+        span = std::max(span, dist);
+      }
+      dsum += d / double(b - a);
+    }
+    avg_dist[l] = v.empty() ? 0.0 : dsum / double(v.size());
+    max_span[l] = span;
+    double rsum = 0, usum = 0;
+    int64_t groups = 0;
+    for (size_t i = 0; i < v.size(); i += size_t(wave)) {
+      int32_t lo = INT32_MAX, hi = 0;
+      uniq.clear();
+      for (size_t j = i; j < std::min(v.size(), i + size_t(wave)); ++j)
+        for (int32_t e = nl.fin_off[size_t(v[j])]; e < nl.fin_off[size_t(v[j]) + 1]; ++e) {
+          const int32_t q = nl.fin[size_t(e)];
+          lo = std::min(lo, q);
+          hi = std::max(hi, q);
+          uniq.insert(q);
+        }
+      if (lo != INT32_MAX) rsum += double(hi - lo);
+      usum += double(uniq.size());
+      ++groups;
+    }
+    avg_range[l] = groups ? rsum / double(groups) : 0.0;
+    avg_unique[l] = groups ? usum / double(groups) : 0.0;
+  }
+  return LSIM_OK;
 }
 
 int lsim_visualization_text(const lsim_sim *s, char **text, size_t *len) {
