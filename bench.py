@@ -172,6 +172,8 @@ def bench_batch(args) -> int:
                            pair_threads=args.pair_threads)
     if args.pair_dense >= 0:
         batch.set_option("pair_dense", args.pair_dense)
+    if args.pair_nopend >= 0:
+        batch.set_option("pair_nopend", args.pair_nopend)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
@@ -360,6 +362,9 @@ def main() -> int:
     ap.add_argument("--pair-dense", type=int, default=-1,
                     help="pair sweep: passes whose nodes average this many active lanes go one node per wave "
                          "(0: never; -1: the engine default)")
+    ap.add_argument("--pair-nopend", type=int, default=-1,
+                    help="pair sweep without pending flags: 0 never, 1 dense steps, 2 every step (-1: the engine "
+                         "default)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
@@ -408,6 +413,8 @@ def main() -> int:
     sim.set_option("pair_threads", args.pair_threads)
     if args.pair_dense >= 0:
         sim.set_option("pair_dense", args.pair_dense)
+    if args.pair_nopend >= 0:
+        sim.set_option("pair_nopend", args.pair_nopend)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")

@@ -81,6 +81,10 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *  "pair_dense"         pair sweep: a level pass whose staged nodes average at
  *                            least this many active lanes evaluates one node per
  *                            wave (default 32; 0 = always flattened pairs).
+ *  "pair_nopend"        pair sweep without pending flags (every node of a walked
+ *                            level is a candidate): 0 never, 1 dense steps (the
+ *                            first step of a dropping job, full-matrix steps;
+ *                            default), 2 every step.
  *  "pool_cap"           pair sweep: at most this many pool words per slot
  *                            (default 0 = as memory allows); a tile that needs
  *                            more is swept again in a slot that holds any tile.

@@ -155,6 +155,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
     eo.pair_threads = int(value);
   } else if (k == "pool_cap") {
     eo.pool_cap = value < 0 ? 0 : value;
+  } else if (k == "pair_nopend") {
+    if (value < 0 || value > 2) return fail(err, LSIM_E_ARG, "pair_nopend must be 0, 1 or 2");
+    eo.pair_nopend = int(value);
   } else if (k == "pair_dense") {
     if (value < 0 || value > 64) return fail(err, LSIM_E_ARG, "pair_dense must be 0..64");
     eo.pair_dense = int(value);

@@ -779,6 +779,7 @@ struct PairArgs {
   int *ovf_list;
   uint32_t epoch_base;
   int dense_lanes;             // a pass whose nodes average this many active lanes goes one node per wave (0: never)
+  int nopend;                  // no pending flags: every node of a walked level is a candidate
   unsigned long long *tstats;  // as SweepArgs::tstats, or null
 };
 
@@ -832,7 +833,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       s_lmask[lane] = L.mask;
       s_acc[lane] = 0;
       if (valid) {
-        pend[site] = epoch;
+        if (!a.nopend) pend[site] = epoch;
         s_lvl[L.level] = 1;
       }
     }
@@ -845,7 +846,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     bool aborted = false;
     for (int l = T.start_level; l <= s_maxl && !aborted; ++l) {
       l_end = l;
-      if (s_lvl[l] == 0) continue;
+      if (!a.nopend && s_lvl[l] == 0) continue;
       ++n_lv;
       const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += CHUNK) {
@@ -859,17 +860,19 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         int4 f = make_int4(0, 0, 0, 0);
         int fo0 = 0, fo1 = 0;
         if (in) {
-          pf = pend[u];
+          if (!a.nopend) pf = pend[u];
           inf = J.info[u];
           f = J.fin[u];
           fo0 = J.fo_off[u];
           fo1 = J.fo_off[u + 1];
         }
-        const bool p = in && pf == epoch;
+        // Without pending flags every node of the level is a candidate and is
+        // staged when some fanin differs in some lane or it injects.
+        const bool cand = in && (a.nopend || pf == epoch);
         uint64_t act = 0;
         uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
         uint32_t fb[4] = {0, 0, 0, 0};
-        if (p) {
+        if (cand) {
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
           if (type != G_PI) {
             const int fi[4] = {f.x, f.y, f.z, f.w};
@@ -887,7 +890,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           act = m[0] | m[1] | m[2] | m[3] | inj_lanes(s_lsite, u);
           bytes += 20 + (type != G_PI ? 24 * nin : 0);
         }
-        bytes += 4;
+        if (!a.nopend) bytes += 4;
+        const bool p = cand && (!a.nopend || act != 0);
         const uint64_t slot = wave_claim(p ? ((uint64_t(1) << 32) | uint64_t(__popcll(act))) : 0, lane, &s_cnt2);
         if (p) {
           const int k = int(slot >> 32);
@@ -1070,14 +1074,14 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             if (kStats) ++n_written;
           }
         }
-        int incl = deg;
+        int incl = a.nopend ? 0 : deg;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
           const int y = __shfl_up(incl, o);
           if (lane >= o) incl += y;
         }
-        const int excl = incl - deg;
-        const int Ew = __shfl(incl, 63);
+        const int excl = incl - (a.nopend ? 0 : deg);
+        const int Ew = __shfl(incl, 63);  // 0 without pending flags: no fanout is marked
         // D. kFanoutsPerLane fanouts per lane at a time, loads interleaved.
         // The fanout e belongs to the last lane whose prefix is <= e.
         if (tid == 0) s_pool = pb + uint32_t(P);
@@ -1104,7 +1108,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               bytes += 12;
             }
         }
-        if (kStats) n_staged += n;
+        n_staged += n;
         __syncthreads();
       }
       if (tid == 0 && !aborted) s_lvl[l] = 0;
@@ -1116,8 +1120,12 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         s_cnt2 = 0;
         a.ovf_list[atomicAdd(a.ovf_count, 1)] = t;
       }
-    } else if (tid < T.nlanes) {
-      a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+    } else {
+      if (tid < T.nlanes) a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+      if (tid == 0) {
+        atomicAdd(&a.bytes[1], (unsigned long long)n_staged);
+        atomicAdd(&a.bytes[2], (unsigned long long)(J.live_lvl[l_end + 1] - J.live_lvl[T.start_level]));
+      }
     }
     unsigned long long wbytes = bytes;
     for (int o = 32; o > 0; o >>= 1) wbytes += __shfl_xor(wbytes, o);
@@ -1675,7 +1683,9 @@ struct Engine::Impl {
   DevBuf<Tile> d_tiles_raw;
   DevBuf<int> d_tile_key, d_tile_idx;
   DevBuf<uint8_t> d_cub;
-  HostBuf<unsigned long long> h_bytes;
+  HostBuf<unsigned long long> h_bytes;  // sweep bytes; pair sweep staged and walked nodes
+  double staged_share = 0;              // of the last pair sweep launch
+  static constexpr double kNopendShare = 0.45;
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
@@ -1756,8 +1766,9 @@ struct Engine::Impl {
   void finalize_jobs(const std::vector<JobState *> &js);
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
   void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles, const std::vector<Tile> *tiles,
-             int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st);
-  void sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
+             int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, bool nopend,
+             lsim_run_stats &st);
+  void sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense, bool nopend,
                    unsigned long long *tstats, lsim_run_stats &st);
   void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu);
   void ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, int64_t slots_needed, int64_t full_slots);
@@ -1792,8 +1803,8 @@ Engine::Engine(int device) : impl_(new Impl) {
     }
     LSIM_HIP(hipEventCreate(&impl_->ev[0]));
     LSIM_HIP(hipEventCreate(&impl_->ev[1]));
-    impl_->d_bytes.alloc(1);
-    LSIM_HIP(hipMemsetAsync(impl_->d_bytes.p, 0, sizeof(unsigned long long), impl_->stream));
+    impl_->d_bytes.alloc(3);
+    LSIM_HIP(hipMemsetAsync(impl_->d_bytes.p, 0, 3 * sizeof(unsigned long long), impl_->stream));
   } catch (...) {
     delete impl_;
     impl_ = nullptr;
@@ -2553,7 +2564,7 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
 // (`lanes` and `tiles` not null) and copied there first.
 void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles,
                          const std::vector<Tile> *tiles, int lvl_entries, int64_t nl_max,
-                         const CircuitRes *plain_circuit, bool dense, lsim_run_stats &st) {
+                         const CircuitRes *plain_circuit, bool dense, bool nopend, lsim_run_stats &st) {
   if (!ntiles) return;
   if (lanes && tiles) {
     d_lanes.alloc(size_t(nlanes));
@@ -2583,7 +2594,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   if (!plain_circuit && eo.sweep_kernel == 0) {
     st.sweep_tiles += ntiles;
     time_begin();
-    sweep_pairs(eo, ntiles, lvl_entries, nl_max, dense, trace ? d_tstats.p : nullptr, st);
+    sweep_pairs(eo, ntiles, lvl_entries, nl_max, dense, nopend, trace ? d_tstats.p : nullptr, st);
     st.t_sweep_ms += time_end();
     if (trace) print_stats();
     return;
@@ -2659,7 +2670,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
 }
 
 void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
-                               unsigned long long *tstats, lsim_run_stats &st) {
+                               bool nopend, unsigned long long *tstats, lsim_run_stats &st) {
   const size_t lds = size_t(lvl_entries);  // one flag byte per level
   int pc256 = 0, pc512 = 0;
   LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false), 256, lds));
@@ -2705,6 +2716,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   a.ovf_list = d_ovf_list.p;
   a.tstats = tstats;
   a.dense_lanes = eo.pair_dense;
+  a.nopend = nopend ? 1 : 0;
   // A share of the device's resident workgroups (another engine on the same
   // device may hold the rest).
   const double share = std::min(1.0, std::max(0.0, eo.slot_share));
@@ -2943,7 +2955,12 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     }
     h3 = clk::now();
     const bool big_groups = dense || 2 * wide_tiles > ntiles;
-    sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, st);
+    // Dense steps evaluate every node of the levels a tile walks instead of
+    // marking fanouts pending (most nodes are reached anyway): a dropping
+    // job's first step, full-matrix steps, and steps after one that staged
+    // more than kNopendShare of the nodes it walked.
+    const bool nopend = eo.pair_nopend == 2 || (eo.pair_nopend == 1 && (dense || staged_share > kNopendShare));
+    sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, nopend, st);
   } else {
     // The needed bits decide which (unit, word) pairs become sweep lanes.
     h_need.alloc(size_t(words));
@@ -3042,7 +3059,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     for (const Tile &t : tiles) wide_tiles += act[t.job]->c->wide ? 1 : 0;
     ntiles = int64_t(tiles.size());
     const bool big_groups = dense || 2 * wide_tiles > ntiles;
-    sweep(eo, lanes, nlanes, ntiles, &tiles, lvl_entries, nl_max, act[0]->c, big_groups, st);
+    sweep(eo, lanes, nlanes, ntiles, &tiles, lvl_entries, nl_max, act[0]->c, big_groups, false, st);
   }
   const auto h4 = clk::now();
   time_begin();
@@ -3063,11 +3080,14 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     LSIM_HIP(hipGetLastError());
     d2h(h_first_out.p, d_first_out.p, size_t(ng) * sizeof(int32_t));
   }
-  h_bytes.alloc(1);
-  d2h(h_bytes.p, d_bytes.p, sizeof(unsigned long long));
-  LSIM_HIP(hipMemsetAsync(d_bytes.p, 0, sizeof(unsigned long long), stream));
+  h_bytes.alloc(3);
+  d2h(h_bytes.p, d_bytes.p, 3 * sizeof(unsigned long long));
+  LSIM_HIP(hipMemsetAsync(d_bytes.p, 0, 3 * sizeof(unsigned long long), stream));
   st.t_detect_ms += time_end();
   st.sweep_bytes = int64_t(h_bytes.p[0]);
+  // Staged share of the nodes in the levels the pair sweep's tiles walked:
+  // the next step drops pending flags while it is high.
+  if (h_bytes.p[2]) staged_share = double(h_bytes.p[1]) / double(h_bytes.p[2]);
   const auto h5 = clk::now();
   // Advance every job (in parallel: a dropping step rebuilds each job's list
   // of faults still undetected).
@@ -3101,9 +3121,9 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   if (trace) {
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::fprintf(stderr,
-                 "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld | prep %.2f local %.2f "
-                 "lanes %.2f sweep %.2f (dev %.2f) detect %.2f advance %.2f ms\n",
-                 nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)ntiles,
+                 "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld staged share %.3f | prep %.2f "
+                 "local %.2f lanes %.2f sweep %.2f (dev %.2f) detect %.2f advance %.2f ms\n",
+                 nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)ntiles, staged_share,
                  ms(h0, h1), ms(h1, h2), ms(h2, h3), ms(h3, h4), st.t_sweep_ms, ms(h4, h5), ms(h5, clk::now()));
   }
 }

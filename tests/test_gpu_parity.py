@@ -17,12 +17,16 @@ pytestmark = pytest.mark.gpu
 # two sweep kernels (flattened (node, lane) pairs; one node per wave). The pair
 # sweep evaluates dense level passes one node per wave: by default where the
 # nodes average 32 active lanes, and forced never (pairs_flat) or always
-# (pairs_dense).
+# (pairs_dense). Dense steps walk every node of a level instead of pending
+# flags: by default, and forced never (pairs_pend) or in every step
+# (pairs_nopend).
 MODES = [(lsim_amd.Simulator.Mode.PLAIN, {}), (lsim_amd.Simulator.Mode.PARTITIONED, {}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"sweep_kernel": 1}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 0}),
-         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 1})]
-MODE_IDS = ["plain", "pairs", "nodewave", "pairs_flat", "pairs_dense"]
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 1}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 0}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 2})]
+MODE_IDS = ["plain", "pairs", "nodewave", "pairs_flat", "pairs_dense", "pairs_pend", "pairs_nopend"]
 
 
 def _sim(ckt, flst, ptn, mode=None, **opts):

@@ -1,7 +1,7 @@
 """Parity at the 2M-gate size (C3: 2,000,000 gates, 4,000,000 faults,
 32,768 patterns, fault dropping): the two persistent sweep kernels, and the
-pair sweep with and without its one-node-per-wave dense passes, agree on
-every fault, and a seeded fault sample agrees with the event-driven
+pair sweep with and without its one-node-per-wave dense passes and with and
+without pending flags in its dense first step, agree on every fault, and a seeded fault sample agrees with the event-driven
 restatement on the first four rounds (a fault the GPU first detects in round
 g < 4 is undetected before g and detected in g; every other sampled fault is
 undetected in rounds 0-3)."""
@@ -40,6 +40,9 @@ def test_c3_kernels_agree_and_sample_matches_oracle(c3):
     first_fl, table_fl = _run(ckt, flst, ptn, drop=1, pair_dense=0)
     assert np.array_equal(first, first_fl)
     assert np.array_equal(table, table_fl)
+    first_pd, table_pd = _run(ckt, flst, ptn, drop=1, pair_nopend=0)
+    assert np.array_equal(first, first_pd)
+    assert np.array_equal(table, table_pd)
     F = first.size
     assert F == 4_000_000
     # table = {found in the last round, last round} for every fault
