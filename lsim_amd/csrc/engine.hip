@@ -980,6 +980,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
                 if (kStats) ++n_pairs;
               }
               if (d) {
+                // Dense passes write only the differing lanes: most lanes are
+                // active, so the node's block is mostly written anyway.
                 pool[pb + uint32_t(s_pref[kk[i]]) + lanes_below(act[i])] = r[i];
                 if (type[i] == G_PO) atomicOr(&s_acc[lane], (unsigned long long)d);
                 bytes += 8;
@@ -989,7 +991,9 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           }
         } else {
           // B. (node, active lane) pairs, kPairsPerThread per thread at a time
-          // with their loads interleaved; a pair that differs writes its pool word.
+          // with their loads interleaved. Every pair writes its pool word
+          // (readers use only the lanes that differ): the pass's words then
+          // fill one contiguous block, written in whole lines.
   #pragma unroll 1
           for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
             int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread], rk[kPairsPerThread];
@@ -1041,8 +1045,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               }
               const uint64_t d = r[i] ^ gu[i];
               bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
+              pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
               if (d) {
-                pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
                 atomicOr(&s_nm[k[i]], 1ull << b);
                 if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
                 bytes += 8;
