@@ -679,6 +679,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 // the tile's epoch, so nothing is cleared between tiles.
 constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
+constexpr int kDenseNodes = 2;      // pair sweep dense passes: nodes per wave in flight
 
 // Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
@@ -922,7 +923,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           break;
         }
         if (a.dense_lanes > 0 && int64_t(P) >= int64_t(a.dense_lanes) * n) {
-          // B'. Dense pass: one node per wave at a time, two in flight, the
+          // B'. Dense pass: one node per wave at a time, kDenseNodes in flight, the
           // wave's lanes being the tile's lanes. A lane's pool word within a
           // node is the count of the node's active lanes below it (mbcnt), so
           // no pair has to be located by search.
@@ -932,13 +933,14 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           const uint32_t kind_l = uint32_t(s_lkind[lane]);
           const uint64_t mask_l = s_lmask[lane];
 #pragma unroll 1
-          for (int k0 = wave; k0 < n; k0 += 2 * (CHUNK / 64)) {
-            int kk[2], v[2];
-            bool has[2], on[2];
-            uint32_t type[2], nin[2];
-            uint64_t act[2], gu[2], r[2];
+          for (int k0 = wave; k0 < n; k0 += kDenseNodes * (CHUNK / 64)) {
+            int kk[kDenseNodes], v[kDenseNodes];
+            bool has[kDenseNodes], on[kDenseNodes];
+            uint32_t type[kDenseNodes], nin[kDenseNodes];
+            uint64_t act[kDenseNodes], gu[kDenseNodes], r[kDenseNodes], vals[kDenseNodes][4];
+            // Every load of the group's nodes first, then the evaluations.
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < kDenseNodes; ++i) {
               const int ki = k0 + i * (CHUNK / 64);
               has[i] = ki < n;
               kk[i] = has[i] ? ki : k0;
@@ -949,27 +951,23 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               act[i] = s_act[kk[i]];
               on[i] = has[i] && ((act[i] >> lane) & 1);
               gu[i] = on[i] ? good[int64_t(v[i]) * RW + w_l] : 0;
-            }
+              const int4 fv = s_fin[kk[i]];
+              const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-              r[i] = gu[i];
-              if (on[i] && type[i] != G_PI) {
-                const int4 fv = s_fin[kk[i]];
-                const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
-                uint64_t vals[4] = {0, 0, 0, 0};
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                  if (j < int(nin[i])) {
-                    const uint64_t fa = s_fa[kk[i]][j];
-                    vals[j] = ((s_m[kk[i]][j] >> lane) & 1)
-                                  ? pool[s_fb[kk[i]][j] + lanes_below(fa)]
-                                  : good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
-                  }
-                r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
+              for (int j = 0; j < 4; ++j) {
+                vals[i][j] = 0;
+                if (on[i] && type[i] != G_PI && j < int(nin[i]))
+                  vals[i][j] = ((s_m[kk[i]][j] >> lane) & 1)
+                                   ? pool[s_fb[kk[i]][j] + lanes_below(s_fa[kk[i]][j])]
+                                   : good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
               }
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < kDenseNodes; ++i)
+              r[i] = (on[i] && type[i] != G_PI) ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], vals[i][2], vals[i][3])
+                                                : gu[i];
+#pragma unroll
+            for (int i = 0; i < kDenseNodes; ++i) {
               if (!has[i]) continue;
               if (on[i] && site_l == v[i])
                 r[i] = kind_l == K_FLIP ? (gu[i] ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
