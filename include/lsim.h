@@ -80,7 +80,7 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *  "pair_threads"       pair sweep workgroup: 0 by step (default), 256 or 512.
  *  "pair_dense"         pair sweep: a level pass whose staged nodes average at
  *                            least this many active lanes evaluates one node per
- *                            wave (default 32; 0 = always flattened pairs).
+ *                            wave (default 24; 0 = always flattened pairs).
  *  "pair_nopend"        pair sweep without pending flags (every node of a walked
  *                            level is a candidate): 0 never, 1 dense steps (the
  *                            first step of a dropping job, full-matrix steps;

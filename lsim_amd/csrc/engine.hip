@@ -679,7 +679,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 // the tile's epoch, so nothing is cleared between tiles.
 constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
-constexpr int kDenseNodes = 2;      // pair sweep dense passes: nodes per wave in flight
+constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
 
 // Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 # Engines: the per-level form, and the persistent event form with each of its
 # two sweep kernels (flattened (node, lane) pairs; one node per wave). The pair
 # sweep evaluates dense level passes one node per wave: by default where the
-# nodes average 32 active lanes, and forced never (pairs_flat) or always
+# nodes average 24 active lanes, and forced never (pairs_flat) or always
 # (pairs_dense). Dense steps walk every node of a level instead of pending
 # flags: by default, and forced never (pairs_pend) or in every step
 # (pairs_nopend).
