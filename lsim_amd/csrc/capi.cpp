@@ -103,6 +103,7 @@ struct lsim_sim {
   lsim::Netlist nl;
   lsim::FaultStructure fs;
   lsim::CircuitRes *circuit = nullptr;
+  lsim::JobState *job = nullptr;  // the inputs' job, kept resident from run to run
   bool loaded = false;
   bool ran = false;
   lsim::JobOptions jopt;
@@ -119,6 +120,10 @@ int load_texts(lsim_sim *s, const char *ckt, size_t ckt_len, const char *flst, s
     s->loaded = false;
     s->ran = false;
     s->res = lsim::RunResults();
+    if (s->engine && s->job) {
+      s->engine->release_job(s->job);
+      s->job = nullptr;
+    }
     if (s->engine && s->circuit) {
       s->engine->remove_circuit(s->circuit);
       s->circuit = nullptr;
@@ -256,16 +261,23 @@ int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
     eo.mode = m;
     lsim::JobOptions jo = s->jopt;
     jo.keep_good_last = true;
-    lsim::JobState *job = s->engine->add_job(s->circuit, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns,
-                                             s->nl.R, s->nl.patterns, jo);
+    // The fault list and patterns stay resident between runs of the same
+    // inputs; only a change of dropping or last-round reporting rebuilds them.
+    if (s->job && !s->engine->reset_job(s->job, jo)) {
+      s->engine->release_job(s->job);
+      s->job = nullptr;
+    }
+    if (!s->job)
+      s->job = s->engine->add_job(s->circuit, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns, s->nl.R,
+                                  s->nl.patterns, jo);
     try {
-      s->engine->run({job}, eo);
-      s->res = std::move(s->engine->results(job));
+      s->engine->run({s->job}, eo);
+      s->res = std::move(s->engine->results(s->job));
     } catch (...) {
-      s->engine->release_job(job);
+      s->engine->release_job(s->job);
+      s->job = nullptr;
       throw;
     }
-    s->engine->release_job(job);
     s->ran = true;
   });
 }

@@ -2061,6 +2061,36 @@ void Engine::release_job(JobState *j) {
     }
 }
 
+bool Engine::reset_job(JobState *j, const JobOptions &opt) {
+  Impl &m = *impl_;
+  if (opt.drop != j->opt.drop || (opt.drop && opt.report_last_round != j->opt.report_last_round)) return false;
+  LSIM_HIP(hipSetDevice(m.device));
+  AllocStream alloc_on(m.stream);
+  j->opt = opt;
+  j->direct = opt.direct || j->c->has_mux;
+  j->started = j->done = j->last_done = false;
+  j->w0 = 0;
+  j->Wt = std::max(1, std::min(64, opt.first_words));
+  j->Wg = std::max(1, std::min(64, opt.word_growth));
+  j->kind = JobState::MATRIX;
+  j->step_faults.clear();
+  j->units.clear();
+  j->step_w0 = 0;
+  j->step_Wn = 0;
+  j->alive = j->live;
+  j->res = RunResults();
+  j->res.stats.live_nodes = j->c->NL;
+  j->res.stats.live_faults = int64_t(j->live.size());
+  const int64_t F1 = std::max<int64_t>(j->F, 1);
+  k_fill_i32<<<blocks_for(F1, 256), 256, 0, m.stream>>>(j->first.p, F1, INT32_MAX);
+  k_fill_i32<<<blocks_for(int64_t(j->head_slot.n), 256), 256, 0, m.stream>>>(j->head_slot.p, int64_t(j->head_slot.n),
+                                                                              -1);
+  LSIM_HIP(hipGetLastError());
+  if (j->detbits.p) LSIM_HIP(hipMemsetAsync(j->detbits.p, 0, j->detbits.bytes(), m.stream));
+  if (j->last.p) LSIM_HIP(hipMemsetAsync(j->last.p, 0, j->last.bytes(), m.stream));
+  return true;
+}
+
 bool Engine::done(const JobState *j) const { return j->done; }
 RunResults &Engine::results(JobState *j) { return j->res; }
 

@@ -86,6 +86,11 @@ class Engine {
   JobState *add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::vector<uint8_t> fault_sa,
                     int64_t num_patterns, int64_t R, const std::vector<uint64_t> &patterns, const JobOptions &opt);
   void release_job(JobState *j);
+  // Makes a finished (or never run) job runnable again with the same inputs
+  // still resident, under `opt`; false when `opt` needs other device arrays
+  // (dropping or last-round reporting changed), in which case the caller
+  // releases the job and adds a new one.
+  bool reset_job(JobState *j, const JobOptions &opt);
   // Device bytes a job on circuit c would hold (admission by footprint).
   static int64_t job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop);
   int64_t free_bytes() const;

@@ -184,3 +184,30 @@ def test_good_machine_forms_agree(c2, form):
     other.run(lsim_amd.Simulator.Mode.PARTITIONED)
     assert all(np.array_equal(a, b) for a, b in zip(s.good_results(), other.good_results()))
     assert np.array_equal(s.detect_matrix(), other.detect_matrix())
+
+
+def test_repeated_runs_keep_inputs_resident(c1):
+    """run() again on the same inputs reuses the resident fault list and
+    patterns: every run, also after option changes, gives the same answer."""
+    ckt, flst, ptn, ref = c1
+    s = _sim(ckt, flst, ptn)
+    mode = lsim_amd.Simulator.Mode.PARTITIONED
+    s.run(mode)
+    det, table = ref["det"], ref["table"]
+    assert np.array_equal(s.detect_matrix(), det) and np.array_equal(s.found_fault_to_pattern(), table)
+    first = np.where(det.any(1), det.argmax(1), -1)
+    s.run(mode)
+    assert np.array_equal(s.detect_matrix(), det) and np.array_equal(s.found_fault_to_pattern(), table)
+    s.set_option("direct", 1)
+    s.run(mode)
+    assert np.array_equal(s.detect_matrix(), det)
+    s.set_option("drop", 1)
+    s.run(mode)
+    assert np.array_equal(s.first_detect_round(), first)
+    assert np.array_equal(s.found_fault_to_pattern(), table)
+    s.set_option("direct", 0)
+    s.run(mode)
+    assert np.array_equal(s.first_detect_round(), first)
+    s.set_option("drop", 0)
+    s.run(lsim_amd.Simulator.Mode.PLAIN)
+    assert np.array_equal(s.detect_matrix(), det)
