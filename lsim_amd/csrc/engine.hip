@@ -1760,6 +1760,7 @@ struct Engine::Impl {
     }
   };
   std::map<std::string, TuneState> tuned;
+  uint64_t setup_events = 0;  // sweep scratch (re)allocations: a run that made one is not timed for the tuner
   std::string tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo, bool &any_drop) const;
   TuneState &tune_state(const std::string &key, const EngineOptions &eo, bool any_drop);
   bool plan(JobState &j);
@@ -2538,6 +2539,7 @@ void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, 
                                         std::to_string(nl) + "-node circuit");
   const int64_t ns = std::max<int64_t>(1, std::min(slots_needed, fit));
   const int64_t pool_words = std::min((budget - ns * per_slot) / 8, ns * nl * 64);
+  ++setup_events;
   const auto t0 = std::chrono::steady_clock::now();
   d_prec.alloc_direct(size_t(ns * nl));
   d_ppend.alloc_direct(size_t(ns * nl));
@@ -2580,6 +2582,7 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
     d_masks.release();
     d_mrec.release();
     d_pend.release();
+    ++setup_events;
     d_rows.alloc_direct(size_t(ns) * nl * 64);
     d_masks.alloc_direct(size_t(ns) * nl);
     d_mrec.alloc_direct(size_t(ns) * nl);
@@ -3245,10 +3248,13 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
       }
   }
   const auto t1 = clk::now();
+  const uint64_t setups = m.setup_events;
   while (iterate(jobs, e2) > 0) {
   }
   const auto t2 = clk::now();
-  if (ts) {
+  // A run that (re)allocated sweep scratch is not a fair sample: the
+  // candidate stays untried and runs again.
+  if (ts && m.setup_events == setups) {
     const double steps_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     double &slot = ts->ms[size_t(cand)];
     slot = slot < 0 ? steps_ms : std::min(slot, steps_ms);

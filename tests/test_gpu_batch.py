@@ -164,8 +164,9 @@ def test_server_rejects_job_above_budget():
 
 def test_autotuned_batch_matches_single_runs():
     """With the autotuner on, successive batches over the same circuits try
-    every kernel / word-schedule candidate once and then keep the fastest;
-    every batch's results are those of the untuned single runs."""
+    every kernel / word-schedule candidate (a run that allocated sweep
+    scratch is not timed, so that candidate runs again) and then keep the
+    fastest; every batch's results are those of the untuned single runs."""
     b = lsim_amd.Batch(device=0, autotune=1)
     cids, texts = [], []
     for spec in SPECS[:3]:
@@ -174,16 +175,18 @@ def test_autotuned_batch_matches_single_runs():
         texts.append((ckt, flst, ptn))
     want = [_single(ck, f, p, True)["first"] for ck, f, p in texts]
     seen = set()
-    for rnd in range(10):
+    trials = 0
+    for rnd in range(16):
         jobs = [b.add_job(c, f, p, drop=True) for c, (ck, f, p) in zip(cids, texts)]
         b.run()
         st = b.last_run_stats()
         seen.add(st["tuned"])
-        assert st["tune_trials"] == min(rnd + 1, 8)
+        assert trials <= st["tune_trials"] <= trials + 1
+        trials = st["tune_trials"]
         for jid, w in zip(jobs, want):
             assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid)
             b.release_job(jid)
-    assert len(seen) == 8  # 2 kernels x 4 word schedules
+    assert trials == 8 and len(seen) == 8  # 2 kernels x 4 word schedules
 
 
 def test_server_two_lanes_match_single_runs():
