@@ -43,8 +43,8 @@ struct EngineOptions {
   int good_form = 0;                 // good machine of a lone job: 0 one launch per level, 1 level windows
   int pair_threads = 0;              // workgroup size of the pair sweep: 512, 256, or 0 = by step density
   int64_t pool_cap = 0;              // pair sweep: at most this many pool words per slot (0 = as memory allows)
-  int pair_dense = 24;
-  int pair_nopend = 1;               // pair sweep without pending flags: 0 never, 1 dense steps, 2 every step               // pair sweep: passes averaging this many active lanes per node go one node per wave (0: never)
+  int pair_dense = 24;               // pair sweep: passes averaging this many active lanes per node go one node per wave (0: never)
+  int pair_nopend = 1;               // pair sweep without pending flags: 0 never, 1 dense steps, 2 every step
   double slot_share = 1.0;           // pair sweep: use at most this share of the resident workgroup slots
   bool autotune = false;             // run(): pick sweep kernel and word schedule by timed probes (cached)
 };
