@@ -677,6 +677,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 // lanes it has. A node's injecting lanes are found by a binary search
 // over the tile's lane sites in LDS. Pending flags and mask records carry
 // the tile's epoch, so nothing is cleared between tiles.
+constexpr int kTileStats = 13;  // sweep statistics counters (LSIM_TRACE)
 constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
 constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
@@ -845,6 +846,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     __syncthreads();
     int l_end = T.start_level;
     bool aborted = false;
+    // Statistics: cycles of the tile's walk and of each pass phase (thread 0,
+    // which waits at every barrier): A scan and stage, B pairs (flattened or
+    // dense), C-D records and fanout marking.
+    uint64_t cy0 = kStats ? clock64() : 0, cy_a = 0, cy_bp = 0, cy_bd = 0, cy_cd = 0, cy_t = 0;
+    bool dense_pass = false;
     for (int l = T.start_level; l <= s_maxl && !aborted; ++l) {
       l_end = l;
       if (!a.nopend && s_lvl[l] == 0) continue;
@@ -852,6 +858,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
       for (int c0 = lo; c0 < hi; c0 += CHUNK) {
         ++n_pass;
+        if (kStats) cy_t = clock64();
         // A. scan and stage. The node's static data (type, fanins, fanout
         // range) is loaded together with its pending flag, so a pending node
         // costs two dependent loads: these, then its fanins' records.
@@ -912,6 +919,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           s_deg[k] = fo1 - fo0;
         }
         __syncthreads();
+        if (kStats) {
+          const uint64_t c = clock64();
+          cy_a += c - cy_t;
+          cy_t = c;
+        }
         const uint64_t c2 = s_cnt2;
         const int n = int(c2 >> 32), P = int(uint32_t(c2));
         const uint32_t pb = s_pool;
@@ -922,7 +934,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           aborted = true;
           break;
         }
-        if (a.dense_lanes > 0 && int64_t(P) >= int64_t(a.dense_lanes) * n) {
+        dense_pass = a.dense_lanes > 0 && int64_t(P) >= int64_t(a.dense_lanes) * n;
+        if (dense_pass) {
           // B'. Dense pass: one node per wave at a time, kDenseNodes in flight, the
           // wave's lanes being the tile's lanes. A lane's pool word within a
           // node is the count of the node's active lanes below it (mbcnt), so
@@ -1054,6 +1067,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           }
         }
         __syncthreads();
+        if (kStats) {
+          const uint64_t c = clock64();
+          (dense_pass ? cy_bd : cy_bp) += c - cy_t;
+          cy_t = c;
+        }
         // C. nodes that differ somewhere write their record; each wave then
         // marks the live fanouts of its own written nodes pending.
         if (tid == 0) s_cnt2 = 0;
@@ -1112,6 +1130,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         }
         n_staged += n;
         __syncthreads();
+        if (kStats) cy_cd += clock64() - cy_t;
       }
       if (tid == 0 && !aborted) s_lvl[l] = 0;
     }
@@ -1151,6 +1170,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         atomicAdd(&a.tstats[4], s_nwt);
         atomicMax(&a.tstats[5], s_nwt);
         atomicAdd(&a.tstats[6], (unsigned long long)(l_end - T.start_level + 1));
+        atomicAdd(&a.tstats[8], (unsigned long long)(clock64() - cy0));
+        atomicAdd(&a.tstats[9], (unsigned long long)cy_a);
+        atomicAdd(&a.tstats[10], (unsigned long long)cy_bp);
+        atomicAdd(&a.tstats[11], (unsigned long long)cy_bd);
+        atomicAdd(&a.tstats[12], (unsigned long long)cy_cd);
         s_nwt = 0;
       }
     }
@@ -1688,6 +1712,12 @@ struct Engine::Impl {
   HostBuf<unsigned long long> h_bytes;  // sweep bytes; pair sweep staged and walked nodes
   double staged_share = 0;              // of the last pair sweep launch
   static constexpr double kNopendShare = 0.45;
+  // Steps after one that staged a middle share of the nodes its tiles walked
+  // run in 256-thread groups even on wide levels: twice the tiles in flight
+  // outweigh the second scan pass per level there (on the 2M-gate circuit,
+  // 13-16 ms less in each of the three word steps after shares 0.69, 0.53,
+  // 0.37; more at 0.83 and at 0.25).
+  static constexpr double kMidShareLo = 0.3, kMidShareHi = 0.75;
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
@@ -2612,14 +2642,22 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   const size_t lds = size_t(lvl_entries) * sizeof(int);
   const bool trace = trace_on();
   if (trace) {
-    d_tstats.alloc(8);
-    LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, 8 * sizeof(unsigned long long), stream));
+    d_tstats.alloc(kTileStats);
+    LSIM_HIP(hipMemsetAsync(d_tstats.p, 0, kTileStats * sizeof(unsigned long long), stream));
   }
   auto print_stats = [&] {
-    unsigned long long ts[8];
+    unsigned long long ts[kTileStats];
     d2h(ts, d_tstats.p, sizeof(ts));
     LSIM_HIP(hipStreamSynchronize(stream));
     const double nt = double(std::max<unsigned long long>(ts[0], 1));
+    if (ts[8]) {
+      const double cy = double(ts[8]);
+      std::fprintf(stderr,
+                   "[lsim] pair sweep cycles per tile %.0f: scan and stage %.1f %%, pairs %.1f %%, dense pairs %.1f %%, "
+                   "records and fanouts %.1f %%, between passes %.1f %%\n",
+                   cy / nt, 100.0 * ts[9] / cy, 100.0 * ts[10] / cy, 100.0 * ts[11] / cy, 100.0 * ts[12] / cy,
+                   100.0 * (cy - double(ts[9] + ts[10] + ts[11] + ts[12])) / cy);
+    }
     std::fprintf(stderr,
                  "[lsim] tiles %llu: per tile levels walked %.1f, pending levels %.1f, scan passes %.1f, staged %.1f, "
                  "written %.1f (max %llu), active lanes per staged node %.2f\n",
@@ -2989,7 +3027,8 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       LSIM_HIP(hipGetLastError());
     }
     h3 = clk::now();
-    const bool big_groups = dense || 2 * wide_tiles > ntiles;
+    const bool mid = staged_share >= kMidShareLo && staged_share <= kMidShareHi;
+    const bool big_groups = dense || (2 * wide_tiles > ntiles && !mid);
     // Dense steps evaluate every node of the levels a tile walks instead of
     // marking fanouts pending (most nodes are reached anyway): a dropping
     // job's first step, full-matrix steps, and steps after one that staged
