@@ -174,6 +174,8 @@ def bench_batch(args) -> int:
         batch.set_option("pair_dense", args.pair_dense)
     if args.pair_nopend >= 0:
         batch.set_option("pair_nopend", args.pair_nopend)
+    if args.max_step_words >= 0:
+        batch.set_option("max_step_words", args.max_step_words)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
@@ -365,6 +367,8 @@ def main() -> int:
     ap.add_argument("--pair-nopend", type=int, default=-1,
                     help="pair sweep without pending flags: 0 never, 1 dense steps, 2 every step (-1: the engine "
                          "default)")
+    ap.add_argument("--max-step-words", type=int, default=-1,
+                    help="with fault dropping: most words of one step (-1: the engine default)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
@@ -415,6 +419,8 @@ def main() -> int:
         sim.set_option("pair_dense", args.pair_dense)
     if args.pair_nopend >= 0:
         sim.set_option("pair_nopend", args.pair_nopend)
+    if args.max_step_words >= 0:
+        sim.set_option("max_step_words", args.max_step_words)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")

@@ -71,6 +71,12 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *  "report_last_round"  0/1  with dropping on, also evaluate every fault in the
  *                            last round so found_fault_to_pattern stays exact
  *                            (default 1).
+ *  "first_words"        with dropping: words (64 patterns each) of the first
+ *                            step, 1..64 (default 1).
+ *  "word_growth"        with dropping: factor of words per step after each
+ *                            step, 1..64 (default 2).
+ *  "max_step_words"     with dropping: the growth stops at this many words per
+ *                            step, 1..4096 (default 128).
  *  "scratch_mb"         MiB of device memory the fault sweeps may use for
  *                            per-tile state (default 0 = automatic).
  *  "direct"             0/1  inject every fault directly instead of stem

@@ -163,6 +163,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "pair_nopend") {
     if (value < 0 || value > 2) return fail(err, LSIM_E_ARG, "pair_nopend must be 0, 1 or 2");
     eo.pair_nopend = int(value);
+  } else if (k == "max_step_words") {
+    if (value < 1 || value > 4096) return fail(err, LSIM_E_ARG, "max_step_words must be 1..4096");
+    eo.max_step_words = int(value);
   } else if (k == "pair_dense") {
     if (value < 0 || value > 64) return fail(err, LSIM_E_ARG, "pair_dense must be 0..64");
     eo.pair_dense = int(value);
@@ -239,7 +242,10 @@ int lsim_set_option(lsim_sim *s, const char *key, int64_t value) {
   if (k == "drop") s->jopt.drop = value != 0;
   else if (k == "report_last_round") s->jopt.report_last_round = value != 0;
   else if (k == "direct") s->jopt.direct = value != 0;
-  else return fail(&s->err, LSIM_E_ARG, "unknown option " + k);
+  else if (k == "first_words" || k == "word_growth") {
+    if (value < 1 || value > 64) return fail(&s->err, LSIM_E_ARG, k + " must be 1..64");
+    (k == "first_words" ? s->jopt.first_words : s->jopt.word_growth) = int(value);
+  } else return fail(&s->err, LSIM_E_ARG, "unknown option " + k);
   return LSIM_OK;
 }
 

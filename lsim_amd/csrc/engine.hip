@@ -3186,7 +3186,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
           if (fr[k] == INT32_MAX) next.push_back(j.step_faults[k]);
         j.alive.swap(next);
         j.w0 += j.step_Wn;
-        j.Wt = std::min(64, j.Wt * j.Wg);
+        j.Wt = std::min(std::max(1, eo.max_step_words), j.Wt * j.Wg);
         break;
       }
       case JobState::LAST: j.last_done = true; break;

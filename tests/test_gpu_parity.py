@@ -127,6 +127,15 @@ def test_c2_sampled_against_oracle(c2):
     assert np.array_equal(runs["plain"], runs["pairs"])
     assert np.array_equal(runs["plain"], runs["nodewave"])
     assert np.array_equal(runs["plain"], runs["direct"])
+    # With dropping, in steps of 32 then 96 words (a stem's words of one step
+    # span two tiles), and in the default steps: the matrix's first rounds.
+    det = runs["plain"]
+    want = np.where(det.any(1), det.argmax(1), -1)
+    for opts in ({"first_words": 32, "word_growth": 4, "max_step_words": 128}, {}):
+        s = _sim(ckt, flst, ptn, MODES[1], drop=1, **opts)
+        s.run(MODES[1][0])
+        assert np.array_equal(s.first_detect_round(), want), opts
+        s.close()
     o = oracle.Oracle(ckt, flst, ptn)
     rng = np.random.default_rng(2002)
     sample = np.sort(rng.choice(o.F, size=160, replace=False))
