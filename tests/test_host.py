@@ -151,6 +151,30 @@ def test_accessor_size_checks():
     assert e.value.code == lsim_amd.LSIM_E_STATE
 
 
+@pytest.mark.parametrize("key,good,bad", [
+    ("first_words", [1, 64], [0, 65]),
+    ("word_growth", [1, 64], [0, 65]),
+    ("max_step_words", [1, 128, 4096], [0, 4097]),
+    ("pair_nopend", [0, 1, 2], [-1, 3]),
+    ("pair_dense", [0, 24, 64], [-1, 65]),
+    ("pair_threads", [0, 256, 512], [128, 1024]),
+])
+def test_option_ranges(key, good, bad):
+    """Options are checked on the host: values in range are taken, others
+    fail with LSIM_E_ARG and leave the handle usable."""
+    s = lsim_amd.Simulator(device=-1)
+    for v in good:
+        s.set_option(key, v)
+    for v in bad:
+        with pytest.raises(lsim_amd.LsimError) as e:
+            s.set_option(key, v)
+        assert e.value.code == lsim_amd.LSIM_E_ARG
+    s.set_option(key, good[0])
+    with pytest.raises(lsim_amd.LsimError) as e:
+        s.set_option("no_such_option", 1)
+    assert e.value.code == lsim_amd.LSIM_E_ARG
+
+
 def test_stem_regions_on_generated_circuit():
     """Every live node's fanout-free path ends at a PO or at a node with two or
     more live fanouts; dead nodes reach no PO. Checked by brute force."""
