@@ -1715,9 +1715,9 @@ struct Engine::Impl {
   // Steps after one that staged a middle share of the nodes its tiles walked
   // run in 256-thread groups even on wide levels: twice the tiles in flight
   // outweigh the second scan pass per level there (on the 2M-gate circuit,
-  // 13-16 ms less in each of the three word steps after shares 0.69, 0.53,
-  // 0.37; more at 0.83 and at 0.25).
-  static constexpr double kMidShareLo = 0.3, kMidShareHi = 0.75;
+  // 9-16 ms less in each of the four word steps after shares 0.69, 0.53,
+  // 0.37 and 0.25; more at 0.83 and at 0.11, the same at 0.17).
+  static constexpr double kMidShareLo = 0.2, kMidShareHi = 0.75;
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
