@@ -2440,6 +2440,10 @@ bool Engine::Impl::plan(JobState &j) {
     j.step_faults = j.alive;
     j.step_w0 = j.w0;
     j.step_Wn = int(std::min<int64_t>(j.Wt, j.R - j.w0));
+    // A remainder of at most a quarter of this step joins it: a launch for a
+    // few last words would cost a whole walk of the deepest tiles.
+    const int64_t rest = j.R - j.w0 - j.step_Wn;
+    if (rest > 0 && 4 * rest <= j.step_Wn) j.step_Wn = int(j.R - j.w0);
     return true;
   }
   if (j.opt.report_last_round && !j.last_done) {
