@@ -369,6 +369,10 @@ def main() -> int:
                          "default)")
     ap.add_argument("--max-step-words", type=int, default=-1,
                     help="with fault dropping: most words of one step (-1: the engine default)")
+    ap.add_argument("--first-words", type=int, default=0,
+                    help="with fault dropping: words of the first step (0: the engine default)")
+    ap.add_argument("--word-growth", type=int, default=0,
+                    help="with fault dropping: factor of words per step (0: the engine default)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
@@ -421,6 +425,10 @@ def main() -> int:
         sim.set_option("pair_nopend", args.pair_nopend)
     if args.max_step_words >= 0:
         sim.set_option("max_step_words", args.max_step_words)
+    if args.first_words > 0:
+        sim.set_option("first_words", args.first_words)
+    if args.word_growth > 0:
+        sim.set_option("word_growth", args.word_growth)
     t0 = time.perf_counter()
     sim.read_text(ckt, flst, ptn)
     log(f"[bench] read + upload in {time.perf_counter() - t0:.1f}s")
