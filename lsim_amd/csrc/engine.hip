@@ -1540,6 +1540,24 @@ void parallel_for(const std::vector<int> &order, Fn &&fn) {
   if (err) std::rethrow_exception(err);
 }
 
+// Chunks for splitting [0, n) over host threads: 1 below 128k items, else
+// one per 64k items, at most 16.
+int chunk_count(int64_t n, bool allow) {
+  if (!allow) return 1;
+  const int hw = int(std::max(1u, std::thread::hardware_concurrency()));
+  return int(std::max<int64_t>(1, std::min<int64_t>(std::min(16, hw), n / 65536)));
+}
+
+// fn(chunk, begin, end) over `chunks` contiguous pieces of [0, n), in
+// parallel (the same pieces for the same n and chunk count).
+template <typename Fn>
+void run_chunks(int chunks, int64_t n, Fn &&fn) {
+  std::vector<int> ids(size_t(std::max(1, chunks)));
+  std::iota(ids.begin(), ids.end(), 0);
+  const int64_t C = int64_t(ids.size());
+  parallel_for(ids, [&](int c) { fn(c, n * c / C, n * (c + 1) / C); });
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host state
@@ -2877,13 +2895,31 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       j.units.resize(j.step_faults.size());
       for (size_t k = 0; k < j.step_faults.size(); ++k) j.units[k] = j.site[j.step_faults[k]];
     } else {
+      // Stems of the step's faults in position order: marked, counted per
+      // chunk of positions, then listed (a lone job splits both over host
+      // threads).
       std::vector<uint8_t> mark(size_t(c.NL), 0);
-      for (int32_t f : j.step_faults) {
-        const int32_t h = j.head[f];
-        if (!c.head_is_po[h]) mark[h] = 1;
-      }
-      for (int64_t u = 0; u < c.NL; ++u)
-        if (mark[u]) j.units.push_back(int32_t(u));
+      const int64_t nf = int64_t(j.step_faults.size());
+      run_chunks(chunk_count(nf, nj == 1), nf, [&](int, int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) {
+          const int32_t h = j.head[j.step_faults[size_t(k)]];
+          if (!c.head_is_po[h]) __atomic_store_n(&mark[size_t(h)], uint8_t(1), __ATOMIC_RELAXED);
+        }
+      });
+      const int cu = chunk_count(c.NL, nj == 1);
+      std::vector<int64_t> cnt(size_t(cu) + 1, 0);
+      run_chunks(cu, c.NL, [&](int ch, int64_t a, int64_t b) {
+        int64_t m = 0;
+        for (int64_t u = a; u < b; ++u) m += mark[size_t(u)];
+        cnt[size_t(ch) + 1] = m;
+      });
+      for (int ch = 0; ch < cu; ++ch) cnt[size_t(ch) + 1] += cnt[size_t(ch)];
+      j.units.resize(size_t(cnt[size_t(cu)]));
+      run_chunks(cu, c.NL, [&](int ch, int64_t a, int64_t b) {
+        int64_t o = cnt[size_t(ch)];
+        for (int64_t u = a; u < b; ++u)
+          if (mark[size_t(u)]) j.units[size_t(o++)] = int32_t(u);
+      });
     }
   });
   std::vector<int64_t> unit_off(nj + 1, 0);       // stem units of job i in units_all
@@ -3183,11 +3219,24 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     switch (j.kind) {
       case JobState::MATRIX: j.w0 += j.step_Wn; break;
       case JobState::DROP: {
+        // Faults still undetected, in order (a lone job splits the filter
+        // over host threads).
         const int32_t *fr = h_first_out.p + fault_off[i];
-        std::vector<int32_t> next;
-        next.reserve(j.step_faults.size());
-        for (size_t k = 0; k < j.step_faults.size(); ++k)
-          if (fr[k] == INT32_MAX) next.push_back(j.step_faults[k]);
+        const int64_t nf = int64_t(j.step_faults.size());
+        const int C = chunk_count(nf, nj == 1);
+        std::vector<int64_t> cnt(size_t(C) + 1, 0);
+        run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
+          int64_t m = 0;
+          for (int64_t k = a; k < b; ++k) m += fr[k] == INT32_MAX ? 1 : 0;
+          cnt[size_t(ch) + 1] = m;
+        });
+        for (int ch = 0; ch < C; ++ch) cnt[size_t(ch) + 1] += cnt[size_t(ch)];
+        std::vector<int32_t> next(static_cast<size_t>(cnt[size_t(C)]));
+        run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
+          int64_t o = cnt[size_t(ch)];
+          for (int64_t k = a; k < b; ++k)
+            if (fr[k] == INT32_MAX) next[size_t(o++)] = j.step_faults[size_t(k)];
+        });
         j.alive.swap(next);
         j.w0 += j.step_Wn;
         j.Wt = std::min(std::max(1, eo.max_step_words), j.Wt * j.Wg);
