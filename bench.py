@@ -5,8 +5,12 @@ good machine over every round, then the fault sweeps and detection (with fault
 dropping where the configuration asks for it). Circuit parsing and the upload
 to HBM happen before the timed region.
 
-Prints one JSON line (rank 0). See DESIGN.md for the byte model behind
-`roofline` and the sample behind `cpu_baseline`.
+Prints one JSON line (rank 0). `roofline.achieved` is the sweep kernels' own
+count of the bytes they read and write (lsim_run_stats.sweep_bytes) over the
+HIP-event time of their launches; `roofline.traffic` and the request rate come
+from the committed rocprofv3 summary of the same workload under profiles/.
+`cpu_baseline` times the compiled reference simulator on a seeded sample of
+faults and rounds of the same circuit; its cost is linear in faults x rounds.
 """
 from __future__ import annotations
 
@@ -399,7 +403,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fault set would shard "
-              "across GPUs by fault, see DESIGN.md)", file=sys.stderr)
+              "across GPUs by fault, each GPU holding the circuit)", file=sys.stderr)
         return 2
 
     if args.config == "c4":
