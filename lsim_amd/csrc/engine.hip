@@ -38,9 +38,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <exception>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -1507,8 +1510,106 @@ inline bool trace_on() {
 
 inline unsigned blocks_for(int64_t threads, int bs) { return unsigned((threads + bs - 1) / bs); }
 
+// Host worker threads kept for the process's life, so that a parallel region
+// costs a wake-up instead of starting and joining threads. One region runs at
+// a time; the caller works too. A region is finished only when every worker
+// has taken part in it (each worker sees each region once), so no worker can
+// read a finished region's function.
+thread_local bool tl_in_region = false;  // a pool worker, or a caller inside its region
+
+class HostPool {
+ public:
+  explicit HostPool(int workers) : workers_(workers) {
+    for (int i = 0; i < workers_; ++i) th_.emplace_back([this] { loop(); });
+  }
+  int workers() const { return workers_; }
+  // Runs f(k) for k in [0, n) on the caller and every worker; false (nothing
+  // run) when another region holds the pool or the caller is inside one.
+  bool run(int n, const std::function<void(int)> &f) {
+    if (tl_in_region) return false;
+    std::unique_lock<std::mutex> region(busy_, std::try_to_lock);
+    if (!region.owns_lock()) return false;
+    struct InRegion {
+      InRegion() { tl_in_region = true; }
+      ~InRegion() { tl_in_region = false; }
+    } in_region;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      fn_ = &f;
+      n_ = n;
+      next_.store(0);
+      finished_ = 0;
+      failed_.store(false);
+      err_ = nullptr;
+      ++gen_;
+    }
+    cv_work_.notify_all();
+    drain(f);
+    std::exception_ptr err;
+    {
+      std::unique_lock<std::mutex> lk(m_);
+      cv_done_.wait(lk, [&] { return finished_ == workers_; });
+      fn_ = nullptr;
+      err = err_;
+    }
+    if (err) std::rethrow_exception(err);
+    return true;
+  }
+
+ private:
+  void loop() {
+    tl_in_region = true;
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int)> *f;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_work_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        f = fn_;
+      }
+      drain(*f);
+      std::lock_guard<std::mutex> lk(m_);
+      if (++finished_ == workers_) cv_done_.notify_all();
+    }
+  }
+  void drain(const std::function<void(int)> &f) {
+    for (;;) {
+      const int k = next_.fetch_add(1);
+      if (k >= n_ || failed_.load()) return;
+      try {
+        f(k);
+      } catch (...) {
+        if (!failed_.exchange(true)) {
+          std::lock_guard<std::mutex> lk(m_);
+          err_ = std::current_exception();
+        }
+      }
+    }
+  }
+  const int workers_;
+  std::vector<std::thread> th_;
+  std::mutex busy_, m_;
+  std::condition_variable cv_work_, cv_done_;
+  const std::function<void(int)> *fn_ = nullptr;
+  int n_ = 0, finished_ = 0;
+  uint64_t gen_ = 0;
+  std::atomic<int> next_{0};
+  std::atomic<bool> failed_{false};
+  std::exception_ptr err_;
+};
+
+// The process's pool: 15 workers at most (16 threads with the caller), never
+// destroyed (its workers wait on a condition variable until the process ends).
+HostPool &host_pool() {
+  static HostPool *pool = new HostPool(
+      int(std::max(1u, std::min(15u, std::max(1u, std::thread::hardware_concurrency()) - 1u))));
+  return *pool;
+}
+
 // Runs fn(order[k]) for k in [0, n) on up to 16 host threads (items are
 // handed out in order, so put the largest first); rethrows the first error.
+// The pool runs it when free; a nested or concurrent region starts threads.
 template <typename Fn>
 void parallel_for(const std::vector<int> &order, Fn &&fn) {
   const int n = int(order.size());
@@ -1518,6 +1619,8 @@ void parallel_for(const std::vector<int> &order, Fn &&fn) {
     for (int k = 0; k < n; ++k) fn(order[k]);
     return;
   }
+  const std::function<void(int)> body_k = [&](int k) { fn(order[size_t(k)]); };
+  if (host_pool().run(n, body_k)) return;
   std::atomic<int> next{0};
   std::exception_ptr err;
   std::atomic<bool> failed{false};
