@@ -163,6 +163,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "pair_nopend") {
     if (value < 0 || value > 2) return fail(err, LSIM_E_ARG, "pair_nopend must be 0, 1 or 2");
     eo.pair_nopend = int(value);
+  } else if (k == "pair_fo_pre") {
+    if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "pair_fo_pre must be 0 or 1");
+    eo.pair_fo_pre = value != 0;
   } else if (k == "max_step_words") {
     if (value < 1 || value > 4096) return fail(err, LSIM_E_ARG, "max_step_words must be 1..4096");
     eo.max_step_words = int(value);

@@ -785,6 +785,7 @@ struct PairArgs {
   uint32_t epoch_base;
   int dense_lanes;             // a pass whose nodes average this many active lanes goes one node per wave (0: never)
   int nopend;                  // no pending flags: every node of a walked level is a candidate
+  int fo_pre;                  // with pending flags: a staged node's first live fanout is loaded in the scan
   unsigned long long *tstats;  // as SweepArgs::tstats, or null
 };
 
@@ -800,6 +801,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ unsigned long long s_act[CHUNK], s_nm[CHUNK];
   __shared__ int s_pref[CHUNK];
   __shared__ int s_fo0[CHUNK], s_deg[CHUNK];  // live fanout range
+  __shared__ int2 s_fop[CHUNK];               // first live fanout {position, level} (fo_pre)
   __shared__ int s_tile, s_maxl;
   __shared__ uint32_t s_pool;  // next free word of the slot's pool for this tile
   __shared__ unsigned long long s_cnt2;  // nodes << 32 | pairs
@@ -864,7 +866,10 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         if (kStats) cy_t = clock64();
         // A. scan and stage. The node's static data (type, fanins, fanout
         // range) is loaded together with its pending flag, so a pending node
-        // costs two dependent loads: these, then its fanins' records.
+        // costs two dependent loads: these, then its fanins' records. With
+        // fo_pre its first live fanout is loaded beside the records, so that a
+        // written node with one live fanout marks it without a further load.
+        const bool pre = a.fo_pre && !a.nopend;
         const int u = c0 + tid;
         const bool in = u < hi;
         uint32_t pf = 0, inf = 0;
@@ -883,7 +888,12 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         uint64_t act = 0;
         uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
         uint32_t fb[4] = {0, 0, 0, 0};
+        int2 fp = make_int2(0, 0);
         if (cand) {
+          if (pre && fo1 > fo0) {
+            fp = J.fo[fo0];
+            bytes += 8;
+          }
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
           if (type != G_PI) {
             const int fi[4] = {f.x, f.y, f.z, f.w};
@@ -920,6 +930,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           s_pref[k] = int(uint32_t(slot));
           s_fo0[k] = fo0;
           s_deg[k] = fo1 - fo0;
+          s_fop[k] = fp;
         }
         __syncthreads();
         if (kStats) {
@@ -1095,6 +1106,14 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             if (deg) atomicMax(&s_maxl, int(s_info[tid] >> 8));
             bytes += 40;
             if (kStats) ++n_written;
+            if (pre && deg) {
+              const int2 fv = s_fop[tid];
+              pend[fv.x] = epoch;
+              s_lvl[fv.y] = 1;
+              bytes += 4;
+              ++e0;
+              --deg;
+            }
           }
         }
         int incl = a.nopend ? 0 : deg;
@@ -2927,6 +2946,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   a.ovf_list = d_ovf_list.p;
   a.tstats = tstats;
   a.dense_lanes = eo.pair_dense;
+  a.fo_pre = eo.pair_fo_pre ? 1 : 0;
   a.nopend = nopend ? 1 : 0;
   // A share of the device's resident workgroups (another engine on the same
   // device may hold the rest).
