@@ -2626,44 +2626,60 @@ void Engine::Impl::finalize_jobs(const std::vector<JobState *> &js) {
   }
   LSIM_HIP(hipStreamSynchronize(stream));
   const auto t1 = clk::now();
-  // Unpacked per job in parallel, largest first.
+  // Unpacked in parallel, largest job first. A job of many faults is split
+  // into contiguous fault ranges, so that several threads touch and fill its
+  // result arrays (sized here without initialisation).
   std::vector<int> order(js.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return js[size_t(x)]->F > js[size_t(y)]->F; });
-  parallel_for(order, [&](int ii) {
-    const size_t i = size_t(ii);
+  struct Piece {
+    int job;
+    int64_t lo, hi;
+  };
+  std::vector<Piece> pieces;
+  for (int ii : order) {
+    JobState &j = *js[size_t(ii)];
+    RunResults &res = j.res;
+    const int64_t F = j.F;
+    res.first_round.resize(size_t(F));
+    res.last_round.resize(size_t(F));
+    res.RB = j.RB;
+    if (!j.opt.drop) res.det_bits.assign(size_t(F * j.RB), 0);
+    const int64_t n = std::max<int64_t>(1, std::min<int64_t>(16, F >> 18));
+    for (int64_t c = 0; c < n; ++c) pieces.push_back({ii, F * c / n, F * (c + 1) / n});
+  }
+  std::vector<int> piece_ids(pieces.size());
+  std::iota(piece_ids.begin(), piece_ids.end(), 0);
+  parallel_for(piece_ids, [&](int pi) {
+    const Piece &pc = pieces[size_t(pi)];
+    const size_t i = size_t(pc.job);
     JobState &j = *js[i];
     RunResults &res = j.res;
-    const int64_t F = j.F, R = j.R;
+    const int64_t F = j.F, R = j.R, lo = pc.lo, hi = pc.hi;
     const uint8_t *o = h_final.p + at[i];
-    res.first_round.assign(size_t(F), INT32_MAX);
-    if (F) std::memcpy(res.first_round.data(), o, size_t(F) * sizeof(int32_t));
+    const int32_t *first = reinterpret_cast<const int32_t *>(o);
+    for (int64_t f = lo; f < hi; ++f) res.first_round[size_t(f)] = first[f] == INT32_MAX ? -1 : first[f];
     o += size_t(F) * sizeof(int32_t);
-    res.RB = j.RB;
     if (!j.opt.drop) {
-      res.det_bits.assign(size_t(F * j.RB), 0);
-      if (F && j.detbits.p) {
-        std::memcpy(res.det_bits.data(), o, size_t(F * j.RB) * sizeof(uint64_t));
-        o += size_t(F * j.RB) * sizeof(uint64_t);
-      }
+      if (j.detbits.p && hi > lo)
+        std::memcpy(res.det_bits.data() + lo * j.RB, o + size_t(lo * j.RB) * sizeof(uint64_t),
+                    size_t((hi - lo) * j.RB) * sizeof(uint64_t));
+      for (int64_t f = lo; f < hi; ++f)
+        res.last_round[size_t(f)] =
+            R > 0 ? uint8_t((res.det_bits[size_t(f * j.RB + ((R - 1) >> 6))] >> ((R - 1) & 63)) & 1) : uint8_t(0);
+    } else if (j.last.p && j.last_done) {
+      if (hi > lo) std::memcpy(res.last_round.data() + lo, o + lo, size_t(hi - lo));
+    } else if (hi > lo) {
+      std::memset(res.last_round.data() + lo, 0, size_t(hi - lo));
     }
-    res.last_round.assign(size_t(F), 0);
-    if (j.opt.drop && j.last.p && j.last_done && F) std::memcpy(res.last_round.data(), o, size_t(F));
-    if (!j.opt.drop) {
-      res.has_matrix = true;
-      res.has_last = true;
-      if (R > 0)
-        for (int64_t f = 0; f < F; ++f)
-          res.last_round[size_t(f)] = uint8_t((res.det_bits[size_t(f * j.RB + ((R - 1) >> 6))] >> ((R - 1) & 63)) & 1);
-    } else {
-      res.has_matrix = false;
-      res.has_last = j.opt.report_last_round;
-    }
-    for (auto &v : res.first_round)
-      if (v == INT32_MAX) v = -1;
-    res.stats.slots = slots;
-    j.done = true;
   });
+  for (JobState *jp : js) {
+    RunResults &res = jp->res;
+    res.has_matrix = !jp->opt.drop;
+    res.has_last = !jp->opt.drop || jp->opt.report_last_round;
+    res.stats.slots = slots;
+    jp->done = true;
+  }
   if (trace_on()) {
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::fprintf(stderr, "[lsim] finalize jobs=%zu: copy %.2f ms, unpack %.2f ms\n", js.size(), ms(t0, t1),

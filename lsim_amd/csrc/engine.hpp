@@ -11,6 +11,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "lsim.h"
@@ -22,6 +24,30 @@ struct DeviceError : std::runtime_error {
   int code;
   DeviceError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
+
+// Allocator whose value-less construct leaves elements uninitialised: a
+// resize() of a large result array touches no page, so the threads that fill
+// it are the first to touch it.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <typename U>
+  UninitAlloc(const UninitAlloc<U> &) noexcept {}
+  template <typename U>
+  void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using result_vector = std::vector<T, UninitAlloc<T>>;
 
 // Per-job options.
 struct JobOptions {
@@ -56,13 +82,13 @@ struct RunResults {
   // (only with keep_good_last).
   std::vector<uint64_t> good_last;
   // First detecting round per fault, -1 if never.
-  std::vector<int32_t> first_round;
+  result_vector<int32_t> first_round;
   // d(f, r) bit matrix, F rows of RB words (only without dropping).
   std::vector<uint64_t> det_bits;
   int64_t RB = 0;
   bool has_matrix = false;
   // d(f, R-1) per fault, when available.
-  std::vector<uint8_t> last_round;
+  result_vector<uint8_t> last_round;
   bool has_last = false;
   lsim_run_stats stats{};
 };
