@@ -1883,6 +1883,7 @@ struct Engine::Impl {
     circuits.clear();
     if (ev[0]) (void)hipEventDestroy(ev[0]);
     if (ev[1]) (void)hipEventDestroy(ev[1]);
+    if (fin_ev) (void)hipEventDestroy(fin_ev);
     // The stream itself goes with `holder`, after the member buffers.
   }
   void time_begin() { LSIM_HIP(hipEventRecord(ev[0], stream)); }
@@ -1936,7 +1937,14 @@ struct Engine::Impl {
   bool plan(JobState &j);
   std::vector<JobState *> to_finalize;  // jobs plan() found finished
   HostBuf<uint8_t> h_final;
-  void finalize_jobs(const std::vector<JobState *> &js);
+  // Finished jobs whose result copies are in flight (finalize_begin) and not
+  // yet unpacked (finalize_end): the unpack runs while the next sweep does.
+  std::vector<JobState *> fin_js;
+  std::vector<size_t> fin_at;
+  std::chrono::steady_clock::time_point fin_t0;
+  hipEvent_t fin_ev = nullptr;
+  void finalize_begin(const std::vector<JobState *> &js);
+  void finalize_end();
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
   void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles, const std::vector<Tile> *tiles,
              int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, bool nopend,
@@ -1976,6 +1984,7 @@ Engine::Engine(int device) : impl_(new Impl) {
     }
     LSIM_HIP(hipEventCreate(&impl_->ev[0]));
     LSIM_HIP(hipEventCreate(&impl_->ev[1]));
+    LSIM_HIP(hipEventCreateWithFlags(&impl_->fin_ev, hipEventDisableTiming));
     impl_->d_bytes.alloc(3);
     LSIM_HIP(hipMemsetAsync(impl_->d_bytes.p, 0, 3 * sizeof(unsigned long long), impl_->stream));
   } catch (...) {
@@ -2554,7 +2563,7 @@ void Engine::Impl::finish_start(const std::vector<JobState *> &js, double ms) {
   }
 }
 
-// Decides the job's next step; false (and the job queued for finalize_jobs)
+// Decides the job's next step; false (and the job queued for finalize_begin)
 // when none is left.
 bool Engine::Impl::plan(JobState &j) {
   if (j.done) return false;
@@ -2598,12 +2607,15 @@ bool Engine::Impl::plan(JobState &j) {
 }
 
 // Results of finished jobs: every job's device arrays copied into one
-// page-locked block, one wait, then unpacked.
-void Engine::Impl::finalize_jobs(const std::vector<JobState *> &js) {
+// page-locked block (finalize_begin enqueues the copies and records fin_ev),
+// then unpacked (finalize_end waits for fin_ev). The iteration that follows
+// calls finalize_end once its sweep is launched, so the unpack overlaps it.
+void Engine::Impl::finalize_begin(const std::vector<JobState *> &js) {
+  finalize_end();
   if (js.empty()) return;
-  using clk = std::chrono::steady_clock;
-  const auto t0 = clk::now();
-  std::vector<size_t> at(js.size() + 1, 0);
+  fin_t0 = std::chrono::steady_clock::now();
+  std::vector<size_t> &at = fin_at;
+  at.assign(js.size() + 1, 0);
   for (size_t i = 0; i < js.size(); ++i) {
     const JobState &j = *js[i];
     size_t b = size_t(j.F) * sizeof(int32_t);
@@ -2624,7 +2636,18 @@ void Engine::Impl::finalize_jobs(const std::vector<JobState *> &js) {
     }
     if (j.opt.drop && j.last.p && j.last_done && j.F) d2h(o, j.last.p, size_t(j.F));
   }
-  LSIM_HIP(hipStreamSynchronize(stream));
+  LSIM_HIP(hipEventRecord(fin_ev, stream));
+  fin_js = js;
+}
+
+void Engine::Impl::finalize_end() {
+  if (fin_js.empty()) return;
+  using clk = std::chrono::steady_clock;
+  const std::vector<JobState *> js = std::move(fin_js);
+  fin_js.clear();
+  const std::vector<size_t> &at = fin_at;
+  const auto t0 = clk::now();
+  LSIM_HIP(hipEventSynchronize(fin_ev));
   const auto t1 = clk::now();
   // Unpacked in parallel, largest job first. A job of many faults is split
   // into contiguous fault ranges, so that several threads touch and fill its
@@ -2682,8 +2705,8 @@ void Engine::Impl::finalize_jobs(const std::vector<JobState *> &js) {
   }
   if (trace_on()) {
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-    std::fprintf(stderr, "[lsim] finalize jobs=%zu: copy %.2f ms, unpack %.2f ms\n", js.size(), ms(t0, t1),
-                 ms(t1, clk::now()));
+    std::fprintf(stderr, "[lsim] finalize jobs=%zu: copies enqueued %.2f ms before, waited %.2f ms, unpack %.2f ms\n",
+                 js.size(), ms(fin_t0, t0), ms(t0, t1), ms(t1, clk::now()));
   }
 }
 
@@ -2978,6 +3001,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   hipLaunchKernelGGL(pairs_kernel(threads, tstats != nullptr), dim3(unsigned(grid)), dim3(threads), lds, stream, a);
   LSIM_HIP(hipGetLastError());
   st.sweep_launches++;
+  finalize_end();  // finished jobs' results unpacked while the sweep runs
   if (tstats)
     std::fprintf(stderr, "[lsim] pair sweep: %d threads, %lld workgroups (%d/%d per CU at 256/512), %lld-word pools\n",
                  threads, (long long)grid, pc256, pc512, (long long)a.pool_words);
@@ -3412,14 +3436,21 @@ int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const En
   to_finalize.clear();
   for (JobState *j : jobs)
     if (j->started && !j->done && plan(*j)) act.push_back(j);
-  finalize_jobs(to_finalize);
+  finalize_begin(to_finalize);
   to_finalize.clear();
-  if (!act.empty()) {
-    if (eo.mode == LSIM_MODE_PLAIN) {
-      for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
-    } else {
-      step_batch(act, eo);
+  try {
+    if (!act.empty()) {
+      if (eo.mode == LSIM_MODE_PLAIN) {
+        for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
+      } else {
+        step_batch(act, eo);
+      }
     }
+    finalize_end();
+  } catch (...) {
+    // Jobs left unpacked stay unfinished: the next plan() queues them again.
+    fin_js.clear();
+    throw;
   }
   int64_t left = 0;
   for (JobState *j : jobs) left += j->done ? 0 : 1;
