@@ -1318,9 +1318,25 @@ __global__ void k_gather_first(const JobDev *__restrict__ jobs, int njobs, const
 
 // units[i] = {job, stem position, unit index}: head_slot of that job maps the
 // stem to its unit index within the job (or back to -1).
-__global__ void k_set_slots(const JobDev *__restrict__ jobs, const int4 *__restrict__ units, int n, int clear) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) jobs[units[i].x].head_slot[units[i].y] = clear ? -1 : units[i].z;
+// Job of entry t of the step's unit list: the largest i with lane_unit0 <= t.
+__device__ __forceinline__ int job_of_unit(const JobDev *__restrict__ jobs, int nj, int64_t t) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].lane_unit0 <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// unit_pos[lane_unit0 + k] = position of unit k of a job. For stem-tracing
+// jobs, the stem's slot becomes k (or -1 again with clear).
+__global__ void k_set_slots(const JobDev *__restrict__ jobs, int nj, const int32_t *__restrict__ unit_pos, int64_t n,
+                            int clear) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const JobDev &J = jobs[job_of_unit(jobs, nj, t)];
+  if (J.direct) return;
+  J.head_slot[unit_pos[t]] = clear ? -1 : int32_t(t - J.lane_unit0);
 }
 
 // ---------------------------------------------------------------- lanes and tiles on the device
@@ -1367,9 +1383,11 @@ __global__ void k_lane_bounds(const JobDev *__restrict__ jobs, int nj, const uin
   lane_off[t] = o < words ? int64_t(pos[o]) : int64_t(pos[words - 1]) + (need[words - 1] != 0 ? 1 : 0);
 }
 
-// units[lane_unit0 + k] = {position, injection kind} of unit k of a job.
+// unit_pos[lane_unit0 + k] = position of unit k of a job. Its injection is a
+// flip for stem tracing, else the stuck value of step fault k (a direct job's
+// step faults are in unit order).
 __global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
-                            const int32_t *__restrict__ pos, const int2 *__restrict__ units, int64_t words,
+                            const int32_t *__restrict__ pos, const int32_t *__restrict__ unit_pos, int64_t words,
                             Lane *__restrict__ lanes) {
   const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (o >= words) return;
@@ -1379,14 +1397,14 @@ __global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint6
   const int64_t rel = o - J.word0;
   const int64_t k = rel / J.Wn;
   const int jw = int(rel - k * J.Wn);
-  const int2 uk = units[J.lane_unit0 + k];
+  const int32_t site = unit_pos[J.lane_unit0 + k];
   Lane ln;
   ln.mask = m;
-  ln.site = uk.x;
+  ln.site = site;
   ln.word = int32_t(J.w0 + jw);
   ln.out = int32_t(o);
-  ln.kind = uint32_t(uk.y);
-  ln.level = level_of(J.live_lvl, J.max_level, uk.x);
+  ln.kind = uint32_t(J.direct ? (J.sa[J.flist[k]] ? K_SA1 : K_SA0) : K_FLIP);
+  ln.level = level_of(J.live_lvl, J.max_level, site);
   ln.pad = 0;
   lanes[pos[o]] = ln;
 }
@@ -1841,8 +1859,8 @@ struct Engine::Impl {
   HostBuf<uint64_t> h_need;
   HostBuf<Lane> h_lanes;
   // device-built lanes and tiles (persistent sweeps)
-  HostBuf<int2> h_lane_units;
-  DevBuf<int2> d_lane_units;
+  HostBuf<int32_t> h_unit_pos;  // the step's units (positions), job after job
+  DevBuf<int32_t> d_unit_pos;
   DevBuf<int32_t> d_pos;
   DevBuf<int64_t> d_lane_off, d_tile_off;
   HostBuf<int64_t> h_lane_off;
@@ -1861,7 +1879,6 @@ struct Engine::Impl {
   DevBuf<uint64_t> d_need, d_dout, d_gather;
   DevBuf<Lane> d_lanes;
   DevBuf<Tile> d_tiles;
-  DevBuf<int4> d_units;
   DevBuf<int> d_queue;
   DevBuf<unsigned long long> d_bytes, d_tstats;
   // sweep slots
@@ -3046,7 +3063,6 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   std::vector<int64_t> word0(nj + 1, 0);
   std::vector<int64_t> flist_off(nj + 1, 0);  // step faults of job i at [flist_off[i], flist_off[i+1])
   std::vector<int64_t> fault_off(nj + 1, 0);  // dropping jobs' faults in the first-round gather
-  std::vector<int4> units_all;
   int lvl_entries = 1;
   int64_t nl_max = 1;
   lsim_run_stats st{};
@@ -3098,8 +3114,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       });
     }
   });
-  std::vector<int64_t> unit_off(nj + 1, 0);       // stem units of job i in units_all
-  std::vector<int64_t> lane_unit_off(nj + 1, 0);  // units of job i in the {position, kind} list
+  std::vector<int64_t> lane_unit_off(nj + 1, 0);  // units of job i in the step's unit list
   const bool plain = eo.mode == LSIM_MODE_PLAIN;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
@@ -3107,7 +3122,6 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     const int64_t nf = int64_t(j.step_faults.size());
     flist_off[i + 1] = flist_off[i] + nf;
     fault_off[i + 1] = fault_off[i] + (j.kind == JobState::DROP ? nf : 0);
-    unit_off[i + 1] = unit_off[i] + (j.direct ? 0 : int64_t(j.units.size()));
     JobDev d = job_dev(j);
     d.nf = nf;
     d.w0 = j.step_w0;
@@ -3130,23 +3144,18 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   const int64_t nf_all = flist_off[nj];
   h_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
   d_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
-  units_all.resize(size_t(unit_off[nj]));
-  if (!plain) h_lane_units.alloc(size_t(std::max<int64_t>(lane_unit_off[nj], 1)));
+  const int64_t nu_all = lane_unit_off[nj];
+  h_unit_pos.alloc(size_t(std::max<int64_t>(nu_all, 1)));
   parallel_for(order, [&](int i) {
     const JobState &j = *act[i];
     std::memcpy(h_flist.p + flist_off[i], j.step_faults.data(), j.step_faults.size() * sizeof(int32_t));
-    if (!j.direct)
-      for (size_t k = 0; k < j.units.size(); ++k)
-        units_all[size_t(unit_off[i]) + k] = make_int4(i, j.units[k], int(k), 0);
-    if (!plain) {
-      int2 *lu = h_lane_units.p + lane_unit_off[i];
-      for (size_t k = 0; k < j.units.size(); ++k)
-        lu[k] = make_int2(j.units[k], int(j.direct ? (j.sa[j.step_faults[k]] ? K_SA1 : K_SA0) : K_FLIP));
-    }
+    std::memcpy(h_unit_pos.p + lane_unit_off[i], j.units.data(), j.units.size() * sizeof(int32_t));
   });
-  if (!plain && lane_unit_off[nj]) {
-    d_lane_units.alloc(size_t(lane_unit_off[nj]));
-    h2d(d_lane_units.p, h_lane_units.p, size_t(lane_unit_off[nj]) * sizeof(int2));
+  // Only positions travel: the device derives each unit's job, index and
+  // injection kind (k_set_slots, k_lane_fill).
+  if (nu_all) {
+    d_unit_pos.alloc(size_t(nu_all));
+    h2d(d_unit_pos.p, h_unit_pos.p, size_t(nu_all) * sizeof(int32_t));
   }
   for (int i = 0; i < nj; ++i) devs[i].flist = d_flist.p + flist_off[i];
   h2d(d_flist.p, h_flist.p, size_t(nf_all) * sizeof(int32_t));
@@ -3161,11 +3170,8 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   d_dout.alloc(words);
   LSIM_HIP(hipMemsetAsync(d_need.p, 0, words * sizeof(uint64_t), stream));
   LSIM_HIP(hipMemsetAsync(d_dout.p, 0, words * sizeof(uint64_t), stream));
-  const int nu_all = int(units_all.size());
   if (nu_all) {
-    d_units.alloc(nu_all);
-    h2d(d_units.p, units_all.data(), nu_all * sizeof(int4));
-    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, d_units.p, nu_all, 0);
+    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, nj, d_unit_pos.p, nu_all, 0);
     LSIM_HIP(hipGetLastError());
   }
   ItemArgs ia;
@@ -3201,7 +3207,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flags, d_pos.p, int(nw), stream));
       d_cub.alloc(tmp);
       LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(d_cub.p, tmp, flags, d_pos.p, int(nw), stream));
-      k_lane_fill<<<blocks_for(nw, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_lane_units.p, nw,
+      k_lane_fill<<<blocks_for(nw, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_unit_pos.p, nw,
                                                           d_lanes.p);
       LSIM_HIP(hipGetLastError());
     }
@@ -3358,7 +3364,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     LSIM_HIP(hipGetLastError());
   }
   if (nu_all) {
-    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, d_units.p, nu_all, 1);
+    k_set_slots<<<blocks_for(nu_all, 256), 256, 0, stream>>>(d_jobs.p, nj, d_unit_pos.p, nu_all, 1);
     LSIM_HIP(hipGetLastError());
   }
   // One copy of every dropping job's first rounds for the step's faults.
