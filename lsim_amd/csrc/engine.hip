@@ -1756,7 +1756,11 @@ struct JobState {
   int64_t F = 0, R = 0, num_patterns = 0, RB = 0;
   std::vector<int32_t> site, head;  // per fault: device position, region head (-1: dead site)
   std::vector<uint8_t> sa;
+  // Faults with a live site, ordered by injection unit (stable, so by fault id
+  // within a unit): by site for direct injection, by region head otherwise.
+  // Dropping filters keep that order, so a step's units come from one pass.
   std::vector<int32_t> live, alive;
+  bool live_by_site = false;  // the order of `live`
   DevBuf<uint8_t> arena;  // one allocation holding every per-job device array below
   DevBuf<uint64_t> good, pi_words, detbits;
   DevBuf<int32_t> d_site, d_head, first, head_slot;
@@ -2150,6 +2154,22 @@ void Engine::remove_circuit(CircuitRes *c) {
 
 int64_t Engine::circuit_bytes(const CircuitRes *c) const { return c->bytes(); }
 
+namespace {
+// Stable counting sort of j.live by injection unit (site or region head, both
+// live positions in [0, NL)).
+void order_live(JobState &j) {
+  const int64_t NL = j.c->NL;
+  const std::vector<int32_t> &key = j.direct ? j.site : j.head;
+  std::vector<int32_t> cnt(size_t(NL) + 1, 0);
+  for (int32_t f : j.live) cnt[size_t(key[size_t(f)]) + 1]++;
+  for (int64_t u = 0; u < NL; ++u) cnt[size_t(u) + 1] += cnt[size_t(u)];
+  std::vector<int32_t> sorted(j.live.size());
+  for (int32_t f : j.live) sorted[size_t(cnt[size_t(key[size_t(f)])]++)] = f;
+  j.live.swap(sorted);
+  j.live_by_site = j.direct;
+}
+}  // namespace
+
 int64_t Engine::job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop) {
   const int64_t RB = (R + 63) / 64;
   return 8 * c->N * R + 8 * c->P * R + 4 * std::max<int64_t>(c->NL, 1) + faults * (4 + 4 + 1 + 4 + 4 + 1) +
@@ -2234,6 +2254,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   LSIM_HIP(hipGetLastError());
   if (j.detbits.p) LSIM_HIP(hipMemsetAsync(j.detbits.p, 0, j.detbits.bytes(), m.stream));
   if (j.last.p) LSIM_HIP(hipMemsetAsync(j.last.p, 0, j.last.bytes(), m.stream));
+  order_live(j);
   j.alive = j.live;
   j.Wt = std::max(1, std::min(64, opt.first_words));
   j.Wg = std::max(1, std::min(64, opt.word_growth));
@@ -2274,6 +2295,7 @@ bool Engine::reset_job(JobState *j, const JobOptions &opt) {
   j->units.clear();
   j->step_w0 = 0;
   j->step_Wn = 0;
+  if (j->direct != j->live_by_site) order_live(*j);
   j->alive = j->live;
   j->res = RunResults();
   j->res.stats.live_nodes = j->c->NL;
@@ -3077,43 +3099,40 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     JobState &j = *act[i];
     const CircuitRes &c = *j.c;
     j.units.clear();
+    // The step's faults are in unit order (order_live; dropping keeps it).
+    const int64_t nf = int64_t(j.step_faults.size());
+    const int C = chunk_count(nf, nj == 1);
     if (j.direct) {
-      std::vector<int32_t> cnt(size_t(c.NL) + 1, 0);
-      for (int32_t f : j.step_faults) cnt[j.site[f] + 1]++;
-      for (int64_t u = 0; u < c.NL; ++u) cnt[u + 1] += cnt[u];
-      std::vector<int32_t> sorted(j.step_faults.size());
-      for (int32_t f : j.step_faults) sorted[cnt[j.site[f]]++] = f;
-      j.step_faults.swap(sorted);
-      j.units.resize(j.step_faults.size());
-      for (size_t k = 0; k < j.step_faults.size(); ++k) j.units[k] = j.site[j.step_faults[k]];
-    } else {
-      // Stems of the step's faults in position order: marked, counted per
-      // chunk of positions, then listed (a lone job splits both over host
-      // threads).
-      std::vector<uint8_t> mark(size_t(c.NL), 0);
-      const int64_t nf = int64_t(j.step_faults.size());
-      run_chunks(chunk_count(nf, nj == 1), nf, [&](int, int64_t a, int64_t b) {
-        for (int64_t k = a; k < b; ++k) {
-          const int32_t h = j.head[j.step_faults[size_t(k)]];
-          if (!c.head_is_po[h]) __atomic_store_n(&mark[size_t(h)], uint8_t(1), __ATOMIC_RELAXED);
-        }
+      // One unit per fault, at its site.
+      j.units.resize(size_t(nf));
+      run_chunks(C, nf, [&](int, int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) j.units[size_t(k)] = j.site[size_t(j.step_faults[size_t(k)])];
       });
-      const int cu = chunk_count(c.NL, nj == 1);
-      std::vector<int64_t> cnt(size_t(cu) + 1, 0);
-      run_chunks(cu, c.NL, [&](int ch, int64_t a, int64_t b) {
+    } else {
+      // Stems of the step's faults in position order: a fault opens a unit
+      // when its region head is not a PO and differs from the previous
+      // fault's (counted per chunk, then listed; a lone job splits both over
+      // host threads).
+      auto opens = [&](int64_t k) {
+        const int32_t h = j.head[size_t(j.step_faults[size_t(k)])];
+        return !c.head_is_po[size_t(h)] && (k == 0 || j.head[size_t(j.step_faults[size_t(k - 1)])] != h);
+      };
+      std::vector<int64_t> cnt(size_t(C) + 1, 0);
+      run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
         int64_t m = 0;
-        for (int64_t u = a; u < b; ++u) m += mark[size_t(u)];
+        for (int64_t k = a; k < b; ++k) m += opens(k) ? 1 : 0;
         cnt[size_t(ch) + 1] = m;
       });
-      for (int ch = 0; ch < cu; ++ch) cnt[size_t(ch) + 1] += cnt[size_t(ch)];
-      j.units.resize(size_t(cnt[size_t(cu)]));
-      run_chunks(cu, c.NL, [&](int ch, int64_t a, int64_t b) {
+      for (int ch = 0; ch < C; ++ch) cnt[size_t(ch) + 1] += cnt[size_t(ch)];
+      j.units.resize(size_t(cnt[size_t(C)]));
+      run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
         int64_t o = cnt[size_t(ch)];
-        for (int64_t u = a; u < b; ++u)
-          if (mark[size_t(u)]) j.units[size_t(o++)] = int32_t(u);
+        for (int64_t k = a; k < b; ++k)
+          if (opens(k)) j.units[size_t(o++)] = j.head[size_t(j.step_faults[size_t(k)])];
       });
     }
   });
+  const auto h0u = clk::now();
   std::vector<int64_t> lane_unit_off(nj + 1, 0);  // units of job i in the step's unit list
   const bool plain = eo.mode == LSIM_MODE_PLAIN;
   for (int i = 0; i < nj; ++i) {
@@ -3146,6 +3165,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   d_flist.alloc(size_t(std::max<int64_t>(nf_all, 1)));
   const int64_t nu_all = lane_unit_off[nj];
   h_unit_pos.alloc(size_t(std::max<int64_t>(nu_all, 1)));
+  const auto h0s = clk::now();
   parallel_for(order, [&](int i) {
     const JobState &j = *act[i];
     std::memcpy(h_flist.p + flist_off[i], j.step_faults.data(), j.step_faults.size() * sizeof(int32_t));
@@ -3431,9 +3451,11 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::fprintf(stderr,
                  "[lsim] iteration jobs=%d items=%lld units=%lld lanes=%lld tiles=%lld staged share %.3f | prep %.2f "
-                 "local %.2f lanes %.2f sweep %.2f (dev %.2f) detect %.2f advance %.2f ms\n",
+                 "(units %.2f, offsets %.2f, lists %.2f) local %.2f lanes %.2f sweep %.2f (dev %.2f) detect %.2f "
+                 "advance %.2f ms\n",
                  nj, (long long)items, (long long)word0[nj], (long long)nlanes, (long long)ntiles, staged_share,
-                 ms(h0, h1), ms(h1, h2), ms(h2, h3), ms(h3, h4), st.t_sweep_ms, ms(h4, h5), ms(h5, clk::now()));
+                 ms(h0, h1), ms(h0, h0u), ms(h0u, h0s), ms(h0s, h1), ms(h1, h2), ms(h2, h3), ms(h3, h4), st.t_sweep_ms,
+                 ms(h4, h5), ms(h5, clk::now()));
   }
 }
 
