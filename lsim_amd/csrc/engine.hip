@@ -3113,22 +3113,28 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       // when its region head is not a PO and differs from the previous
       // fault's (counted per chunk, then listed; a lone job splits both over
       // host threads).
-      auto opens = [&](int64_t k) {
-        const int32_t h = j.head[size_t(j.step_faults[size_t(k)])];
-        return !c.head_is_po[size_t(h)] && (k == 0 || j.head[size_t(j.step_faults[size_t(k - 1)])] != h);
-      };
+      auto head_at = [&](int64_t k) { return j.head[size_t(j.step_faults[size_t(k)])]; };
       std::vector<int64_t> cnt(size_t(C) + 1, 0);
       run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
         int64_t m = 0;
-        for (int64_t k = a; k < b; ++k) m += opens(k) ? 1 : 0;
+        int32_t prev = a > 0 ? head_at(a - 1) : -1;
+        for (int64_t k = a; k < b; ++k) {
+          const int32_t h = head_at(k);
+          m += (h != prev && !c.head_is_po[size_t(h)]) ? 1 : 0;
+          prev = h;
+        }
         cnt[size_t(ch) + 1] = m;
       });
       for (int ch = 0; ch < C; ++ch) cnt[size_t(ch) + 1] += cnt[size_t(ch)];
       j.units.resize(size_t(cnt[size_t(C)]));
       run_chunks(C, nf, [&](int ch, int64_t a, int64_t b) {
         int64_t o = cnt[size_t(ch)];
-        for (int64_t k = a; k < b; ++k)
-          if (opens(k)) j.units[size_t(o++)] = j.head[size_t(j.step_faults[size_t(k)])];
+        int32_t prev = a > 0 ? head_at(a - 1) : -1;
+        for (int64_t k = a; k < b; ++k) {
+          const int32_t h = head_at(k);
+          if (h != prev && !c.head_is_po[size_t(h)]) j.units[size_t(o++)] = h;
+          prev = h;
+        }
       });
     }
   });
