@@ -3211,9 +3211,8 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     k_local<<<blocks_for(items, 256), 256, 0, stream>>>(ia);
     LSIM_HIP(hipGetLastError());
   }
-  // 512-thread pair sweep groups for dense steps (a dropping job's first
-  // step: all faults, most lanes active at most nodes; full-matrix steps) and
-  // when most tiles walk levels wider than 256 nodes (one scan pass each).
+  // Dense steps: a dropping job's first step (all faults, most lanes active
+  // at most nodes) and full-matrix steps.
   bool dense = false;
   for (const JobState *j : act)
     dense = dense || j->kind == JobState::MATRIX || (j->kind == JobState::DROP && j->w0 == 0);
@@ -3276,7 +3275,10 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     }
     h3 = clk::now();
     const bool mid = staged_share >= kMidShareLo && staged_share <= kMidShareHi;
-    const bool big_groups = dense || (2 * wide_tiles > ntiles && !mid);
+    // 512-thread groups when most tiles walk levels wider than 256 nodes, in
+    // dense steps and outside a middle staged share; 256 otherwise (narrow
+    // levels leave most of a 512-thread scan idle, dense steps included).
+    const bool big_groups = 2 * wide_tiles > ntiles && (dense || !mid);
     // Dense steps evaluate every node of the levels a tile walks instead of
     // marking fanouts pending (most nodes are reached anyway): a dropping
     // job's first step, full-matrix steps, and steps after one that staged
