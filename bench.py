@@ -366,7 +366,8 @@ def main() -> int:
     ap.add_argument("--sweep-kernel", default="pairs", choices=["pairs", "nodewave"],
                     help="persistent sweep kernel of the partitioned engine (without --autotune)")
     ap.add_argument("--pair-threads", type=int, default=0, choices=[0, 256, 512],
-                    help="workgroup size of the pair sweep (0: 512 for dense steps, 256 otherwise)")
+                    help="workgroup size of the pair sweep (0: 512 where most tiles walk levels wider than 256 nodes, 256 "
+                         "otherwise)")
     ap.add_argument("--pair-dense", type=int, default=-1,
                     help="pair sweep: passes whose nodes average this many active lanes go one node per wave "
                          "(0: never; -1: the engine default)")
