@@ -180,6 +180,8 @@ def bench_batch(args) -> int:
         batch.set_option("pair_nopend", args.pair_nopend)
     if args.pair_fo_pre >= 0:
         batch.set_option("pair_fo_pre", args.pair_fo_pre)
+    if args.pair_nopend_pct >= 0:
+        batch.set_option("pair_nopend_pct", args.pair_nopend_pct)
     if args.max_step_words >= 0:
         batch.set_option("max_step_words", args.max_step_words)
     cids, jobs_text = [], []
@@ -374,6 +376,9 @@ def main() -> int:
     ap.add_argument("--pair-nopend", type=int, default=-1,
                     help="pair sweep without pending flags: 0 never, 1 dense steps, 2 every step (-1: the engine "
                          "default)")
+    ap.add_argument("--pair-nopend-pct", type=int, default=-1,
+                    help="with pair_nopend 1: flag-free after a step that staged more than this %% of walked nodes "
+                         "(-1: the engine default)")
     ap.add_argument("--pair-fo-pre", type=int, default=-1, choices=[-1, 0, 1],
                     help="pair sweep with pending flags: load a staged node's first fanout in the scan (-1: the "
                          "engine default)")
@@ -435,6 +440,8 @@ def main() -> int:
         sim.set_option("pair_nopend", args.pair_nopend)
     if args.pair_fo_pre >= 0:
         sim.set_option("pair_fo_pre", args.pair_fo_pre)
+    if args.pair_nopend_pct >= 0:
+        sim.set_option("pair_nopend_pct", args.pair_nopend_pct)
     if args.max_step_words >= 0:
         sim.set_option("max_step_words", args.max_step_words)
     if args.first_words > 0:

@@ -91,6 +91,9 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            level is a candidate): 0 never, 1 dense steps (the
  *                            first step of a dropping job, full-matrix steps;
  *                            default), 2 every step.
+ *  "pair_nopend_pct"    0..100: with pair_nopend 1, a step after one that staged
+ *                            more than this percentage of the nodes its tiles
+ *                            walked also runs without pending flags (default 45).
  *  "pair_fo_pre"        0/1  pair sweep with pending flags: a staged node's first
  *                            live fanout is loaded beside its fanin records, so a
  *                            node with one live fanout marks it pending without a
@@ -203,7 +206,7 @@ LSIM_API int lsim_batch_create(int device, lsim_batch **out);
 LSIM_API void lsim_batch_destroy(lsim_batch *b);
 LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
 /* Options as lsim_set_option (engine options: "scratch_mb", "block_waves", "sweep_kernel", "pair_threads",
- * "pair_dense", "pair_nopend", "pair_fo_pre", "max_step_words", "pool_cap", "autotune", "good_graph",
+ * "pair_dense", "pair_nopend", "pair_nopend_pct", "pair_fo_pre", "max_step_words", "pool_cap", "autotune", "good_graph",
  * "good_form") and "mode" (0 plain, 1 persistent). */
 LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
 /* Makes a circuit (.ckt text) resident; returns its id. */

@@ -163,6 +163,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "pair_nopend") {
     if (value < 0 || value > 2) return fail(err, LSIM_E_ARG, "pair_nopend must be 0, 1 or 2");
     eo.pair_nopend = int(value);
+  } else if (k == "pair_nopend_pct") {
+    if (value < 0 || value > 100) return fail(err, LSIM_E_ARG, "pair_nopend_pct must be 0..100");
+    eo.pair_nopend_pct = int(value);
   } else if (k == "pair_fo_pre") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "pair_fo_pre must be 0 or 1");
     eo.pair_fo_pre = value != 0;

@@ -1873,7 +1873,6 @@ struct Engine::Impl {
   DevBuf<uint8_t> d_cub;
   HostBuf<unsigned long long> h_bytes;  // sweep bytes; pair sweep staged and walked nodes
   double staged_share = 0;              // of the last pair sweep launch
-  static constexpr double kNopendShare = 0.45;
   // Steps after one that staged a middle share of the nodes its tiles walked
   // run in 256-thread groups even on wide levels: twice the tiles in flight
   // outweigh the second scan pass per level there (on the 2M-gate circuit,
@@ -3282,8 +3281,9 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     // Dense steps evaluate every node of the levels a tile walks instead of
     // marking fanouts pending (most nodes are reached anyway): a dropping
     // job's first step, full-matrix steps, and steps after one that staged
-    // more than kNopendShare of the nodes it walked.
-    const bool nopend = eo.pair_nopend == 2 || (eo.pair_nopend == 1 && (dense || staged_share > kNopendShare));
+    // more than pair_nopend_pct percent of the nodes it walked.
+    const bool nopend =
+        eo.pair_nopend == 2 || (eo.pair_nopend == 1 && (dense || 100.0 * staged_share > double(eo.pair_nopend_pct)));
     sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, nopend, st);
   } else {
     // The needed bits decide which (unit, word) pairs become sweep lanes.

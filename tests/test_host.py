@@ -158,6 +158,7 @@ def test_accessor_size_checks():
     ("pair_nopend", [0, 1, 2], [-1, 3]),
     ("pair_dense", [0, 24, 64], [-1, 65]),
     ("pair_fo_pre", [0, 1], [-1, 2]),
+    ("pair_nopend_pct", [0, 45, 100], [-1, 101]),
     ("pair_threads", [0, 256, 512], [128, 1024]),
 ])
 def test_option_ranges(key, good, bad):
