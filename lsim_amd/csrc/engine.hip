@@ -1318,6 +1318,28 @@ __global__ void k_gather_first(const JobDev *__restrict__ jobs, int njobs, const
 
 // units[i] = {job, stem position, unit index}: head_slot of that job maps the
 // stem to its unit index within the job (or back to -1).
+// One device array of a finished job, copied to byte offset dst of the
+// result block.
+struct CopySeg {
+  const uint8_t *src;
+  int64_t dst, bytes;
+};
+
+// Gathers the segments (dst ascending, non-overlapping) into one block, so
+// that the host copies it back once; bytes between segments stay untouched.
+__global__ void k_gather_bytes(const CopySeg *__restrict__ segs, int nseg, int64_t end, uint8_t *__restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= end) return;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].dst <= t) lo = mid; else hi = mid - 1;
+  }
+  const CopySeg s = segs[lo];
+  const int64_t o = t - s.dst;
+  if (o >= 0 && o < s.bytes) out[t] = s.src[o];
+}
+
 // Job of entry t of the step's unit list: the largest i with lane_unit0 <= t.
 __device__ __forceinline__ int job_of_unit(const JobDev *__restrict__ jobs, int nj, int64_t t) {
   int lo = 0, hi = nj - 1;
@@ -1961,6 +1983,8 @@ struct Engine::Impl {
   // yet unpacked (finalize_end): the unpack runs while the next sweep does.
   std::vector<JobState *> fin_js;
   std::vector<size_t> fin_at;
+  DevBuf<uint8_t> d_final;
+  DevBuf<CopySeg> d_segs;
   std::chrono::steady_clock::time_point fin_t0;
   hipEvent_t fin_ev = nullptr;
   void finalize_begin(const std::vector<JobState *> &js);
@@ -2662,17 +2686,29 @@ void Engine::Impl::finalize_begin(const std::vector<JobState *> &js) {
     at[i + 1] = at[i] + ((b + 63) & ~size_t(63));
   }
   h_final.alloc(std::max<size_t>(at.back(), 1));
+  // Every array gathered on the device into one block, then one copy back.
+  std::vector<CopySeg> segs;
   for (size_t i = 0; i < js.size(); ++i) {
     const JobState &j = *js[i];
-    uint8_t *o = h_final.p + at[i];
-    const size_t nf = size_t(j.F) * sizeof(int32_t);
-    if (j.F) d2h(o, j.first.p, nf);
+    int64_t o = int64_t(at[i]);
+    const int64_t nf = j.F * int64_t(sizeof(int32_t));
+    if (j.F) segs.push_back({reinterpret_cast<const uint8_t *>(j.first.p), o, nf});
     o += nf;
     if (!j.opt.drop && j.detbits.p && j.F) {
-      d2h(o, j.detbits.p, size_t(j.F * j.RB) * sizeof(uint64_t));
-      o += size_t(j.F * j.RB) * sizeof(uint64_t);
+      const int64_t nb = j.F * j.RB * int64_t(sizeof(uint64_t));
+      segs.push_back({reinterpret_cast<const uint8_t *>(j.detbits.p), o, nb});
+      o += nb;
     }
-    if (j.opt.drop && j.last.p && j.last_done && j.F) d2h(o, j.last.p, size_t(j.F));
+    if (j.opt.drop && j.last.p && j.last_done && j.F) segs.push_back({j.last.p, o, j.F});
+  }
+  if (!segs.empty()) {
+    const int64_t end = segs.back().dst + segs.back().bytes;
+    d_final.alloc(size_t(end));
+    d_segs.alloc(segs.size());
+    h2d(d_segs.p, segs.data(), segs.size() * sizeof(CopySeg));
+    k_gather_bytes<<<blocks_for(end, 256), 256, 0, stream>>>(d_segs.p, int(segs.size()), end, d_final.p);
+    LSIM_HIP(hipGetLastError());
+    d2h(h_final.p, d_final.p, size_t(end));
   }
   LSIM_HIP(hipEventRecord(fin_ev, stream));
   fin_js = js;
