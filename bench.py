@@ -9,17 +9,16 @@ Prints one JSON line (rank 0). `roofline.achieved` is the sweep kernels' own
 count of the bytes they read and write (lsim_run_stats.sweep_bytes) over the
 HIP-event time of their launches; `roofline.traffic` and the request rate come
 from the committed rocprofv3 summary of the same workload under profiles/.
-`cpu_baseline` times the compiled reference simulator on a seeded sample of
-faults and rounds of the same circuit; its cost is linear in faults x rounds.
+`cpu_baseline` times the oracle's restatement of the reference CPU simulator
+on a seeded sample of faults and rounds of the same circuit; its cost is linear
+in faults x rounds.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import subprocess
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -48,28 +47,43 @@ def profiled_value(config: str, kernel: str, key: str = "traffic_bytes_per_launc
 
 
 def probe_random(nbytes: int = 16 << 30) -> dict:
-    """Random 8-byte reads per second the device serves (lsim_probe_random_reads)."""
-    import lsim_amd
+    """Random 8-byte reads per second the device serves, from the measurement
+    library lsim_amd/liblsim_probe.so (not part of the simulator library)."""
+    import ctypes
 
-    return lsim_amd.probe_random_reads(0, nbytes)
+    path = os.path.join(ROOT, "lsim_amd", "liblsim_probe.so")
+    if not os.path.exists(path):
+        raise RuntimeError("lsim_amd/liblsim_probe.so is not built (python lsim_amd/build.py)")
+    L = ctypes.CDLL(path)
+    fn = L.lsim_bench_probe_random_reads
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    rate, lat = ctypes.c_double(), ctypes.c_double()
+    rc = fn(0, int(nbytes), ctypes.byref(rate), ctypes.byref(lat))
+    if rc != 0:
+        raise RuntimeError(f"random-read probe failed ({rc})")
+    return {"reads_per_s": rate.value, "latency_ns": lat.value, "buffer_bytes": int(nbytes)}
 
 
 def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: float) -> dict:
-    """Time the reference simulator (oracle/_ref/fsim_ref, SEQUENTIAL, 1 core)
-    on a bounded sample: a seeded subset of faults over the first rounds."""
+    """Time the oracle's restatement of the reference CPU simulator
+    (oracle_run_literal: a whole-circuit faulty pass per fault and round,
+    fsim/simulation.cpp:104-228) on a bounded sample of the same workload: a
+    seeded subset of faults over the first rounds, on one core and on the
+    host's cores (faults split over threads). Its cost is F x R x N node
+    evaluations whatever the detections, so the sample's rate stands for the
+    whole job."""
     import numpy as np
 
     import oracle
 
-    P = preset["num_pis"]
     lines = flst.split(b"\n")
     F = int(lines[0])
     N = preset["num_pis"] + preset["num_pos"] + preset["num_gates"]
     rounds_total = (preset["num_patterns"] + 63) // 64
-    # The reference re-simulates all N nodes per fault-round (fsim/simulation.cpp:
-    # 186-228); measured at about 4.5e6 nodes/s on one core (2M-node circuit),
-    # so size the sample to about budget_s seconds.
-    fr_budget = max(4, int(budget_s * 4.5e6 / N))
+    # The restatement evaluates about 7e7 nodes/s on one core; size the sample
+    # to about budget_s seconds.
+    fr_budget = max(4, int(budget_s * 7e7 / N))
     n_rounds = max(1, min(rounds_total, 4, fr_budget // 4))
     n_faults = max(1, min(F, fr_budget // n_rounds))
     rng = np.random.default_rng(4242)
@@ -81,27 +95,20 @@ def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: fl
     sample = (f"{n_faults} seeded faults x {n_rounds} rounds ({n_pat} patterns) of the full circuit "
               f"({N} nodes)")
     log(f"[bench] cpu baseline: {sample}")
-    # Host cores to use for the level-parallel mode: the box's CPU share (16 per GPU) at most.
+    # Host cores for the threaded run: the box's CPU share (16 per GPU) at most.
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
-    par = None
-    if oracle.ref_available():
-        _, secs = oracle.run_reference(ckt, sub_flst, sub_ptn, mode=0, threads=1, want_output=False)
-        kind = "reference"
-        # PARALLEL_OMP (fsim/simulation.cpp:544-661): one parallel-for per level.
-        _, psecs = oracle.run_reference(ckt, sub_flst, sub_ptn, mode=2, threads=threads, want_output=False)
-        par = {"value": n_faults * n_pat / psecs, "cores": threads, "mode": "PARALLEL_OMP",
-               "seconds": round(psecs, 3)}
-    else:
-        o = oracle.Oracle(ckt, sub_flst, sub_ptn)
-        t0 = time.perf_counter()
-        o.run_literal(threads=1)
-        secs = time.perf_counter() - t0
-        kind = "port"
-    out = {"value": n_faults * n_pat / secs, "unit": "fault*pattern evals/s", "cores": 1, "kind": kind,
-           "sample": sample, "seconds": round(secs, 3), "mode": "SEQUENTIAL", "cpu": _cpu_model()}
-    if par:
-        out["level_parallel"] = par
-    return out
+    o = oracle.Oracle(ckt, sub_flst, sub_ptn)
+    t0 = time.perf_counter()
+    o.run_literal(threads=1)
+    secs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.run_literal(threads=threads)
+    psecs = time.perf_counter() - t0
+    return {"value": n_faults * n_pat / secs, "unit": "fault*pattern evals/s", "cores": 1, "kind": "port",
+            "sample": sample, "seconds": round(secs, 3), "cpu": _cpu_model(),
+            "algorithm": "restatement of the reference SEQUENTIAL mode (oracle_run_literal)",
+            "threaded": {"value": n_faults * n_pat / psecs, "cores": threads, "seconds": round(psecs, 3),
+                         "split": "faults over OpenMP threads"}}
 
 
 def _cpu_model() -> str:
@@ -391,26 +398,13 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work")
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of one-core CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-random", type=int, default=0, metavar="MIB",
                     help="only run the random-read probe over a MIB MiB buffer and print its figures")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.probe_random:
         print(json.dumps(probe_random(args.probe_random << 20)), flush=True)
-        return 0
-    if args.calibrate_copy:
-        import torch
-
-        n = args.calibrate_copy * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.int64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibrate_copy_bytes": n * 8, "copies": 3}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fault set would shard "

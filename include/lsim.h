@@ -36,17 +36,18 @@ extern "C" {
 
 typedef struct lsim_sim lsim_sim;
 
-/* Engines. All run on the GPU.
+/* Modes of lsim_run. All run on the GPU and give identical results.
+ * The reference's modes (fsim::Simulator::Mode, fsim/simulation.hpp:45)
+ * SEQUENTIAL(0), PARALLEL_TF(1) and PARALLEL_OMP(2) all run the persistent
+ * engine. Two explicit engine choices have values of their own:
  *  PLAIN:        per-level kernel launches (good machine and fault sweeps).
  *  PARTITIONED:  one persistent launch per sweep; each workgroup owns a tile
- *                and walks the levels with workgroup barriers.
- * The reference's CPU modes (fsim/simulation.hpp:45) map onto them:
- * SEQUENTIAL(0) and PARALLEL_OMP(2) -> PLAIN, PARALLEL_TF(1) -> PARTITIONED. */
-#define LSIM_MODE_PLAIN 0
-#define LSIM_MODE_PARTITIONED 1
+ *                and walks the levels with workgroup barriers. */
 #define LSIM_MODE_REF_SEQUENTIAL 0
 #define LSIM_MODE_REF_PARALLEL_TF 1
 #define LSIM_MODE_REF_PARALLEL_OMP 2
+#define LSIM_MODE_PLAIN 16
+#define LSIM_MODE_PARTITIONED 17
 
 /* Create a simulator bound to HIP device `device`.
  * Replaces constructing fsim::Simulator (fsim/simulation.hpp:39). */
@@ -193,6 +194,8 @@ typedef struct {
   int64_t tune_trials;     /* autotuner candidates measured so far for this job set (0: off) */
   int64_t tuned;           /* configuration used: sweep kernel | first step words << 8 |
                               word growth << 16 */
+  int64_t engine;          /* sweep kernel that ran: 0 k_sweep_level (per-level form),
+                              1 k_sweep_pairs, 2 k_sweep_events (persistent forms) */
 } lsim_run_stats;
 LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
 
@@ -334,12 +337,6 @@ LSIM_API int lsim_generate_faults(uint64_t seed, int64_t num_pis, int64_t num_po
                                   double fraction, char **flst, size_t *flst_len);
 LSIM_API int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns, char **ptn,
                                     size_t *ptn_len);
-
-/* Measurement (no reference counterpart): independent random 8-byte reads
- * served per second over a `bytes`-sized device buffer, and the latency of one
- * read in a dependent chain. The fault sweeps' memory traffic is random small
- * records, so bench.py sets their traffic against this rate. */
-LSIM_API int lsim_probe_random_reads(int device, int64_t bytes, double *reads_per_s, double *latency_ns);
 
 #ifdef __cplusplus
 }

@@ -51,7 +51,8 @@ class Pattern {
 
 class Simulator {
  public:
-  // Reference mode names (fsim/simulation.hpp:45); all run on the GPU.
+  // Reference mode names (fsim/simulation.hpp:45); all three run the
+  // persistent GPU engine.
   enum class Mode { SEQUENTIAL = 0, PARALLEL_TF, PARALLEL_OMP };
 
   explicit Simulator(int device = 0) {
@@ -81,6 +82,13 @@ class Simulator {
     table_.assign(2 * faults_.size(), 0);
     const int rc = lsim_found_fault_to_pattern(h_, ptr(table_), table_.size());
     if (rc != LSIM_OK && rc != LSIM_E_STATE) check(rc);
+  }
+
+  // No reference counterpart: timing, traffic and the engine of the last run.
+  lsim_run_stats last_run_stats() const {
+    lsim_run_stats st{};
+    check(lsim_last_run_stats(h_, &st));
+    return st;
   }
 
   const std::vector<size_t> &g_pi_results() const { return g_pi_; }

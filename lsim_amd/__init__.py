@@ -57,6 +57,7 @@ class RunStats(ctypes.Structure):
         ("sweep_reruns", ctypes.c_int64),
         ("tune_trials", ctypes.c_int64),
         ("tuned", ctypes.c_int64),
+        ("engine", ctypes.c_int64),
     ]
 
 
@@ -177,8 +178,6 @@ def lib() -> ctypes.CDLL:
                                               ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
         "lsim_produce_patterns": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]),
-        "lsim_probe_random_reads": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
-                                                   ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,12 +212,13 @@ class Simulator:
     work, run() raises LsimError(LSIM_E_DEVICE)."""
 
     class Mode(enum.IntEnum):
-        # reference names (fsim/simulation.hpp:45) and the engine names
+        # reference names (fsim/simulation.hpp:45): all three run the
+        # persistent engine; the two engine names choose an engine explicitly
         SEQUENTIAL = 0
         PARALLEL_TF = 1
         PARALLEL_OMP = 2
-        PLAIN = 0
-        PARTITIONED = 1
+        PLAIN = 16
+        PARTITIONED = 17
 
     def __init__(self, device: int = 0):
         self._L = lib()
@@ -489,16 +489,6 @@ def convert_timing_dump(dump, mode: str = "bfs"):
                                         ctypes.byref(st))
     text = _take_text(rc, b, n)
     return text, {name: getattr(st, name) for name, _ in IngestStats._fields_}
-
-
-def probe_random_reads(device: int = 0, nbytes: int = 16 << 30) -> dict:
-    """Independent random 8-byte reads the device serves per second over an
-    `nbytes` buffer, and the latency of one read in a dependent chain (ns)."""
-    rate, lat = ctypes.c_double(), ctypes.c_double()
-    rc = lib().lsim_probe_random_reads(device, int(nbytes), ctypes.byref(rate), ctypes.byref(lat))
-    if rc != LSIM_OK:
-        raise LsimError(rc, lib().lsim_last_error(None).decode())
-    return {"reads_per_s": rate.value, "latency_ns": lat.value, "buffer_bytes": int(nbytes)}
 
 
 class Batch:

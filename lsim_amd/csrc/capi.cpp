@@ -20,7 +20,6 @@
 #include "ingest.hpp"
 #include "loops.hpp"
 #include "producers.hpp"
-#include "probe.hpp"
 #include "lsim.h"
 #include "netlist.hpp"
 
@@ -263,8 +262,10 @@ int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
   int m;
   switch (mode) {
     case LSIM_MODE_REF_SEQUENTIAL:
-    case LSIM_MODE_REF_PARALLEL_OMP: m = LSIM_MODE_PLAIN; break;
-    case LSIM_MODE_REF_PARALLEL_TF: m = LSIM_MODE_PARTITIONED; break;
+    case LSIM_MODE_REF_PARALLEL_TF:
+    case LSIM_MODE_REF_PARALLEL_OMP:
+    case LSIM_MODE_PARTITIONED: m = lsim::ENGINE_PERSISTENT; break;
+    case LSIM_MODE_PLAIN: m = lsim::ENGINE_PLAIN; break;
     default: return fail(&s->err, LSIM_E_ARG, "unknown mode " + std::to_string(mode));
   }
   return guarded(&s->err, [&] {
@@ -456,8 +457,8 @@ int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value) {
   int rc = set_engine_option(b->eopt, &b->err, k, value, known);
   if (rc || known) return rc;
   if (k == "mode") {
-    if (value != LSIM_MODE_PLAIN && value != LSIM_MODE_PARTITIONED) return fail(&b->err, LSIM_E_ARG, "mode must be 0 or 1");
-    b->eopt.mode = int(value);
+    if (value != 0 && value != 1) return fail(&b->err, LSIM_E_ARG, "mode must be 0 or 1");
+    b->eopt.mode = value ? lsim::ENGINE_PERSISTENT : lsim::ENGINE_PLAIN;
     return LSIM_OK;
   }
   return fail(&b->err, LSIM_E_ARG, "unknown option " + k);
@@ -787,7 +788,7 @@ int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, ls
     s->lane[1].busy_share = 0.75;
     s->lane[1].max_running = 1;
     for (ServerLane &L : s->lane) {
-      L.eopt.mode = LSIM_MODE_PARTITIONED;
+      L.eopt.mode = lsim::ENGINE_PERSISTENT;
       // Serving jobs rarely repeat a circuit's round count, so a captured
       // good-machine graph would seldom be replayed.
       L.eopt.good_graph = false;
@@ -1251,19 +1252,6 @@ int lsim_generate_patterns(uint64_t seed, int64_t num_pis, int64_t num_patterns,
   std::string text;
   int rc = guarded(&g_create_error, [&] { text = lsim::generate_ptn(seed, num_pis, num_patterns); });
   return rc ? rc : emit_text(text, ptn, ptn_len);
-}
-
-int lsim_probe_random_reads(int device, int64_t bytes, double *reads_per_s, double *latency_ns) {
-  if (!reads_per_s || !latency_ns) return LSIM_E_ARG;
-  try {
-    const lsim::ProbeResult r = lsim::probe_random_reads(device, bytes);
-    *reads_per_s = r.reads_per_s;
-    *latency_ns = r.latency_ns;
-    return LSIM_OK;
-  } catch (const std::exception &e) {
-    g_create_error = e.what();
-    return LSIM_E_DEVICE;
-  }
 }
 
 }  // extern "C"

@@ -1325,10 +1325,12 @@ struct CopySeg {
   int64_t dst, bytes;
 };
 
-// Gathers the segments (dst ascending, non-overlapping) into one block, so
-// that the host copies it back once; bytes between segments stay untouched.
-__global__ void k_gather_bytes(const CopySeg *__restrict__ segs, int nseg, int64_t end, uint8_t *__restrict__ out) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Gathers the segments (dst ascending, non-overlapping, each starting at a
+// multiple of 4 bytes from a 4-byte aligned source) into one block, so that
+// the host copies it back once. One thread per 4-byte word; a segment's last,
+// partial word goes byte by byte. Bytes between segments stay untouched.
+__global__ void k_gather_words(const CopySeg *__restrict__ segs, int nseg, int64_t end, uint8_t *__restrict__ out) {
+  const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
   if (t >= end) return;
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
@@ -1337,7 +1339,12 @@ __global__ void k_gather_bytes(const CopySeg *__restrict__ segs, int nseg, int64
   }
   const CopySeg s = segs[lo];
   const int64_t o = t - s.dst;
-  if (o >= 0 && o < s.bytes) out[t] = s.src[o];
+  if (o < 0 || o >= s.bytes) return;
+  if (o + 4 <= s.bytes) {
+    *reinterpret_cast<uint32_t *>(out + t) = *reinterpret_cast<const uint32_t *>(s.src + o);
+  } else {
+    for (int64_t k = 0; o + k < s.bytes; ++k) out[t + k] = s.src[o + k];
+  }
 }
 
 // Job of entry t of the step's unit list: the largest i with lane_unit0 <= t.
@@ -1983,6 +1990,11 @@ struct Engine::Impl {
   // yet unpacked (finalize_end): the unpack runs while the next sweep does.
   std::vector<JobState *> fin_js;
   std::vector<size_t> fin_at;
+  struct Scatter {
+    int64_t compact_off, host_off, bytes;
+  };
+  std::vector<Scatter> fin_scatter;  // gathered arrays: place in h_stage -> place in h_final
+  HostBuf<uint8_t> h_stage;
   DevBuf<uint8_t> d_final;
   DevBuf<CopySeg> d_segs;
   std::chrono::steady_clock::time_point fin_t0;
@@ -2070,19 +2082,15 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   c.has_mux = nl.has_mux;
   const int64_t N = c.N;
   // Device order: live nodes (a PO is reachable) first, dead nodes after, each
-  // in topological order so every level is a contiguous range in both parts.
+  // part sorted by level and then by topological position, so every level is
+  // a contiguous range in both parts. (FIFO Kahn order is already level-sorted
+  // unless a node reads an unfilled pin as node 0; fanins sit on lower levels,
+  // so the level sort stays topological.)
   c.pos2dp.assign(N, 0);
   c.dp2pos.assign(N, 0);
   int64_t nl_count = 0;
-  for (int64_t p = 0; p < N; ++p)
-    if (fs.reach_po[p]) c.pos2dp[p] = int32_t(nl_count++);
+  for (int64_t p = 0; p < N; ++p) nl_count += fs.reach_po[p] ? 1 : 0;
   c.NL = nl_count;
-  int64_t nd = nl_count;
-  for (int64_t p = 0; p < N; ++p)
-    if (!fs.reach_po[p]) c.pos2dp[p] = int32_t(nd++);
-  for (int64_t p = 0; p < N; ++p) c.dp2pos[c.pos2dp[p]] = int32_t(p);
-  c.level_dp.assign(N, 0);
-  for (int64_t p = 0; p < N; ++p) c.level_dp[c.pos2dp[p]] = nl.level[p];
   c.live_lvl.assign(c.L + 2, 0);
   c.dead_lvl.assign(c.L + 2, 0);
   {
@@ -2094,7 +2102,14 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
       c.live_lvl[l + 1] = int32_t(c.live_lvl[l] + cl[l + 1]);
       c.dead_lvl[l + 1] = int32_t(c.dead_lvl[l] + cd[l + 1]);
     }
+    std::vector<int32_t> at_live(c.live_lvl.begin(), c.live_lvl.end() - 1);
+    std::vector<int32_t> at_dead(c.dead_lvl.begin(), c.dead_lvl.end() - 1);
+    for (int64_t p = 0; p < N; ++p)
+      c.pos2dp[p] = fs.reach_po[p] ? at_live[nl.level[p]]++ : at_dead[nl.level[p]]++;
   }
+  for (int64_t p = 0; p < N; ++p) c.dp2pos[c.pos2dp[p]] = int32_t(p);
+  c.level_dp.assign(N, 0);
+  for (int64_t p = 0; p < N; ++p) c.level_dp[c.pos2dp[p]] = nl.level[p];
   {
     int live_levels = 0;
     for (int l = 0; l <= c.L; ++l) live_levels += c.live_lvl[l + 1] > c.live_lvl[l] ? 1 : 0;
@@ -2686,29 +2701,46 @@ void Engine::Impl::finalize_begin(const std::vector<JobState *> &js) {
     at[i + 1] = at[i] + ((b + 63) & ~size_t(63));
   }
   h_final.alloc(std::max<size_t>(at.back(), 1));
-  // Every array gathered on the device into one block, then one copy back.
+  // Arrays of at least kDirectBytes are copied straight into their place in
+  // h_final. The smaller ones are gathered on the device into one compact
+  // block (d_final), copied back once into h_stage and moved into place by
+  // finalize_end.
+  constexpr int64_t kDirectBytes = int64_t(1) << 20;
   std::vector<CopySeg> segs;
+  fin_scatter.clear();
+  int64_t compact = 0;
+  auto put = [&](const uint8_t *src, size_t host_off, int64_t bytes) {
+    if (bytes <= 0) return;
+    if (bytes >= kDirectBytes) {
+      d2h(h_final.p + host_off, src, size_t(bytes));
+      return;
+    }
+    segs.push_back({src, compact, bytes});
+    fin_scatter.push_back({compact, int64_t(host_off), bytes});
+    compact = (compact + bytes + 7) & ~int64_t(7);
+  };
   for (size_t i = 0; i < js.size(); ++i) {
     const JobState &j = *js[i];
-    int64_t o = int64_t(at[i]);
+    size_t o = at[i];
     const int64_t nf = j.F * int64_t(sizeof(int32_t));
-    if (j.F) segs.push_back({reinterpret_cast<const uint8_t *>(j.first.p), o, nf});
-    o += nf;
-    if (!j.opt.drop && j.detbits.p && j.F) {
+    put(reinterpret_cast<const uint8_t *>(j.first.p), o, nf);
+    o += size_t(nf);
+    if (!j.opt.drop && j.detbits.p) {
       const int64_t nb = j.F * j.RB * int64_t(sizeof(uint64_t));
-      segs.push_back({reinterpret_cast<const uint8_t *>(j.detbits.p), o, nb});
-      o += nb;
+      put(reinterpret_cast<const uint8_t *>(j.detbits.p), o, nb);
+      o += size_t(nb);
     }
-    if (j.opt.drop && j.last.p && j.last_done && j.F) segs.push_back({j.last.p, o, j.F});
+    if (j.opt.drop && j.last.p && j.last_done) put(j.last.p, o, j.F);
   }
   if (!segs.empty()) {
-    const int64_t end = segs.back().dst + segs.back().bytes;
-    d_final.alloc(size_t(end));
+    d_final.alloc(size_t(compact));
     d_segs.alloc(segs.size());
+    h_stage.alloc(size_t(compact));
     h2d(d_segs.p, segs.data(), segs.size() * sizeof(CopySeg));
-    k_gather_bytes<<<blocks_for(end, 256), 256, 0, stream>>>(d_segs.p, int(segs.size()), end, d_final.p);
+    k_gather_words<<<blocks_for((compact + 3) / 4, 256), 256, 0, stream>>>(d_segs.p, int(segs.size()), compact,
+                                                                          d_final.p);
     LSIM_HIP(hipGetLastError());
-    d2h(h_final.p, d_final.p, size_t(end));
+    d2h(h_stage.p, d_final.p, size_t(compact));
   }
   LSIM_HIP(hipEventRecord(fin_ev, stream));
   fin_js = js;
@@ -2722,6 +2754,12 @@ void Engine::Impl::finalize_end() {
   const std::vector<size_t> &at = fin_at;
   const auto t0 = clk::now();
   LSIM_HIP(hipEventSynchronize(fin_ev));
+  for (const Scatter &sc : fin_scatter)
+    std::memcpy(h_final.p + sc.host_off, h_stage.p + sc.compact_off, size_t(sc.bytes));
+  fin_scatter.clear();
+  // The gather block is sized by the small arrays of one finalize; a block
+  // left large by an unusual batch is given back.
+  if (d_final.bytes() > (size_t(64) << 20)) d_final.release();
   const auto t1 = clk::now();
   // Unpacked in parallel, largest job first. A job of many faults is split
   // into contiguous fault ranges, so that several threads touch and fill its
@@ -3175,7 +3213,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   });
   const auto h0u = clk::now();
   std::vector<int64_t> lane_unit_off(nj + 1, 0);  // units of job i in the step's unit list
-  const bool plain = eo.mode == LSIM_MODE_PLAIN;
+  const bool plain = eo.mode == ENGINE_PLAIN;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
     const CircuitRes &c = *j.c;
@@ -3512,7 +3550,7 @@ int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const En
   to_finalize.clear();
   try {
     if (!act.empty()) {
-      if (eo.mode == LSIM_MODE_PLAIN) {
+      if (eo.mode == ENGINE_PLAIN) {
         for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
       } else {
         step_batch(act, eo);
@@ -3521,7 +3559,11 @@ int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const En
     finalize_end();
   } catch (...) {
     // Jobs left unpacked stay unfinished: the next plan() queues them again.
+    // Their result copies may still be in flight into h_final / h_stage:
+    // wait for them before the buffers can be reused.
+    if (!fin_js.empty()) (void)hipEventSynchronize(fin_ev);
     fin_js.clear();
+    fin_scatter.clear();
     throw;
   }
   int64_t left = 0;
@@ -3557,7 +3599,7 @@ Engine::Impl::TuneState &Engine::Impl::tune_state(const std::string &key, const 
   auto it = tuned.find(key);
   if (it != tuned.end()) return it->second;
   TuneState st;
-  const int nk = eo.mode == LSIM_MODE_PLAIN ? 1 : 2;
+  const int nk = eo.mode == ENGINE_PLAIN ? 1 : 2;
   const int sched[4][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}};
   for (int k = 0; k < nk; ++k)
     for (int s = 0; s < (any_drop ? 4 : 1); ++s) {
@@ -3614,6 +3656,7 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
     j->res.stats.t_total_ms = ms;
     j->res.stats.tune_trials = ts ? ts->tried() : 0;
     j->res.stats.tuned = int64_t(e2.sweep_kernel) | (int64_t(first) << 8) | (int64_t(growth) << 16);
+    j->res.stats.engine = e2.mode == ENGINE_PLAIN ? 0 : (e2.sweep_kernel == 0 ? 1 : 2);
   }
 }
 

@@ -59,9 +59,15 @@ struct JobOptions {
   int word_growth = 2;            // with dropping: factor of words per step after each step (up to 64)
 };
 
+// Sweep engines.
+enum EngineKind : int {
+  ENGINE_PLAIN = 0,       // per-level launches (k_sweep_level)
+  ENGINE_PERSISTENT = 1,  // persistent sweeps (k_sweep_pairs / k_sweep_events)
+};
+
 // Engine-wide options of a run / iteration.
 struct EngineOptions {
-  int mode = LSIM_MODE_PARTITIONED;  // PLAIN (per-level launches) or PARTITIONED (persistent events)
+  int mode = ENGINE_PERSISTENT;      // ENGINE_PLAIN or ENGINE_PERSISTENT
   int64_t scratch_mb = 0;            // sweep scratch budget, 0 = automatic
   int block_waves = 8;               // waves per workgroup of the node-wave persistent sweep (1..8)
   int sweep_kernel = 0;              // persistent sweep: 0 = flattened (node, lane) pairs, 1 = one node per wave

@@ -6,11 +6,19 @@
 // id order, fanins in pin order with pin 5 (MUX select) stored as pin 3, levels
 // = 1 + max fanin level with every PO one level above the deepest gate.
 //
+// Pins follow the reference's zero-filled pin table (fsim/simulation.cpp:676-711,
+// 851-877): a node with k wires reads pins 1..k, a pin slot no wire filled reads
+// node id 0 (PI 0), a second wire into one pin overwrites the first, and wires
+// into pins above k are not read (they still order the node in Kahn's queue).
+// The levels follow the pins actually read, so they need not be sorted in
+// topological order.
+//
 // Inputs on which the reference has undefined behaviour are rejected with an
-// InputError instead of being reproduced: PI->PO edges, PO fanout, fanin into a
-// PI, >4 fanins, duplicate or non-dense pins, gates or POs without fanin,
-// unreachable or cyclic nodes, unknown gate types, fault ids >= N, more rounds
-// than PIs, and truncated or non-numeric text.
+// InputError instead of being reproduced: PI->PO edges, a PO reading an unfilled
+// pin (the reference then indexes its gate array with a PI), PO fanout, fanin
+// into a PI, >4 fanins, gates or POs without fanin, unreachable or cyclic nodes,
+// unknown gate types, fault ids >= N, more rounds than PIs, and truncated or
+// non-numeric text.
 #include "netlist.hpp"
 
 #include <algorithm>
@@ -65,9 +73,7 @@ void read_circuit(Netlist &nl, const char *ckt, size_t ckt_len) {
       if (src < uint64_t(P) && is_po(int64_t(dst))) throw InputError("PI drives a PO directly");
       if (pin < 1 || pin > 5) throw InputError("input pin id must be 1..5");
       if (pin == 5) pin = 3;
-      int64_t &slot = table[size_t(dst) * 4 + (pin - 1)];
-      if (slot != -1) throw InputError("two wires drive the same input pin");
-      slot = int64_t(src);
+      table[size_t(dst) * 4 + (pin - 1)] = int64_t(src);
       if (++pin_count[dst] > 4) throw InputError("more than 4 fanins");
       const int64_t e = int64_t(adj_dst.size());
       adj_dst.push_back(int64_t(dst));
@@ -124,8 +130,12 @@ void read_circuit(Netlist &nl, const char *ckt, size_t ckt_len) {
     if (p >= P) {
       if (cnt == 0) throw InputError(is_po(id) ? "PO without fanin" : "gate without fanin");
       for (int j = 0; j < cnt; ++j) {
-        const int64_t src = table[size_t(id) * 4 + j];
-        if (src < 0) throw InputError("input pins of a node are not dense 1..k");
+        int64_t src = table[size_t(id) * 4 + j];
+        if (src < 0) {
+          // Unfilled pin slot: node id 0 (fsim/simulation.cpp:678-680,856-861).
+          if (is_po(id)) throw InputError("a PO reads an input pin no wire drives");
+          src = 0;
+        }
         nl.fin.push_back(int32_t(nl.order_idx[src]));
       }
       if (nl.type[p] == G_MUX) {
@@ -149,10 +159,6 @@ void read_circuit(Netlist &nl, const char *ckt, size_t ckt_len) {
   for (int64_t p = P + nl.G; p < N; ++p) nl.level[p] = nl.max_level;
   nl.gates_per_level.assign(size_t(nl.max_level) + 1, 0);
   for (int64_t p = 0; p < N; ++p) nl.gates_per_level[nl.level[p]]++;
-  // FIFO Kahn from level-0 sources emits nodes in non-decreasing level order;
-  // the device layout relies on it (each level is one contiguous range).
-  for (int64_t p = 1; p < N; ++p)
-    if (nl.level[p] < nl.level[p - 1]) throw InputError("topological order is not level-sorted");
 
 }
 

@@ -110,3 +110,19 @@ ProbeResult probe_random_reads(int device, int64_t bytes) {
 }
 
 }  // namespace lsim
+
+// Entry point of the measurement library liblsim_probe.so, which only the
+// benchmark loads; the simulator library does not contain the probe.
+extern "C" __attribute__((visibility("default"))) int lsim_bench_probe_random_reads(int device, int64_t bytes,
+                                                                                 double *reads_per_s,
+                                                                                 double *latency_ns) {
+  if (!reads_per_s || !latency_ns) return -1;
+  try {
+    const lsim::ProbeResult r = lsim::probe_random_reads(device, bytes);
+    *reads_per_s = r.reads_per_s;
+    *latency_ns = r.latency_ns;
+    return 0;
+  } catch (const std::exception &) {
+    return -5;
+  }
+}

@@ -1,4 +1,7 @@
-// Random-access probe of the device's memory system.
+// Random-access probe of the device's memory system (measurement only: built
+// into liblsim_probe.so, loaded by the benchmark, not part of liblsim.so).
+// C entry point: int lsim_bench_probe_random_reads(int device, int64_t bytes,
+// double *reads_per_s, double *latency_ns), 0 on success.
 #pragma once
 #include <cstdint>
 #include <stdexcept>

@@ -54,6 +54,7 @@ int main(int argc, char **argv) {
       line("gate", sim.g_gate_results());
       line("po", sim.g_po_results());
       line("table", sim.found_fault_to_pattern());
+      std::cout << "engine " << sim.last_run_stats().engine << '\n';
     }
   } catch (const std::exception &e) {
     std::cout << "error " << e.what() << '\n';

@@ -5,7 +5,10 @@ sources by `make -C oracle ref`) on small seeded circuits that exercise what
 the reference's own unit tests leave open: the MUX word-level `!s`, 3- and
 4-input folds, several rounds, a partial last round whose words carry set bits
 above the pattern count, PI and PO fault sites, duplicate fanins, shuffled
-wire order (Kahn tie order), zero patterns and an empty fault list.
+wire order (Kahn tie order), zero patterns, an empty fault list, and gates
+whose pins are not dense 1..k (an unfilled slot reads node id 0, a pin wired
+twice keeps the later wire; the levels then need not ascend in topological
+order).
 
 Per-round detection d(f, r) is read from the reference by running it once per
 round on a pattern file holding only that round: its report then holds
@@ -29,8 +32,13 @@ import oracle  # noqa: E402
 INV, AND, OR, XOR, NAND, NOR, XNOR, MUX, CLKBUF = range(9)
 
 
-def rand_circuit(rng: random.Random, P: int, O: int, G: int, mux: float, wide: float, dup: float) -> str:
-    """Random levelized netlist in .ckt text with shuffled ids and wire order."""
+def rand_circuit(rng: random.Random, P: int, O: int, G: int, mux: float, wide: float, dup: float,
+                 gap: float = 0.0) -> str:
+    """Random levelized netlist in .ckt text with shuffled ids and wire order.
+    With `gap` > 0 a gate's pins are, with that probability, a non-dense set
+    (a pin slot left unfilled, which the reference reads as node id 0, and pins
+    above the wire count, which it does not read) or repeat a pin (the later
+    wire overwrites the earlier one) (fsim/simulation.cpp:676-711,851-877)."""
     N = P + O + G
     gate_ids = list(range(P + O, N))
     rng.shuffle(gate_ids)  # creation order != id order
@@ -54,6 +62,11 @@ def rand_circuit(rng: random.Random, P: int, O: int, G: int, mux: float, wide: f
                 # mostly recent nodes, sometimes anything earlier
                 srcs.append(avail[-1 - rng.randrange(min(len(avail), 6))] if rng.random() < 0.7 else rng.choice(avail))
         pins = [1, 2, 5] if t == MUX else list(range(1, k + 1))
+        if gap and rng.random() < gap:
+            if rng.random() < 0.25:
+                pins[rng.randrange(k)] = pins[rng.randrange(k)]  # a pin wired twice
+            else:
+                pins = sorted(rng.sample([1, 2, 3, 4], k)) if t != MUX else rng.choice([[2, 4, 5], [1, 4, 5], [2, 1, 4]])
         fanins[gid] = list(zip(srcs, pins))
         types[gid] = t
         avail.append(gid)
@@ -103,9 +116,9 @@ def ptn_text(n: int, rounds: list) -> str:
 
 
 def make_case(name: str, seed: int, P: int, O: int, G: int, F: int, n: int, mux=0.0, wide=0.0, dup=0.0,
-              garbage=False) -> dict:
+              garbage=False, gap=0.0) -> dict:
     rng = random.Random(seed)
-    ckt = rand_circuit(rng, P, O, G, mux, wide, dup)
+    ckt = rand_circuit(rng, P, O, G, mux, wide, dup, gap)
     flst = rand_faults(rng, P + O + G, F)
     rounds = rand_patterns(rng, P, n, garbage)
     ptn = ptn_text(n, rounds)
@@ -144,6 +157,9 @@ CASES = [
     ("empty_fault_list", 21, 4, 2, 30, 0, 128, 0.0, 0.0, 0.0, False),
     ("zero_patterns", 22, 4, 2, 30, 10, 0, 0.0, 0.0, 0.0, False),
     ("larger_mixed", 23, 16, 8, 400, 300, 5 * 64 + 50, 0.05, 0.2, 0.05, True),
+    # name, seed, P, O, G, F, n, mux, wide, dup, garbage, gap
+    ("nondense_pins", 24, 8, 3, 80, 80, 3 * 64 + 11, 0.0, 0.3, 0.0, False, 0.35),
+    ("nondense_pins_mux", 25, 8, 3, 90, 90, 2 * 64, 0.2, 0.3, 0.05, True, 0.3),
 ]
 
 

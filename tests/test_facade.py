@@ -75,3 +75,5 @@ def test_facade_run(unit_vectors, tmp_path):
         _check_structure(c, out)
         assert out["pi"] == c["pi"] and out["gate"] == c["gate"] and out["po"] == c["po"]
         assert out["table"] == c["table"]
+        # the reference's SEQUENTIAL mode runs the persistent pair sweep (k_sweep_pairs)
+        assert out["engine"] == [1], c["name"]

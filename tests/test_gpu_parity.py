@@ -91,6 +91,24 @@ def test_c1_full_against_oracle(c1, mode, direct):
     assert np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
 
 
+def test_reference_modes_run_persistent_engine(c1):
+    """The reference's three modes (fsim/simulation.hpp:45) all run the
+    persistent pair sweep and give identical, oracle-exact results; the
+    per-level form runs only when asked for by its own mode value."""
+    ckt, flst, ptn, ref = c1
+    M = lsim_amd.Simulator.Mode
+    for mode in (M.SEQUENTIAL, M.PARALLEL_TF, M.PARALLEL_OMP, M.PARTITIONED, M.PLAIN):
+        s = _sim(ckt, flst, ptn)
+        s.run(mode, 8)
+        assert s.last_run_stats()["engine"] == (0 if mode == M.PLAIN else 1), mode
+        assert np.array_equal(s.detect_matrix(), ref["det"]), mode
+        assert np.array_equal(s.found_fault_to_pattern(), ref["table"]), mode
+        s.close()
+    s = _sim(ckt, flst, ptn)
+    with pytest.raises(lsim_amd.LsimError):
+        s.run(3)
+
+
 @pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 def test_c1_fault_dropping(c1, mode):
     """With dropping, the first detecting round and the detected set equal the full matrix's."""
