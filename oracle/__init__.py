@@ -53,6 +53,11 @@ def lib() -> ctypes.CDLL:
         L.oracle_run_literal.argtypes = [vp, ctypes.c_int, i64p, ctypes.c_int64, u64p, u8p, u64p, u64p, u64p]
         L.oracle_run_event.restype = ctypes.c_int
         L.oracle_run_event.argtypes = [vp, ctypes.c_int, i64p, ctypes.c_int64, u8p]
+        L.oracle_run_event_range.restype = ctypes.c_int
+        L.oracle_run_event_range.argtypes = [vp, ctypes.c_int, i64p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.c_int64, ctypes.c_int, u8p]
+        L.oracle_good_round.restype = ctypes.c_int
+        L.oracle_good_round.argtypes = [vp, ctypes.c_int64, u64p, u64p, u64p]
         _lib = L
     return _lib
 
@@ -158,8 +163,39 @@ class Oracle:
         if faults is not None:
             arr = np.ascontiguousarray(np.asarray(faults, dtype=np.int64))
             sel, nsel = _p(arr, ctypes.c_int64), arr.size
-        self._L.oracle_run_event(self._h, threads, sel, nsel, _p(det, ctypes.c_uint8))
+        if self._L.oracle_run_event(self._h, threads, sel, nsel, _p(det, ctypes.c_uint8)) != 0:
+            raise MemoryError("oracle_run_event failed")
         return det.reshape(F, R)
+
+    def run_event_range(self, faults: Sequence[int], r0: int = 0, r1: Optional[int] = None, chunk: int = 64,
+                        stop_at_first: bool = False, threads: int = 0) -> np.ndarray:
+        """Event-driven restatement over rounds [r0, r1) for the given faults,
+        the good machine `chunk` rounds at a time. Returns det[len(faults),
+        r1 - r0]; with stop_at_first a fault's rounds after its first
+        detection stay 0."""
+        r1 = self.R if r1 is None else r1
+        arr = np.ascontiguousarray(np.asarray(faults, dtype=np.int64))
+        det = np.zeros(max(1, arr.size * (r1 - r0)), dtype=np.uint8)
+        rc = self._L.oracle_run_event_range(self._h, threads, _p(arr, ctypes.c_int64), arr.size, r0, r1, chunk,
+                                            int(stop_at_first), _p(det, ctypes.c_uint8))
+        if rc != 0:
+            raise ValueError("oracle_run_event_range failed (bad range or out of memory)")
+        return det[:arr.size * (r1 - r0)].reshape(arr.size, r1 - r0)
+
+    def first_rounds(self, faults: Sequence[int], chunk: int = 64, threads: int = 0) -> np.ndarray:
+        """First detecting round of each given fault over all rounds, -1 if
+        never (event-driven, each fault simulated up to its first detection)."""
+        det = self.run_event_range(faults, 0, self.R, chunk=chunk, stop_at_first=True, threads=threads)
+        if det.shape[1] == 0:
+            return -np.ones(det.shape[0], dtype=np.int64)
+        return np.where(det.any(1), det.argmax(1), -1).astype(np.int64)
+
+    def good_round(self, rd: int):
+        """Masked good values (g_pi, g_gate, g_po) of round rd alone."""
+        g = [np.zeros(n, dtype=np.uint64) for n in (self.P, self.G, self.O)]
+        if self._L.oracle_good_round(self._h, rd, *(_p(x, ctypes.c_uint64) for x in g)) != 0:
+            raise ValueError("round out of range")
+        return tuple(g)
 
 
 def ref_available() -> bool:

@@ -492,88 +492,160 @@ static int64_t heap_pop(heap_t *h) {
   return top;
 }
 
-int oracle_run_event(const oracle_sim *s, int threads, const int64_t *fault_idx, int64_t num_sel, uint8_t *det) {
-  const int64_t N = s->N, R = s->R, F = s->F;
-  const int64_t nsel = fault_idx ? num_sel : F;
+/* Good machine of one round (fsim/simulation.cpp:149-184): the values of a
+ * round depend on that round's pattern words only, so one pass gives them. */
+static void good_round(const oracle_sim *s, int64_t rd, uint64_t *g) {
+  const uint64_t *pat = s->patterns + rd * s->P;
+  for (int64_t p = 0; p < s->N; p++) g[p] = eval_node(s, p, g, pat);
+}
+
+int oracle_good_round(const oracle_sim *s, int64_t rd, uint64_t *g_pi, uint64_t *g_gate, uint64_t *g_po) {
+  if (rd < 0 || rd >= s->R) return -1;
+  uint64_t *g = malloc((size_t)(s->N ? s->N : 1) * sizeof(uint64_t));
+  if (!g) return -1;
+  good_round(s, rd, g);
+  const uint64_t mask = round_mask(s, rd);
+  if (g_pi)
+    for (int64_t i = 0; i < s->P; i++) g_pi[i] = g[i] & mask;
+  if (g_gate)
+    for (int64_t i = 0; i < s->G; i++) g_gate[i] = g[s->P + i] & mask;
+  if (g_po)
+    for (int64_t i = 0; i < s->O; i++) g_po[i] = g[s->P + s->G + i] & mask;
+  free(g);
+  return 0;
+}
+
+/* One (fault, round) of the event-driven restatement: the faulty machine
+ * differs from the good one only downstream of the site, and a node whose
+ * fanins all hold good values keeps its good value. Nodes are evaluated in
+ * position (topological) order from a min-heap. Returns d(f, rd). */
+typedef struct {
+  uint64_t *val;
+  int64_t *stamp, *queued;
+  heap_t h;
+  int64_t epoch;
+} event_ws;
+
+static int event_one(const oracle_sim *s, event_ws *w, int64_t f, const uint64_t *g, uint64_t mask) {
   const int64_t po0 = s->P + s->G;
-  uint64_t *good = malloc((size_t)(N ? N : 1) * (size_t)(R ? R : 1) * sizeof(uint64_t)); /* [R][N] */
-  for (int64_t rd = 0; rd < R; rd++) {
-    uint64_t *g = good + rd * N;
-    for (int64_t p = 0; p < N; p++) g[p] = eval_node(s, p, g, s->patterns + rd * s->P);
+  const int64_t site = s->fault_site[f];
+  const int64_t ep = w->epoch++;
+  int found = 0;
+  w->h.n = 0;
+  w->val[site] = s->fault_val[f];
+  w->stamp[site] = ep;
+  if (w->val[site] != g[site]) {
+    if (site >= po0 && ((w->val[site] ^ g[site]) & mask)) found = 1;
+    for (int64_t e = s->out_off[site]; e < s->out_off[site + 1]; e++)
+      if (w->queued[s->out[e]] != ep) {
+        w->queued[s->out[e]] = ep;
+        heap_push(&w->h, s->out[e]);
+      }
+  }
+  while (w->h.n) {
+    const int64_t p = heap_pop(&w->h);
+    if (p == site) continue;
+    /* fanin values: faulty where stamped, good otherwise */
+    const int64_t a = s->in_off[p], b = s->in_off[p + 1];
+    uint64_t loc[4];
+    for (int64_t e = a; e < b; e++) loc[e - a] = (w->stamp[s->in[e]] == ep) ? w->val[s->in[e]] : g[s->in[e]];
+    uint64_t r = loc[0];
+    switch (s->type[p]) {
+      case T_INV: r = ~r; break;
+      case T_AND: for (int64_t j = 1; j < b - a; j++) r &= loc[j]; break;
+      case T_OR: for (int64_t j = 1; j < b - a; j++) r |= loc[j]; break;
+      case T_XOR: for (int64_t j = 1; j < b - a; j++) r ^= loc[j]; break;
+      case T_NAND: for (int64_t j = 1; j < b - a; j++) r &= loc[j]; r = ~r; break;
+      case T_NOR: for (int64_t j = 1; j < b - a; j++) r |= loc[j]; r = ~r; break;
+      case T_XNOR: for (int64_t j = 1; j < b - a; j++) r ^= loc[j]; r = ~r; break;
+      case T_MUX: r = (loc[2] & loc[1]) | (loc[0] & (uint64_t)(!loc[2])); break;
+      default: break; /* CLKBUF / PO copy input 0 */
+    }
+    if (r == g[p]) continue;
+    w->val[p] = r;
+    w->stamp[p] = ep;
+    if (p >= po0 && ((r ^ g[p]) & mask)) found = 1;
+    for (int64_t e = s->out_off[p]; e < s->out_off[p + 1]; e++)
+      if (w->queued[s->out[e]] != ep) {
+        w->queued[s->out[e]] = ep;
+        heap_push(&w->h, s->out[e]);
+      }
+  }
+  return found;
+}
+
+int oracle_run_event_range(const oracle_sim *s, int threads, const int64_t *fault_idx, int64_t num_sel, int64_t r0,
+                           int64_t r1, int64_t chunk, int stop_at_first, uint8_t *det) {
+  const int64_t N = s->N, F = s->F;
+  const int64_t nsel = fault_idx ? num_sel : F;
+  if (r0 < 0 || r1 > s->R || r0 > r1) return -1;
+  const int64_t nr = r1 - r0;
+  if (chunk < 1) chunk = 1;
+  if (chunk > nr) chunk = nr > 0 ? nr : 1;
+  memset(det, 0, (size_t)(nsel * nr));
+  if (nsel == 0 || nr == 0) return 0;
+  uint64_t *good = malloc((size_t)(N ? N : 1) * (size_t)chunk * sizeof(uint64_t)); /* [chunk][N] */
+  uint8_t *done = calloc((size_t)nsel, 1);
+  if (!good || !done) {
+    free(good);
+    free(done);
+    return -1;
   }
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #else
   (void)threads;
 #endif
+  for (int64_t c0 = r0; c0 < r1; c0 += chunk) {
+    const int64_t c1 = c0 + chunk < r1 ? c0 + chunk : r1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t rd = c0; rd < c1; rd++) good_round(s, rd, good + (rd - c0) * N);
 #pragma omp parallel
-  {
-    uint64_t *val = malloc((size_t)N * sizeof(uint64_t));
-    int64_t *stamp = malloc((size_t)N * sizeof(int64_t)); /* value valid for this (fault, round) */
-    int64_t *queued = malloc((size_t)N * sizeof(int64_t));
-    heap_t h;
-    h.a = malloc((size_t)(s->out_off[N] + 2) * sizeof(int64_t));
-    for (int64_t p = 0; p < N; p++) stamp[p] = queued[p] = -1;
-    int64_t epoch = 0;
-#pragma omp for schedule(dynamic, 4)
-    for (int64_t k = 0; k < nsel; k++) {
-      const int64_t f = fault_idx ? fault_idx[k] : k;
-      const int64_t site = s->fault_site[f];
-      for (int64_t rd = 0; rd < R; rd++) {
-        const uint64_t *g = good + rd * N;
-        const uint64_t mask = round_mask(s, rd);
-        const int64_t ep = epoch++;
-        uint8_t found = 0;
-        h.n = 0;
-        /* the faulty machine differs from the good one only downstream of the
-         * site; a node whose fanins all hold good values keeps its good value */
-        val[site] = s->fault_val[f];
-        stamp[site] = ep;
-        if (val[site] != g[site]) {
-          if (site >= po0 && ((val[site] ^ g[site]) & mask)) found = 1;
-          for (int64_t e = s->out_off[site]; e < s->out_off[site + 1]; e++)
-            if (queued[s->out[e]] != ep) {
-              queued[s->out[e]] = ep;
-              heap_push(&h, s->out[e]);
-            }
-        }
-        while (h.n) {
-          const int64_t p = heap_pop(&h);
-          if (p == site) continue;
-          /* gather fanin values: faulty where stamped, good otherwise */
-          const int64_t a = s->in_off[p], b = s->in_off[p + 1];
-          uint64_t loc[4];
-          for (int64_t e = a; e < b; e++) loc[e - a] = (stamp[s->in[e]] == ep) ? val[s->in[e]] : g[s->in[e]];
-          uint64_t r = loc[0];
-          switch (s->type[p]) {
-            case T_INV: r = ~r; break;
-            case T_AND: for (int64_t j = 1; j < b - a; j++) r &= loc[j]; break;
-            case T_OR: for (int64_t j = 1; j < b - a; j++) r |= loc[j]; break;
-            case T_XOR: for (int64_t j = 1; j < b - a; j++) r ^= loc[j]; break;
-            case T_NAND: for (int64_t j = 1; j < b - a; j++) r &= loc[j]; r = ~r; break;
-            case T_NOR: for (int64_t j = 1; j < b - a; j++) r |= loc[j]; r = ~r; break;
-            case T_XNOR: for (int64_t j = 1; j < b - a; j++) r ^= loc[j]; r = ~r; break;
-            case T_MUX: r = (loc[2] & loc[1]) | (loc[0] & (uint64_t)(!loc[2])); break;
-            default: break; /* CLKBUF / PO copy input 0 */
+    {
+      event_ws w;
+      w.val = malloc((size_t)N * sizeof(uint64_t));
+      w.stamp = malloc((size_t)N * sizeof(int64_t));
+      w.queued = malloc((size_t)N * sizeof(int64_t));
+      w.h.a = malloc((size_t)(s->out_off[N] + 2) * sizeof(int64_t));
+      w.epoch = 0;
+      for (int64_t p = 0; p < N; p++) w.stamp[p] = w.queued[p] = -1;
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t k = 0; k < nsel; k++) {
+        if (done[k]) continue;
+        const int64_t f = fault_idx ? fault_idx[k] : k;
+        for (int64_t rd = c0; rd < c1; rd++) {
+          const int d = event_one(s, &w, f, good + (rd - c0) * N, round_mask(s, rd));
+          det[k * nr + (rd - r0)] = (uint8_t)d;
+          if (d && stop_at_first) {
+            done[k] = 1;
+            break;
           }
-          if (r == g[p]) continue;
-          val[p] = r;
-          stamp[p] = ep;
-          if (p >= po0 && ((r ^ g[p]) & mask)) found = 1;
-          for (int64_t e = s->out_off[p]; e < s->out_off[p + 1]; e++)
-            if (queued[s->out[e]] != ep) {
-              queued[s->out[e]] = ep;
-              heap_push(&h, s->out[e]);
-            }
         }
-        det[f * R + rd] = found;
       }
+      free(w.val);
+      free(w.stamp);
+      free(w.queued);
+      free(w.h.a);
     }
-    free(val);
-    free(stamp);
-    free(queued);
-    free(h.a);
   }
   free(good);
+  free(done);
   return 0;
+}
+
+int oracle_run_event(const oracle_sim *s, int threads, const int64_t *fault_idx, int64_t num_sel, uint8_t *det) {
+  /* all rounds in chunks of 64; rows of the selected faults land at their
+   * fault index */
+  const int64_t R = s->R, F = s->F;
+  const int64_t nsel = fault_idx ? num_sel : F;
+  uint8_t *rows = malloc((size_t)(nsel * R ? nsel * R : 1));
+  if (!rows) return -1;
+  int rc = oracle_run_event_range(s, threads, fault_idx, num_sel, 0, R, 64, 0, rows);
+  if (rc == 0)
+    for (int64_t k = 0; k < nsel; k++) {
+      const int64_t f = fault_idx ? fault_idx[k] : k;
+      memcpy(det + f * R, rows + k * R, (size_t)R);
+    }
+  free(rows);
+  return rc;
 }

@@ -58,9 +58,28 @@ int oracle_run_literal(const oracle_sim *s, int threads, const int64_t *fault_id
 
 /* Event-driven single-fault restatement of the same detection function:
  * per (fault, round) only the nodes downstream of the fault site whose fanins
- * changed are re-evaluated. Same outputs as oracle_run_literal for `det`. */
+ * changed are re-evaluated. Same outputs as oracle_run_literal for `det`
+ * (F x R bytes, rows of unselected faults untouched). Runs through
+ * oracle_run_event_range over all rounds. Returns 0, -1 on allocation failure. */
 int oracle_run_event(const oracle_sim *s, int threads, const int64_t *fault_idx, int64_t num_sel,
                      uint8_t *det);
+
+/* The same over rounds [r0, r1), for large circuits: the good machine is
+ * evaluated `chunk` rounds at a time (N x chunk words of memory, not N x R),
+ * rounds of a chunk in parallel, then the faults of the chunk in parallel.
+ * det holds num_sel x (r1 - r0) bytes, det[k * (r1 - r0) + r - r0] =
+ * d(fault_idx[k], r) (fault_idx NULL: all F faults). With stop_at_first a
+ * fault's rounds after its first detection are not simulated and stay 0, so
+ * the first detecting round of each fault costs only the rounds up to it.
+ * Returns 0, -1 on a bad range or allocation failure. */
+int oracle_run_event_range(const oracle_sim *s, int threads, const int64_t *fault_idx, int64_t num_sel,
+                           int64_t r0, int64_t r1, int64_t chunk, int stop_at_first, uint8_t *det);
+
+/* Masked good values of round rd alone (fsim/simulation.cpp:149-184): a
+ * round's good machine reads only that round's pattern words, so this equals
+ * the last-round values of oracle_run_literal for rd = R - 1. Returns 0, -1
+ * when rd is out of range. */
+int oracle_good_round(const oracle_sim *s, int64_t rd, uint64_t *g_pi, uint64_t *g_gate, uint64_t *g_po);
 
 #ifdef __cplusplus
 }

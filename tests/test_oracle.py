@@ -169,3 +169,48 @@ def test_subset_run_matches_full():
     part = o.run_literal(threads=2, faults=sel)["det"]
     ev = o.run_event(threads=2, faults=sel)
     assert np.array_equal(part[sel], full[sel]) and np.array_equal(ev[sel], full[sel])
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 64])
+def test_event_range_against_reference_made_vectors(ref_vectors, chunk):
+    """The chunked event-driven restatement (the good machine a few rounds at
+    a time) against the compiled reference's per-round detections, over the
+    full round range and over an inner sub-range; with stop_at_first the
+    first detecting round is the reference's."""
+    for c in ref_vectors:
+        o = oracle.Oracle(c["ckt"], c["flst"], c["ptn"])
+        F, R = o.F, o.R
+        det = np.array(c["det"], dtype=np.uint8).reshape(F, R)
+        sel = np.arange(F)[::-1]  # selection order is kept in the output rows
+        got = o.run_event_range(sel, 0, R, chunk=chunk, threads=2)
+        assert np.array_equal(got, det[sel]), c["name"]
+        if R >= 3:
+            assert np.array_equal(o.run_event_range(sel, 1, R - 1, chunk=chunk, threads=2), det[sel, 1:R - 1])
+        first = np.where(det.any(1), det.argmax(1), -1) if R else -np.ones(F, dtype=np.int64)
+        assert np.array_equal(o.first_rounds(sel, chunk=chunk, threads=2), first[sel]), c["name"]
+
+
+def test_good_round_is_last_round_of_the_reference(unit_vectors, ref_vectors):
+    """A round's good machine alone equals the reference's last-round good
+    values (the accessors hold the last round, fsim/simulation.cpp:163-178)."""
+    for c in list(unit_vectors) + list(ref_vectors):
+        o = oracle.Oracle(c["ckt"], c["flst"], c["ptn"])
+        if o.R == 0:
+            continue
+        g_pi, g_gate, g_po = o.good_round(o.R - 1)
+        want = (c["pi"], c["gate"], c["po"]) if "pi" in c else (c["g_pi"], c["g_gate"], c["g_po"])
+        assert (list(g_pi), list(g_gate), list(g_po)) == tuple(list(w) for w in want), c["name"]
+
+
+def test_event_range_matches_literal_on_generated():
+    """A generated circuit with heavy-tailed fanout: chunked event-driven rows
+    over sub-ranges equal the literal restatement, and earlier rounds of a
+    stop_at_first run stay exact up to the first detection."""
+    ckt, flst, ptn = lsim_amd.generate(seed=77, num_pis=32, num_pos=16, num_gates=3000, num_levels=40,
+                                       num_patterns=64 * 9 + 5, fanout_cap=60)
+    o = oracle.Oracle(ckt, flst, ptn)
+    lit = o.run_literal(threads=4)["det"]
+    sel = np.arange(0, o.F, 7)
+    assert np.array_equal(o.run_event_range(sel, 2, 9, chunk=4, threads=4), lit[sel, 2:9])
+    first = np.where(lit.any(1), lit.argmax(1), -1)
+    assert np.array_equal(o.first_rounds(sel, chunk=2, threads=4), first[sel])
