@@ -535,6 +535,7 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.tune_trials = s.tune_trials;
       b->stats.sweep_reruns = std::max(b->stats.sweep_reruns, s.sweep_reruns);
       b->stats.tuned = s.tuned;
+      b->stats.engine = s.engine;
       b->stats.slots = s.slots;
       b->stats.good_bytes += s.good_bytes;
     }
