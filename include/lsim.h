@@ -115,6 +115,24 @@ LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
  * num_threads is accepted for signature parity and ignored. */
 LSIM_API int lsim_run(lsim_sim *s, int mode, size_t num_threads);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). lsim_run with the same modes and results (read them with the
+ * same accessors), but every device buffer the run needs - the circuit's
+ * device copy, the job's good values, fault state and results, the sweeps'
+ * tile state - is carved from a scratch array of LSIM_NOALLOC_SCRATCH_BYTES
+ * (64 MiB) that the library holds on each GPU it runs on until the process
+ * ends; host staging is pageable. The call makes no hipMalloc, hipFree,
+ * page-locked or other allocating device call; its run stats report
+ * device_allocs 0. A run whose footprint exceeds the scratch fails with
+ * LSIM_E_NOMEM (circuit nodes x rounds x 8 bytes of good values, plus the
+ * fault state and the tile state, must fit). Every call on a GPU shares that
+ * GPU's scratch and leaves its contents behind: calls that overlap on one GPU
+ * (from any handles or threads) race, so callers serialise them. The handle
+ * must not run lsim_run at the same time either (the two share its stream). */
+#define LSIM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+LSIM_API int lsim_run_noalloc(lsim_sim *s, int mode, size_t num_threads);
+
 /* Counts: Simulator::num_PIs/num_POs/num_gates/num_wires/num_rounds
  * (fsim/simulation.hpp:75-80), faults().size() (:70) and
  * num_gates_per_level().size() (:84). */
@@ -196,6 +214,9 @@ typedef struct {
                               word growth << 16 */
   int64_t engine;          /* sweep kernel that ran: 0 k_sweep_level (per-level form),
                               1 k_sweep_pairs, 2 k_sweep_events (persistent forms) */
+  int64_t device_allocs;   /* device and page-locked host allocations the library made
+                              during the call (lsim_run, lsim_run_noalloc; counted over
+                              all threads of the process) */
 } lsim_run_stats;
 LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
 

@@ -58,6 +58,7 @@ class RunStats(ctypes.Structure):
         ("tune_trials", ctypes.c_int64),
         ("tuned", ctypes.c_int64),
         ("engine", ctypes.c_int64),
+        ("device_allocs", ctypes.c_int64),
     ]
 
 
@@ -99,6 +100,7 @@ def lib() -> ctypes.CDLL:
         "lsim_read_buffers": (ctypes.c_int, [P, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz]),
         "lsim_set_option": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int64]),
         "lsim_run": (ctypes.c_int, [P, ctypes.c_int, sz]),
+        "lsim_run_noalloc": (ctypes.c_int, [P, ctypes.c_int, sz]),
         "lsim_num_pis": (ctypes.c_int64, [P]),
         "lsim_num_pos": (ctypes.c_int64, [P]),
         "lsim_num_gates": (ctypes.c_int64, [P]),
@@ -263,6 +265,11 @@ class Simulator:
     # -- run (fsim/simulation.cpp:65-92)
     def run(self, mode: int = 0, num_threads: int = 1) -> None:
         self._check(self._L.lsim_run(self._h, int(mode), int(num_threads)))
+
+    def run_noalloc(self, mode: int = 0, num_threads: int = 1) -> None:
+        """run() with every device buffer carved from the library's fixed
+        per-device scratch (lsim_run_noalloc): no device allocation."""
+        self._check(self._L.lsim_run_noalloc(self._h, int(mode), int(num_threads)))
 
     # -- accessors (fsim/simulation.hpp:58-89)
     def num_PIs(self) -> int:

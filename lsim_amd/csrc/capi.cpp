@@ -99,6 +99,7 @@ struct ResidentCircuit {
 struct lsim_sim {
   int device = -1;  // -1: host-only handle (read and accessors, no run)
   std::unique_ptr<lsim::Engine> engine;
+  void *scratch = nullptr;  // the device's no-allocation scratch (lsim_run_noalloc)
   lsim::Netlist nl;
   lsim::FaultStructure fs;
   lsim::CircuitRes *circuit = nullptr;
@@ -201,7 +202,12 @@ int lsim_create(int device, lsim_sim **out) {
   if (!s) return LSIM_E_NOMEM;
   s->device = device;
   if (device >= 0) {
-    int rc = guarded(&s->err, [&] { s->engine.reset(new lsim::Engine(device)); });
+    // Set-up of the device's no-allocation scratch (its address, looked up
+    // once per device) happens here, with the engine.
+    int rc = guarded(&s->err, [&] {
+      s->engine.reset(new lsim::Engine(device));
+      s->scratch = lsim::noalloc_scratch(device);
+    });
     if (rc) {
       g_create_error = s->err;
       return rc;
@@ -254,20 +260,25 @@ int lsim_set_option(lsim_sim *s, const char *key, int64_t value) {
   return LSIM_OK;
 }
 
+static int engine_of_mode(lsim_sim *s, int mode, int *m) {
+  switch (mode) {
+    case LSIM_MODE_REF_SEQUENTIAL:
+    case LSIM_MODE_REF_PARALLEL_TF:
+    case LSIM_MODE_REF_PARALLEL_OMP:
+    case LSIM_MODE_PARTITIONED: *m = lsim::ENGINE_PERSISTENT; return LSIM_OK;
+    case LSIM_MODE_PLAIN: *m = lsim::ENGINE_PLAIN; return LSIM_OK;
+    default: return fail(&s->err, LSIM_E_ARG, "unknown mode " + std::to_string(mode));
+  }
+}
+
 int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
   (void)num_threads;
   int rc = check_loaded(s);
   if (rc) return rc;
   if (!s->engine) return fail(&s->err, LSIM_E_DEVICE, "handle was created without a device (device -1)");
   int m;
-  switch (mode) {
-    case LSIM_MODE_REF_SEQUENTIAL:
-    case LSIM_MODE_REF_PARALLEL_TF:
-    case LSIM_MODE_REF_PARALLEL_OMP:
-    case LSIM_MODE_PARTITIONED: m = lsim::ENGINE_PERSISTENT; break;
-    case LSIM_MODE_PLAIN: m = lsim::ENGINE_PLAIN; break;
-    default: return fail(&s->err, LSIM_E_ARG, "unknown mode " + std::to_string(mode));
-  }
+  if ((rc = engine_of_mode(s, mode, &m))) return rc;
+  const int64_t allocs0 = lsim::device_allocations();
   return guarded(&s->err, [&] {
     s->ran = false;
     lsim::EngineOptions eo = s->eopt;
@@ -291,6 +302,45 @@ int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
       s->job = nullptr;
       throw;
     }
+    s->res.stats.device_allocs = lsim::device_allocations() - allocs0;
+    s->ran = true;
+  });
+}
+
+int lsim_run_noalloc(lsim_sim *s, int mode, size_t num_threads) {
+  (void)num_threads;
+  int rc = check_loaded(s);
+  if (rc) return rc;
+  if (!s->engine || !s->circuit)
+    return fail(&s->err, LSIM_E_DEVICE, "handle was created without a device (device -1)");
+  int m;
+  if ((rc = engine_of_mode(s, mode, &m))) return rc;
+  if (!s->scratch) return fail(&s->err, LSIM_E_STATE, "no-allocation scratch was not set up for this device");
+  const int64_t need = lsim::Engine::noalloc_min_bytes(s->circuit, int64_t(s->nl.fault_site.size()), s->nl.R,
+                                                       s->jopt.drop && s->jopt.report_last_round, !s->jopt.drop);
+  if (need > int64_t(lsim::kNoAllocScratchBytes))
+    return fail(&s->err, LSIM_E_NOMEM,
+                "this run needs at least " + std::to_string((need + (1 << 20) - 1) >> 20) +
+                    " MiB of device memory; the no-allocation scratch holds " +
+                    std::to_string(lsim::kNoAllocScratchBytes >> 20) + " MiB");
+  const int64_t allocs0 = lsim::device_allocations();
+  return guarded(&s->err, [&] {
+    s->ran = false;
+    lsim::EngineOptions eo = s->eopt;
+    eo.mode = m;
+    eo.good_graph = false;  // a graph's instantiation allocates
+    eo.autotune = false;
+    lsim::JobOptions jo = s->jopt;
+    jo.keep_good_last = true;
+    // A fresh engine per call, carving every buffer from the scratch from
+    // its start; it borrows the handle engine's stream and events.
+    lsim::Engine e(*s->engine, s->scratch, lsim::kNoAllocScratchBytes);
+    lsim::CircuitRes *c = e.add_circuit(s->nl, s->fs);
+    lsim::JobState *j =
+        e.add_job(c, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns, s->nl.R, s->nl.patterns, jo);
+    e.run({j}, eo);
+    s->res = std::move(e.results(j));
+    s->res.stats.device_allocs = lsim::device_allocations() - allocs0;
     s->ran = true;
   });
 }

@@ -1482,10 +1482,37 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
 // engine's kernels).
 thread_local hipStream_t tl_alloc_stream = nullptr;
 
+// Device memory of an engine that must not allocate: one region the caller
+// provides, carved front to back in 256-byte aligned pieces and never freed
+// piecewise (the whole engine is dropped at once).
+struct DeviceArena {
+  uint8_t *base = nullptr;
+  size_t cap = 0, used = 0;
+  void *take(size_t bytes) {
+    const size_t at = (used + 255) & ~size_t(255);
+    if (bytes > cap || at > cap - bytes)
+      throw DeviceError(LSIM_E_NOMEM, "the run needs more device memory than the " + std::to_string(cap >> 20) +
+                                          " MiB no-allocation scratch holds");
+    used = at + bytes;
+    return base + at;
+  }
+  size_t left() const { return cap - std::min(cap, (used + 255) & ~size_t(255)); }
+};
+thread_local DeviceArena *tl_arena = nullptr;
+// Device and page-locked host allocations the library has made (all threads).
+std::atomic<int64_t> g_allocs{0};
+
 struct AllocStream {
   hipStream_t prev;
-  explicit AllocStream(hipStream_t s) : prev(tl_alloc_stream) { tl_alloc_stream = s; }
-  ~AllocStream() { tl_alloc_stream = prev; }
+  DeviceArena *prev_arena;
+  explicit AllocStream(hipStream_t s, DeviceArena *a = nullptr) : prev(tl_alloc_stream), prev_arena(tl_arena) {
+    tl_alloc_stream = s;
+    tl_arena = a;
+  }
+  ~AllocStream() {
+    tl_alloc_stream = prev;
+    tl_arena = prev_arena;
+  }
   AllocStream(const AllocStream &) = delete;
   AllocStream &operator=(const AllocStream &) = delete;
 };
@@ -1496,10 +1523,23 @@ struct DevBuf {
   size_t n = 0;
   hipStream_t s = nullptr;  // stream of a stream-ordered allocation, null for a plain one
   bool own = true;          // false: a view into memory another buffer owns
+  bool carved = false;      // carved from the thread's DeviceArena (reused while large enough, never freed)
+  // Under a DeviceArena (tl_arena) the memory is carved from it and no
+  // allocation is made.
+  bool from_arena(size_t count) {
+    if (!tl_arena) return false;
+    p = static_cast<T *>(tl_arena->take(count * sizeof(T)));
+    n = count;
+    s = nullptr;
+    own = false;
+    carved = true;
+    return true;
+  }
   void alloc(size_t count) {
-    if (count <= n && p && own) return;
+    if (count <= n && p && (own || carved)) return;
     release();
     if (count == 0) count = 1;
+    if (from_arena(count)) return;
     if (tl_alloc_stream) {
       LSIM_HIP(hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), tl_alloc_stream));
       s = tl_alloc_stream;
@@ -1507,14 +1547,17 @@ struct DevBuf {
       LSIM_HIP(hipMalloc(&p, count * sizeof(T)));
       s = nullptr;
     }
+    g_allocs.fetch_add(1, std::memory_order_relaxed);
     n = count;
   }
   // Plain allocation, for the large sweep scratch that lives as long as the engine.
   void alloc_direct(size_t count) {
-    if (count <= n && p && own) return;
+    if (count <= n && p && (own || carved)) return;
     release();
     if (count == 0) count = 1;
+    if (from_arena(count)) return;
     LSIM_HIP(hipMalloc(&p, count * sizeof(T)));
+    g_allocs.fetch_add(1, std::memory_order_relaxed);
     s = nullptr;
     n = count;
   }
@@ -1527,6 +1570,7 @@ struct DevBuf {
     n = 0;
     s = nullptr;
     own = true;
+    carved = false;
   }
   // Points at `count` elements of memory owned elsewhere (never freed here).
   void view(T *ptr, size_t count) {
@@ -1542,22 +1586,37 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
-// Page-locked host staging buffer (grows, never shrinks).
+// Page-locked host staging buffer (grows, never shrinks). Under a
+// DeviceArena it is ordinary pageable memory instead: an engine that must
+// not allocate makes no page-locked allocation either.
 template <typename T>
 struct HostBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool pinned = false;
   void alloc(size_t count) {
     if (count <= n && p) return;
     release();
     if (count == 0) count = 1;
-    LSIM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault));
+    if (tl_arena) {
+      p = static_cast<T *>(std::malloc(count * sizeof(T)));
+      if (!p) throw std::bad_alloc();
+      pinned = false;
+    } else {
+      LSIM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault));
+      g_allocs.fetch_add(1, std::memory_order_relaxed);
+      pinned = true;
+    }
     n = count;
   }
   void release() {
-    if (p) (void)hipHostFree(p);
+    if (p) {
+      if (pinned) (void)hipHostFree(p);
+      else std::free(p);
+    }
     p = nullptr;
     n = 0;
+    pinned = false;
   }
   HostBuf() = default;
   HostBuf(const HostBuf &) = delete;
@@ -1927,13 +1986,23 @@ struct Engine::Impl {
   uint32_t epoch_base = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
 
+  // An engine that must not allocate: every device buffer is carved from
+  // arena_mem, and the stream and events are borrowed from another engine.
+  DeviceArena arena_mem;
+  DeviceArena *arena = nullptr;
+  bool borrowed = false;
+
   ~Impl() {
+    if (borrowed) (void)hipStreamSynchronize(stream);
     jobs.clear();
     circuits.clear();
-    if (ev[0]) (void)hipEventDestroy(ev[0]);
-    if (ev[1]) (void)hipEventDestroy(ev[1]);
-    if (fin_ev) (void)hipEventDestroy(fin_ev);
-    // The stream itself goes with `holder`, after the member buffers.
+    if (!borrowed) {
+      if (ev[0]) (void)hipEventDestroy(ev[0]);
+      if (ev[1]) (void)hipEventDestroy(ev[1]);
+      if (fin_ev) (void)hipEventDestroy(fin_ev);
+    }
+    // The stream itself goes with `holder` (null when borrowed), after the
+    // member buffers.
   }
   void time_begin() { LSIM_HIP(hipEventRecord(ev[0], stream)); }
   double time_end() {
@@ -2050,10 +2119,65 @@ Engine::Engine(int device) : impl_(new Impl) {
   }
 }
 
+Engine::Engine(const Engine &host, void *scratch, size_t bytes) : impl_(new Impl) {
+  const Impl &h = *host.impl_;
+  Impl &m = *impl_;
+  m.device = h.device;
+  m.cus = h.cus;
+  m.stream = h.stream;  // borrowed: holder stays empty
+  m.ev[0] = h.ev[0];
+  m.ev[1] = h.ev[1];
+  m.fin_ev = h.fin_ev;
+  m.borrowed = true;
+  m.arena_mem.base = static_cast<uint8_t *>(scratch);
+  m.arena_mem.cap = bytes;
+  m.arena = &m.arena_mem;
+  try {
+    LSIM_HIP(hipSetDevice(m.device));
+    AllocStream alloc_on(m.stream, m.arena);
+    m.d_bytes.alloc(3);
+    LSIM_HIP(hipMemsetAsync(m.d_bytes.p, 0, 3 * sizeof(unsigned long long), m.stream));
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+int64_t device_allocations() { return g_allocs.load(std::memory_order_relaxed); }
+
+namespace {
+// The no-allocation scratch. In host code `scratch` names a host-side
+// stand-in, not device memory: only the per-device address from
+// hipGetSymbolAddress is passed to kernels or copies.
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // per device, null until looked up
+}  // namespace
+
+void *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const int n = device_count();
+  if (device < 0 || device >= n) throw DeviceError(LSIM_E_DEVICE, "no HIP device " + std::to_string(device));
+  if (g_scratch_ptr.size() < size_t(n)) g_scratch_ptr.resize(size_t(n), nullptr);
+  if (!g_scratch_ptr[size_t(device)]) {
+    int prev = 0;
+    LSIM_HIP(hipGetDevice(&prev));
+    LSIM_HIP(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !ptr)
+      throw DeviceError(LSIM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[size_t(device)] = ptr;
+  }
+  return g_scratch_ptr[size_t(device)];
+}
+
 Engine::~Engine() {
   if (impl_) {
     (void)hipSetDevice(impl_->device);
-    AllocStream alloc_on(impl_->stream);
+    AllocStream alloc_on(impl_->stream, impl_->arena);
     delete impl_;
   }
 }
@@ -2070,7 +2194,7 @@ int64_t Engine::free_bytes() const {
 CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   auto cp = std::make_unique<CircuitRes>();
   CircuitRes &c = *cp;
   c.N = nl.N;
@@ -2181,7 +2305,7 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
 void Engine::remove_circuit(CircuitRes *c) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   LSIM_HIP(hipStreamSynchronize(m.stream));
   for (auto it = m.circuits.begin(); it != m.circuits.end(); ++it)
     if (it->get() == c) {
@@ -2208,6 +2332,13 @@ void order_live(JobState &j) {
 }
 }  // namespace
 
+int64_t Engine::noalloc_min_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool with_last, bool matrix) {
+  const int64_t nl = std::max<int64_t>(c->NL, 1);
+  const int64_t slot = nl * std::max<int64_t>(int64_t(sizeof(PairRec)) + 4 + 64 * 8,
+                                              64 * 8 + 8 + int64_t(sizeof(MaskRec)) + 4);
+  return c->bytes() + job_bytes(c, faults, R, !matrix) + (with_last ? faults : 0) + 2 * slot + (int64_t(1) << 20);
+}
+
 int64_t Engine::job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop) {
   const int64_t RB = (R + 63) / 64;
   return 8 * c->N * R + 8 * c->P * R + 4 * std::max<int64_t>(c->NL, 1) + faults * (4 + 4 + 1 + 4 + 4 + 1) +
@@ -2219,7 +2350,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
                           const JobOptions &opt) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   auto jp = std::make_unique<JobState>();
   JobState &j = *jp;
   j.c = c;
@@ -2308,7 +2439,7 @@ void Engine::release_job(JobState *j) {
   // The job's device memory is one stream-ordered allocation: it goes to the
   // engine's cache, where only work queued later on the same stream reuses
   // it, so no wait.
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   for (auto it = m.jobs.begin(); it != m.jobs.end(); ++it)
     if (it->get() == j) {
       m.arenas.put(j->arena);
@@ -2321,7 +2452,7 @@ bool Engine::reset_job(JobState *j, const JobOptions &opt) {
   Impl &m = *impl_;
   if (opt.drop != j->opt.drop || (opt.drop && opt.report_last_round != j->opt.report_last_round)) return false;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   j->opt = opt;
   j->direct = opt.direct || j->c->has_mux;
   j->started = j->done = j->last_done = false;
@@ -2565,7 +2696,7 @@ GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
 void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   std::vector<JobState *> js;
   for (JobState *j : jobs)
     if (!j->started) js.push_back(j);
@@ -2823,6 +2954,9 @@ void Engine::Impl::finalize_end() {
 }
 
 int64_t Engine::Impl::scratch_budget(const EngineOptions &eo, int64_t held) const {
+  // Without allocation: half of what the arena has left, the rest for the
+  // step buffers.
+  if (arena) return int64_t(arena->left() / 2);
   if (eo.scratch_mb > 0) return eo.scratch_mb << 20;
   size_t fr = 0, tot = 0;
   LSIM_HIP(hipMemGetInfo(&fr, &tot));
@@ -2858,9 +2992,9 @@ void Engine::Impl::ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, 
   release_pair_scratch();
   // Cached job memory and what the stream-ordered pool keeps count as free
   // for the scratch budget.
-  arenas.clear();
-  LSIM_HIP(hipStreamSynchronize(stream));
-  {
+  if (!arena) {
+    arenas.clear();
+    LSIM_HIP(hipStreamSynchronize(stream));
     hipMemPool_t mp = nullptr;
     if (hipDeviceGetDefaultMemPool(&mp, device) == hipSuccess && mp) (void)hipMemPoolTrimTo(mp, 0);
   }
@@ -3574,7 +3708,7 @@ int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const En
 int64_t Engine::iterate(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
-  AllocStream alloc_on(m.stream);
+  AllocStream alloc_on(m.stream, m.arena);
   return m.iterate_jobs(jobs, eo);
 }
 

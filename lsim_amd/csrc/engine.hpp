@@ -106,6 +106,11 @@ struct JobState;    // job (engine.hip)
 class Engine {
  public:
   explicit Engine(int device);
+  // An engine that allocates nothing: every device buffer it needs is carved
+  // from `scratch` (`bytes` long, on host's device), host staging is pageable,
+  // and it borrows host's stream and events (so it must not run while host
+  // does). Running past `bytes` fails with DeviceError(LSIM_E_NOMEM).
+  Engine(const Engine &host, void *scratch, size_t bytes);
   ~Engine();
   Engine(const Engine &) = delete;
   Engine &operator=(const Engine &) = delete;
@@ -128,6 +133,10 @@ class Engine {
   bool reset_job(JobState *j, const JobOptions &opt);
   // Device bytes a job on circuit c would hold (admission by footprint).
   static int64_t job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop);
+  // Least device memory a lone job on c needs in an engine that must not
+  // allocate: the circuit's copy, the job's arrays, and (twice, as the
+  // sweeps take half of what is left) one tile slot with its pool.
+  static int64_t noalloc_min_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool with_last, bool matrix);
   int64_t free_bytes() const;
 
   // Good machine for jobs not started yet (one batched launch per level).
@@ -148,5 +157,15 @@ class Engine {
 
 // Number of visible HIP devices (0 when the runtime has none).
 int device_count();
+
+// Device and page-locked host allocations the library has made so far, on
+// all threads.
+int64_t device_allocations();
+
+// The library's fixed no-allocation scratch on `device` (a static device
+// array of kNoAllocScratchBytes, one copy per device): its device address,
+// looked up once per device with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+void *noalloc_scratch(int device);
 
 }  // namespace lsim
