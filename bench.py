@@ -5,6 +5,13 @@ good machine over every round, then the fault sweeps and detection (with fault
 dropping where the configuration asks for it). Circuit parsing and the upload
 to HBM happen before the timed region.
 
+The default run measures the 2M-gate workload (the headline line) and, in the
+same JSON line, the two systems forms on their own workloads: `batching` (256
+circuits in one batch, autotuned, evals/s and ms per batch) and `serving` (64
+resident circuits, Poisson job arrivals: p50/p99 latency beside unloaded
+latency, throughput, and whether every job's result is the same alone and in
+the mix).
+
 Prints one JSON line (rank 0). `roofline.achieved` is the sweep kernels' own
 count of the bytes they read and write (lsim_run_stats.sweep_bytes) over the
 HIP-event time of their launches; `roofline.traffic` and the request rate come
@@ -122,7 +129,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def _roofline(kernel: str, agg: dict, config: str) -> dict:
+def _roofline(kernel: str, agg: dict, config: str, probe: bool = True) -> dict:
     launches = max(1, agg["sweep_launches"])
     t_launch_s = agg["t_sweep_ms"] / 1000.0 / launches
     bytes_launch = agg["sweep_bytes"] / launches
@@ -139,6 +146,10 @@ def _roofline(kernel: str, agg: dict, config: str) -> dict:
     # memory requests per second against the rate the device serves random
     # 8-byte reads (measured live on this device).
     tx = profiled_value(config, kernel, "transactions_per_launch")
+    if not probe:
+        if tx and t_launch_s > 0:
+            out["transactions_per_s"] = tx / t_launch_s
+        return out
     try:
         pr = probe_random()
     except Exception as e:  # reported, never fatal for the GPU figure
@@ -168,47 +179,53 @@ def _tuned(st: dict) -> dict:
             "word_growth": (t >> 16) & 0xff}
 
 
-def bench_batch(args) -> int:
-    """256 circuits of uneven size, each with its full fault list and its own
-    pattern block, run together through the batch API (one persistent sweep
-    launch per word step over all jobs). A step adds the 256 jobs (parse and
-    pattern upload, untimed) and times run() to the last result on the host."""
+def _batch_options(batch, args) -> None:
+    for key in ("pair_dense", "pair_nopend", "pair_fo_pre", "pair_nopend_pct", "max_step_words"):
+        v = getattr(args, key)
+        if v >= 0:
+            batch.set_option(key, v)
+
+
+def batching_leg(args, steps: int, warmup: int) -> dict:
+    """256 circuits of uneven size (1k-200k gates), each with its full fault
+    list and its own pattern block (2,048-16,384 patterns), run together
+    through the batch API: one persistent sweep launch per word step over all
+    jobs, fault dropping on. A step adds the 256 jobs (parse and pattern
+    upload, untimed) and times run() to the last result on the host. With the
+    autotuner on, the warmup runs explore its candidates."""
     import lsim_amd
     from lsim_amd import workloads
 
     specs = workloads.c4_specs()
     t0 = time.perf_counter()
     batch = lsim_amd.Batch(device=0, mode=1 if args.mode == "partitioned" else 0,
-                           sweep_kernel=0 if args.sweep_kernel == "pairs" else 1, autotune=args.autotune,
+                           sweep_kernel=0 if args.sweep_kernel == "pairs" else 1, autotune=args.c4_autotune,
                            pair_threads=args.pair_threads)
-    if args.pair_dense >= 0:
-        batch.set_option("pair_dense", args.pair_dense)
-    if args.pair_nopend >= 0:
-        batch.set_option("pair_nopend", args.pair_nopend)
-    if args.pair_fo_pre >= 0:
-        batch.set_option("pair_fo_pre", args.pair_fo_pre)
-    if args.pair_nopend_pct >= 0:
-        batch.set_option("pair_nopend_pct", args.pair_nopend_pct)
-    if args.max_step_words >= 0:
-        batch.set_option("max_step_words", args.max_step_words)
+    _batch_options(batch, args)
     cids, jobs_text = [], []
     for i, sp in enumerate(specs):
         cids.append(batch.add_circuit(workloads.circuit_text(sp)))
         jobs_text.append(workloads.batch_job_texts(sp))
-        if i % 32 == 31:
+        if i % 64 == 63:
             log(f"[bench] c4: {i + 1} circuits resident ({time.perf_counter() - t0:.1f}s)")
     F_tot = sum(2 * sp["num_gates"] for sp in specs)
     evals = sum(2 * sp["num_gates"] * sp["num_patterns"] for sp in specs)
-    times, agg, detected, trials = [], None, 0, 0
-    for it in range(args.warmup + args.steps):
+    times, agg, detected, st = [], None, 0, None
+    it, timed_runs = 0, 0
+    while timed_runs < steps:
+        # Timed once the warmup is done and, with the autotuner, once the
+        # previous run reported every candidate measured (it then keeps the
+        # fastest).
+        explored = st is not None and st["tune_trials"] >= st["tune_candidates"]
+        timed = it >= warmup and (not args.c4_autotune or explored or it > 60)
         ids = [batch.add_job(c, f, p, drop=True) for c, (f, p) in zip(cids, jobs_text)]
         t0 = time.perf_counter()
         batch.run()
         dt = time.perf_counter() - t0
         st = batch.last_run_stats()
-        trials = st["tune_trials"]
-        timed = it >= args.warmup
+        it += 1
         if timed:
+            timed_runs += 1
             times.append(dt)
             if agg is None:
                 agg = {k: 0 for k in st}
@@ -217,27 +234,39 @@ def bench_batch(args) -> int:
             detected = sum(int((batch.first_detect_round(j) >= 0).sum()) for j in ids)
         log(f"[bench] c4 {'step' if timed else 'warmup'} {dt:.3f}s sweep {st['t_sweep_ms']:.1f}ms "
             f"detect {st['t_detect_ms']:.1f}ms good {st['t_good_ms']:.1f}ms steps {st['steps']} "
-            f"launches {st['sweep_launches']}")
+            f"launches {st['sweep_launches']} tuned {_tuned(st)}")
         for j in ids:
             batch.release_job(j)
+    batch.close()
     elapsed = sum(times)
     kernel = _kernel_name(args, st["tuned"])
-    out = {
+    return {
         "metric": "fault x pattern evals/sec",
-        "value": evals / (elapsed / args.steps),
+        "value": evals / (elapsed / steps),
         "unit": "fault*pattern evals/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "ms_per_batch": 1000.0 * elapsed / steps,
+        "steps": steps, "warmup": it - steps,
+        "workload": f"c4: {len(specs)} seeded circuits, {sum(sp['num_gates'] for sp in specs)} gates in all "
+                    f"(1k-200k each), {F_tot} SA0/SA1 faults, 2,048-16,384 patterns per circuit, "
+                    "fault dropping on, one batch",
+        "faults": F_tot, "evals_per_batch": evals, "detected": detected,
+        "kernel": kernel,
+        "sweep": _roofline(kernel, agg, "c4", probe=False),
+        "autotune": dict(_tuned(st), enabled=bool(args.c4_autotune), candidates_measured=st["tune_trials"],
+                         candidates=st["tune_candidates"]),
+        "breakdown_ms": {k: agg[k] / steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
+    }
+
+
+def bench_batch(args) -> int:
+    leg = batching_leg(args, args.steps, args.warmup)
+    out = {
+        "metric": leg["metric"], "value": leg["value"], "unit": leg["unit"],
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": leg["ms_per_batch"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {
-            "workload": f"c4: {len(specs)} seeded circuits, {sum(sp['num_gates'] for sp in specs)} gates in all "
-                        f"(1k-200k each), {F_tot} SA0/SA1 faults, 2,048-16,384 patterns per circuit, "
-                        "fault dropping on, one batch",
-            "engine": args.mode, "faults": F_tot, "evals_per_step": evals, "detected": detected,
-        },
-        "roofline": _roofline(kernel, agg, "c4"),
-        "autotune": dict(_tuned(st), enabled=bool(args.autotune), candidates_measured=trials),
-        "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
+        "config": {"workload": leg["workload"], "engine": args.mode, "faults": leg["faults"],
+                   "evals_per_step": leg["evals_per_batch"], "detected": leg["detected"]},
+        "roofline": leg["sweep"], "autotune": leg["autotune"], "breakdown_ms": leg["breakdown_ms"],
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -249,7 +278,7 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else 0.0
 
 
-def bench_serving(args) -> int:
+def serving_leg(args, n_jobs: int) -> dict:
     """Serving over 64 resident circuits. Phase 1: every job alone (unloaded
     latency). Phase 2: all jobs at once (saturated job rate). Phase 3: the same
     jobs as a Poisson stream at `--c5-load` x the saturated rate; reports
@@ -271,7 +300,7 @@ def bench_serving(args) -> int:
         cids.append(srv.add_circuit(workloads.circuit_text(sp)))
         if i % 8 == 7:
             log(f"[bench] c5: {i + 1} circuits resident ({time.perf_counter() - t0:.1f}s)")
-    jobs = workloads.c5_jobs(circuits, args.c5_jobs)
+    jobs = workloads.c5_jobs(circuits, n_jobs)
     texts = [workloads.job_texts(circuits[j["circuit"]], j) for j in jobs]
     F = [int(f.split(b"\n", 1)[0]) for f, _ in texts]
     evals = sum(f * j["num_patterns"] for f, j in zip(F, jobs))
@@ -335,19 +364,12 @@ def bench_serving(args) -> int:
              "faults": F[k], "patterns": jobs[k]["num_patterns"],
              "gates": circuits[jobs[k]["circuit"]]["num_gates"]} for k in slowest]
     p99_alone = _pct(alone, 99)
-    out = {
-        "metric": "p99 latency at fixed QPS",
-        "value": _pct(lat, 99) * 1e3,
-        "unit": "ms",
-        "n_gpus": 1, "steps": len(jobs), "warmup": 0,
-        "ms_per_step": 1e3 * t_run / len(jobs),
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {
-            "workload": f"c5: {len(circuits)} resident seeded circuits (10k-2M gates, "
-                        f"{sum(sp['num_gates'] for sp in circuits)} in all), {len(jobs)} jobs (1-100% of faults, "
-                        "1,024-32,768 patterns), fault dropping on, Poisson arrivals",
-            "qps": lam, "load": args.c5_load,
-        },
+    return {
+        "metric": "p99 latency at fixed QPS", "value": _pct(lat, 99) * 1e3, "unit": "ms",
+        "workload": f"c5: {len(circuits)} resident seeded circuits (10k-2M gates, "
+                    f"{sum(sp['num_gates'] for sp in circuits)} in all), {len(jobs)} jobs (1-100% of faults, "
+                    "1,024-32,768 patterns), fault dropping on, Poisson arrivals",
+        "jobs": len(jobs), "qps": lam, "load": args.c5_load,
         "latency_ms": {"p50": _pct(lat, 50) * 1e3, "p99": _pct(lat, 99) * 1e3,
                        "unloaded_p50": _pct(alone, 50) * 1e3, "unloaded_p99": p99_alone * 1e3,
                        "p99_over_unloaded_p99": _pct(lat, 99) / max(p99_alone, 1e-9),
@@ -357,7 +379,19 @@ def bench_serving(args) -> int:
         "bit_exact_alone_vs_mix": bool(exact),
         "slowest_jobs": tail,
         "server": cnt,
+        "ms_per_job": 1e3 * t_run / len(jobs),
     }
+
+
+def bench_serving(args) -> int:
+    leg = serving_leg(args, args.c5_jobs)
+    out = {
+        "metric": leg["metric"], "value": leg["value"], "unit": leg["unit"],
+        "n_gpus": 1, "steps": leg["jobs"], "warmup": 0, "ms_per_step": leg["ms_per_job"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": leg["workload"], "qps": leg["qps"], "load": leg["load"]},
+    }
+    out.update({k: leg[k] for k in ("latency_ms", "throughput", "bit_exact_alone_vs_mix", "slowest_jobs", "server")})
     print(json.dumps(out), flush=True)
     return 0
 
@@ -368,7 +402,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--c5-jobs", type=int, default=96, help="jobs per phase of the serving run")
+    ap.add_argument("--c5-jobs", type=int, default=64, help="jobs per phase of the serving run")
     ap.add_argument("--c5-load", type=float, default=0.5, help="arrival rate as a fraction of the saturated rate")
     ap.add_argument("--c5-growth", type=int, default=4, help="word growth per step of serving jobs")
     ap.add_argument("--mode", default="partitioned", choices=["plain", "partitioned"])
@@ -398,6 +432,15 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
+    ap.add_argument("--c4-autotune", type=int, default=1,
+                    help="batching leg: 1 lets the autotuner explore its candidates in the warmup runs and keep "
+                         "the fastest")
+    ap.add_argument("--c4-steps", type=int, default=3, help="timed batches of the batching leg")
+    ap.add_argument("--c4-warmup", type=int, default=2,
+                    help="untimed batches of the batching leg (with the autotuner: at least until every candidate "
+                         "was measured)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="the 2M-gate line only, without the batching and serving legs")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of one-core CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-random", type=int, default=0, metavar="MIB",
@@ -499,6 +542,17 @@ def main() -> int:
         "autotune": dict(_tuned(st), enabled=bool(args.autotune), candidates_measured=st["tune_trials"]),
         "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
     }
+    sim.close()
+    if not args.no_legs and args.config == "c3":
+        # The two systems forms on their own workloads, in the same run.
+        for name, fn in (("batching", lambda: batching_leg(args, args.c4_steps, args.c4_warmup)),
+                         ("serving", lambda: serving_leg(args, args.c5_jobs))):
+            t0 = time.perf_counter()
+            try:
+                out[name] = fn()
+            except Exception as e:  # reported, never fatal for the headline figure
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] {name} leg in {time.perf_counter() - t0:.1f}s")
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(ckt, flst, ptn, preset, args.cpu_budget)

@@ -214,6 +214,7 @@ typedef struct {
                               word growth << 16 */
   int64_t engine;          /* sweep kernel that ran: 0 k_sweep_level (per-level form),
                               1 k_sweep_pairs, 2 k_sweep_events (persistent forms) */
+  int64_t tune_candidates; /* autotuner candidates for this job set (0: off) */
   int64_t device_allocs;   /* device and page-locked host allocations the library made
                               during the call (lsim_run, lsim_run_noalloc; counted over
                               all threads of the process) */

@@ -58,6 +58,7 @@ class RunStats(ctypes.Structure):
         ("tune_trials", ctypes.c_int64),
         ("tuned", ctypes.c_int64),
         ("engine", ctypes.c_int64),
+        ("tune_candidates", ctypes.c_int64),
         ("device_allocs", ctypes.c_int64),
     ]
 

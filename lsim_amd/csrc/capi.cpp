@@ -583,6 +583,7 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.sweep_node_steps = std::max(b->stats.sweep_node_steps, s.sweep_node_steps);
       b->stats.sweep_bytes = std::max(b->stats.sweep_bytes, s.sweep_bytes);
       b->stats.tune_trials = s.tune_trials;
+      b->stats.tune_candidates = s.tune_candidates;
       b->stats.sweep_reruns = std::max(b->stats.sweep_reruns, s.sweep_reruns);
       b->stats.tuned = s.tuned;
       b->stats.engine = s.engine;

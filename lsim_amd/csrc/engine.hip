@@ -3789,6 +3789,7 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   for (JobState *j : jobs) {
     j->res.stats.t_total_ms = ms;
     j->res.stats.tune_trials = ts ? ts->tried() : 0;
+    j->res.stats.tune_candidates = ts ? int64_t(ts->cands.size()) : 0;
     j->res.stats.tuned = int64_t(e2.sweep_kernel) | (int64_t(first) << 8) | (int64_t(growth) << 16);
     j->res.stats.engine = e2.mode == ENGINE_PLAIN ? 0 : (e2.sweep_kernel == 0 ? 1 : 2);
   }
