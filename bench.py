@@ -53,16 +53,37 @@ def profiled_value(config: str, kernel: str, key: str = "traffic_bytes_per_launc
     return None
 
 
-def probe_random(nbytes: int = 16 << 30) -> dict:
-    """Random 8-byte reads per second the device serves, from the measurement
-    library lsim_amd/liblsim_probe.so (not part of the simulator library)."""
+def _probe_lib():
     import ctypes
 
     path = os.path.join(ROOT, "lsim_amd", "liblsim_probe.so")
     if not os.path.exists(path):
         raise RuntimeError("lsim_amd/liblsim_probe.so is not built (python lsim_amd/build.py)")
-    L = ctypes.CDLL(path)
-    fn = L.lsim_bench_probe_random_reads
+    return ctypes.CDLL(path)
+
+
+def copy_probe(nbytes: int, reps: int = 3) -> dict:
+    """A plain device copy of nbytes, reps times (the known-size kernel
+    k_probe_copy that memory counters are checked against)."""
+    import ctypes
+
+    fn = _probe_lib().lsim_bench_copy
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    rc = fn(0, int(nbytes), int(reps), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"copy probe failed ({rc})")
+    return {"copy_bytes": int(nbytes), "reps": reps, "ms_per_copy": ms.value,
+            "gbs": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+
+
+def probe_random(nbytes: int = 16 << 30) -> dict:
+    """Random 8-byte reads per second the device serves, from the measurement
+    library lsim_amd/liblsim_probe.so (not part of the simulator library)."""
+    import ctypes
+
+    fn = _probe_lib().lsim_bench_probe_random_reads
     fn.restype = ctypes.c_int
     fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     rate, lat = ctypes.c_double(), ctypes.c_double()
@@ -445,9 +466,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-random", type=int, default=0, metavar="MIB",
                     help="only run the random-read probe over a MIB MiB buffer and print its figures")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the known-size kernel for memory "
+                         "counters) and print its figures")
     args = ap.parse_args()
     if args.probe_random:
         print(json.dumps(probe_random(args.probe_random << 20)), flush=True)
+        return 0
+    if args.copy_probe:
+        print(json.dumps(copy_probe(args.copy_probe << 20)), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fault set would shard "

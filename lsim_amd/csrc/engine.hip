@@ -955,10 +955,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           // node is the count of the node's active lanes below it (mbcnt), so
           // no pair has to be located by search.
           const int wave = tid >> 6;
-          const int64_t w_l = s_lword[lane];
-          const int site_l = s_lsite[lane];
-          const uint32_t kind_l = uint32_t(s_lkind[lane]);
-          const uint64_t mask_l = s_lmask[lane];
+          const int w_l = s_lword[lane];
 #pragma unroll 1
           for (int k0 = wave; k0 < n; k0 += kDenseNodes * (CHUNK / 64)) {
             int kk[kDenseNodes], v[kDenseNodes];
@@ -993,11 +990,16 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             for (int i = 0; i < kDenseNodes; ++i)
               r[i] = (on[i] && type[i] != G_PI) ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], vals[i][2], vals[i][3])
                                                 : gu[i];
+            // The lane's injection, read from LDS here rather than held in
+            // registers across the loads above.
+            const int site_l = s_lsite[lane];
 #pragma unroll
             for (int i = 0; i < kDenseNodes; ++i) {
               if (!has[i]) continue;
-              if (on[i] && site_l == v[i])
-                r[i] = kind_l == K_FLIP ? (gu[i] ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+              if (on[i] && site_l == v[i]) {
+                const uint32_t kind_l = uint32_t(s_lkind[lane]);
+                r[i] = kind_l == K_FLIP ? (gu[i] ^ s_lmask[lane]) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+              }
               const uint64_t d = on[i] ? (r[i] ^ gu[i]) : 0;
               const uint64_t nm = __ballot(d != 0);
               if (on[i]) {
@@ -1164,7 +1166,12 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         a.ovf_list[atomicAdd(a.ovf_count, 1)] = t;
       }
     } else {
-      if (tid < T.nlanes) a.out[a.lanes[T.lane0 + tid].out] = s_acc[tid];
+      // The index goes through an empty asm so that the LDS address is formed
+      // here, once per tile, rather than kept live in a register (or a spill
+      // slot) across the whole walk.
+      int t_out = tid;
+      asm volatile("" : "+v"(t_out));
+      if (t_out < T.nlanes) a.out[a.lanes[T.lane0 + t_out].out] = s_acc[t_out];
       if (tid == 0) {
         atomicAdd(&a.bytes[1], (unsigned long long)n_staged);
         atomicAdd(&a.bytes[2], (unsigned long long)(J.live_lvl[l_end + 1] - J.live_lvl[T.start_level]));
@@ -3197,7 +3204,8 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   for (const auto &c : circuits) nl_max = std::max(nl_max, c->NL);
   ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
   // Dense or wide steps use 512-thread workgroups, the others 256-thread
-  // ones, twice as many tiles at once.
+  // ones, twice as many tiles at once. (128-thread groups, 8 tiles per CU,
+  // were slower in every word step of the 2M-gate workload.)
   int threads = eo.pair_threads ? eo.pair_threads : (dense ? 512 : 256);
   if (threads == 256 && pc256 < 1) threads = 512;
   const int per_cu = threads == 256 ? pc256 : pc512;

@@ -51,6 +51,13 @@ __global__ void k_probe_random(const uint64_t *buf, uint64_t n, int rounds, uint
   out[t] = acc;
 }
 
+// Plain streaming copy, 16 bytes per thread per step (grid-stride): the
+// known-size reference for the memory counters (n16 16-byte units).
+__global__ void k_probe_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 void check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw ProbeError(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -111,6 +118,32 @@ ProbeResult probe_random_reads(int device, int64_t bytes) {
 
 }  // namespace lsim
 
+namespace lsim {
+
+double probe_copy(int device, int64_t bytes, int reps) {
+  if (bytes < (int64_t(1) << 20) || bytes % 16) throw ProbeError("copy size must be a multiple of 16, at least 1 MiB");
+  check(hipSetDevice(device), "hipSetDevice");
+  Mem src, dst;
+  check(hipMalloc(&src.p, size_t(bytes)), "copy source");
+  check(hipMalloc(&dst.p, size_t(bytes)), "copy destination");
+  check(hipMemset(src.p, 1, size_t(bytes)), "copy fill");
+  Events ev;
+  check(hipEventCreate(&ev.a), "event");
+  check(hipEventCreate(&ev.b), "event");
+  const uint64_t n16 = uint64_t(bytes) / 16;
+  check(hipEventRecord(ev.a, 0), "event");
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, 0, static_cast<const uint4 *>(src.p),
+                       static_cast<uint4 *>(dst.p), n16);
+  check(hipEventRecord(ev.b, 0), "event");
+  check(hipEventSynchronize(ev.b), "copy");
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, ev.a, ev.b), "event");
+  return double(ms) / reps;
+}
+
+}  // namespace lsim
+
 // Entry point of the measurement library liblsim_probe.so, which only the
 // benchmark loads; the simulator library does not contain the probe.
 extern "C" __attribute__((visibility("default"))) int lsim_bench_probe_random_reads(int device, int64_t bytes,
@@ -121,6 +154,20 @@ extern "C" __attribute__((visibility("default"))) int lsim_bench_probe_random_re
     const lsim::ProbeResult r = lsim::probe_random_reads(device, bytes);
     *reads_per_s = r.reads_per_s;
     *latency_ns = r.latency_ns;
+    return 0;
+  } catch (const std::exception &) {
+    return -5;
+  }
+}
+
+// A plain device copy of `bytes` (read and written once per rep): the
+// known-size kernel the memory counters are checked against. Average ms per
+// copy in *ms.
+extern "C" __attribute__((visibility("default"))) int lsim_bench_copy(int device, int64_t bytes, int reps,
+                                                                   double *ms) {
+  if (!ms || reps < 1) return -1;
+  try {
+    *ms = lsim::probe_copy(device, bytes, reps);
     return 0;
   } catch (const std::exception &) {
     return -5;

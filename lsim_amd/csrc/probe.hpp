@@ -1,7 +1,8 @@
 // Random-access probe of the device's memory system (measurement only: built
 // into liblsim_probe.so, loaded by the benchmark, not part of liblsim.so).
-// C entry point: int lsim_bench_probe_random_reads(int device, int64_t bytes,
-// double *reads_per_s, double *latency_ns), 0 on success.
+// C entry points: int lsim_bench_probe_random_reads(int device, int64_t bytes,
+// double *reads_per_s, double *latency_ns) and int lsim_bench_copy(int device,
+// int64_t bytes, int reps, double *ms), 0 on success.
 #pragma once
 #include <cstdint>
 #include <stdexcept>
@@ -21,5 +22,7 @@ struct ProbeResult {
 
 // Allocates `bytes` of device memory for the duration of the probe.
 ProbeResult probe_random_reads(int device, int64_t bytes);
+// `reps` plain copies of `bytes` between two device buffers; ms per copy.
+double probe_copy(int device, int64_t bytes, int reps);
 
 }  // namespace lsim
