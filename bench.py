@@ -450,6 +450,10 @@ def main() -> int:
                     help="with fault dropping: words of the first step (0: the engine default)")
     ap.add_argument("--word-growth", type=int, default=0,
                     help="with fault dropping: factor of words per step (0: the engine default)")
+    ap.add_argument("--good-form", type=int, default=-1, choices=[-1, 0, 1],
+                    help="good machine: 0 one launch per level, 1 LDS-sized level windows (-1: the engine default, 1)")
+    ap.add_argument("--win-levels", type=int, default=0, help="level windows: most levels per window (0: default)")
+    ap.add_argument("--win-cap", type=int, default=0, help="level windows: nodes per block (0: from the LDS)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: the engine's autotuner tries one kernel/word-schedule candidate per run on the "
                          "first runs (give --warmup >= 8 to finish exploring) and then keeps the fastest")
@@ -498,6 +502,11 @@ def main() -> int:
     sim.set_option("sweep_kernel", 0 if args.sweep_kernel == "pairs" else 1)
     sim.set_option("autotune", args.autotune)
     sim.set_option("pair_threads", args.pair_threads)
+    if args.good_form >= 0:
+        sim.set_option("good_form", args.good_form)
+    if args.win_levels:
+        sim.set_option("win_levels", args.win_levels)
+    sim.set_option("win_cap", args.win_cap)
     if args.pair_dense >= 0:
         sim.set_option("pair_dense", args.pair_dense)
     if args.pair_nopend >= 0:

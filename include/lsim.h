@@ -107,8 +107,14 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            word schedule) candidate, then keep the fastest.
  *  "good_graph"         0/1  a lone job replays its circuit's captured
  *                            good-machine graph (default 1).
- *  "good_form"          good machine of a lone job: 0 one launch per level
- *                            (default), 1 LDS-staged level windows. */
+ *  "good_form"          good machine of a lone job: 0 one launch per level,
+ *                            1 LDS-staged level windows (the partitioned form,
+ *                            fsim/cuda/simulation.cu:487-733; default).
+ *  "win_levels"         level windows: most levels per window, 1..8 (default 4);
+ *                            a window whose deepest cone outgrows a block is
+ *                            cut shorter.
+ *  "win_cap"            level windows: nodes per block, 0..4096 (default 0 = half
+ *                            the CU's LDS at 544 B per node: 150 on gfx950). */
 LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
 
 /* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).
@@ -232,7 +238,7 @@ LSIM_API void lsim_batch_destroy(lsim_batch *b);
 LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
 /* Options as lsim_set_option (engine options: "scratch_mb", "block_waves", "sweep_kernel", "pair_threads",
  * "pair_dense", "pair_nopend", "pair_nopend_pct", "pair_fo_pre", "max_step_words", "pool_cap", "autotune", "good_graph",
- * "good_form") and "mode" (0 plain, 1 persistent). */
+ * "good_form", "win_levels", "win_cap") and "mode" (0 plain, 1 persistent). */
 LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
 /* Makes a circuit (.ckt text) resident; returns its id. */
 LSIM_API int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id);

@@ -180,6 +180,12 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "good_form") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "good_form must be 0 or 1");
     eo.good_form = int(value);
+  } else if (k == "win_levels") {
+    if (value < 1 || value > 8) return fail(err, LSIM_E_ARG, "win_levels must be 1..8");
+    eo.win_levels = int(value);
+  } else if (k == "win_cap") {
+    if (value < 0 || value > 4096) return fail(err, LSIM_E_ARG, "win_cap must be 0..4096");
+    eo.win_cap = int(value);
   } else if (k == "good_graph") {
     eo.good_graph = value != 0;
   } else if (k == "sweep_kernel") {

@@ -248,14 +248,18 @@ __global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const i
 }
 
 // Masked good values of one round, gathered per device position.
-// Level-window form of the good machine. A window is kWinLevels consecutive
-// levels; its nodes are split into blocks, each closed under fanins inside
-// the window (the backward cones of the window's outputs, shared nodes
-// repeated in every block that needs them, each written by one owner). A
-// workgroup evaluates one block for 64 words level by level, keeping the
-// block's values in LDS and reading only fanins of earlier windows from HBM.
-constexpr int kWinLevels = 4;
-constexpr int kWinNodes = 128;  // a block's values for 64 words: 64 KB of LDS
+// Level-window form of the good machine (the partitioned form): the levels
+// are cut into windows of up to `win_levels` consecutive levels; a window's
+// nodes are split into blocks, each closed under fanins inside the window (the
+// backward cones of the window's outputs, shared nodes repeated in every
+// block that needs them, each written by one owner). A block holds at most
+// `cap` nodes, sized from the device's LDS: a workgroup evaluates one block for
+// 64 words level by level, keeping the block's values (cap x 64 words of 8 B)
+// and node records (32 B each) in LDS and reading only fanins of earlier
+// windows from HBM. On gfx950 (160 KB of LDS per CU, two blocks resident per
+// CU) that is 150 nodes in 80 KB. A window whose deepest cone does not fit is
+// cut to fewer levels.
+constexpr int kWinLevelsMax = 8;
 
 struct WinNode {
   int32_t u;      // device position
@@ -264,13 +268,22 @@ struct WinNode {
 };
 
 struct WinBlock {
-  int32_t node0;               // first WinNode of the block
-  int32_t seg[kWinLevels + 1];  // seg[k]: first block node on the window's k-th level; seg[kWinLevels]: count
+  int32_t node0;                   // first WinNode of the block
+  int32_t seg[kWinLevelsMax + 1];  // seg[k]: first block node on the window's k-th level; seg[depth]: count
 };
 
+constexpr int kWinGroup = 4;  // nodes per wave with their loads in flight together
+
+// LDS of one window block: values [cap][64] words, then the block's node records.
+__host__ __device__ constexpr size_t win_lds_bytes(int cap) {
+  return size_t(cap) * 64 * sizeof(uint64_t) + size_t(cap) * sizeof(WinNode);
+}
+
 __global__ void __launch_bounds__(512) k_good_window(const JobDev *__restrict__ jd, const WinBlock *__restrict__ blocks,
-                                                     const WinNode *__restrict__ nodes, int block0, int wgroups) {
-  __shared__ uint64_t s_val[kWinNodes][64];
+                                                     const WinNode *__restrict__ nodes, int block0, int wgroups,
+                                                     int depth, int cap) {
+  extern __shared__ uint64_t s_val[];  // [cap][64], then WinNode[cap]
+  WinNode *s_node = reinterpret_cast<WinNode *>(s_val + int64_t(cap) * 64);
   const JobDev &J = jd[0];
   const WinBlock B = blocks[block0 + int(blockIdx.x) / wgroups];
   const int g = int(blockIdx.x) % wgroups;
@@ -279,25 +292,50 @@ __global__ void __launch_bounds__(512) k_good_window(const JobDev *__restrict__ 
   const int64_t w = int64_t(g) * 64 + lane;
   const bool wok = w < RW;
   const uint64_t *good = J.good;
-  for (int k = 0; k < kWinLevels; ++k) {
-    for (int i = B.seg[k] + wave; i < B.seg[k + 1]; i += nwaves) {
-      const WinNode n = nodes[B.node0 + i];
-      const uint32_t type = n.info & 15u, nin = (n.info >> 4) & 7u;
-      uint64_t r = 0;
-      if (wok) {
-        if (type == G_PI) {
-          r = J.pi_words[int64_t(n.fin.x) * RW + w];
-        } else {
-          const int fi[4] = {n.fin.x, n.fin.y, n.fin.z, n.fin.w};
-          uint64_t v[4] = {0, 0, 0, 0};
+  // The block's node records, once, in one coalesced pass.
+  const int nb = B.seg[depth];
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s_node[i] = nodes[B.node0 + i];
+  __syncthreads();
+  for (int k = 0; k < depth; ++k) {
+    const int a = B.seg[k], b = B.seg[k + 1];
+    // Nodes of one level do not read each other: a wave issues the fanin
+    // loads of kWinGroup nodes before it evaluates any of them.
+    for (int i0 = a + wave; i0 < b; i0 += kWinGroup * nwaves) {
+      uint64_t v[kWinGroup][4];
+      uint32_t info[kWinGroup];
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (q < int(nin)) v[q] = fi[q] < 0 ? s_val[~fi[q]][lane] : good[int64_t(fi[q]) * RW + w];
-          r = eval_gate(type, nin, v[0], v[1], v[2], v[3]);
+      for (int t = 0; t < kWinGroup; ++t) {
+        const int i = i0 + t * nwaves;
+        info[t] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[t][q] = 0;
+        if (i < b && wok) {
+          const WinNode n = s_node[i];
+          info[t] = n.info;
+          const uint32_t type = n.info & 15u, nin = (n.info >> 4) & 7u;
+          if (type == G_PI) {
+            v[t][0] = J.pi_words[int64_t(n.fin.x) * RW + w];
+          } else {
+            const int fi[4] = {n.fin.x, n.fin.y, n.fin.z, n.fin.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (q < int(nin))
+                v[t][q] = fi[q] < 0 ? s_val[int64_t(~fi[q]) * 64 + lane] : good[int64_t(fi[q]) * RW + w];
+          }
         }
-        if ((n.info >> 8) & 1u) J.good[int64_t(n.u) * RW + w] = r;
       }
-      s_val[i][lane] = r;
+#pragma unroll
+      for (int t = 0; t < kWinGroup; ++t) {
+        const int i = i0 + t * nwaves;
+        if (i >= b) continue;
+        uint64_t r = 0;
+        if (wok) {
+          const uint32_t type = info[t] & 15u, nin = (info[t] >> 4) & 7u;
+          r = type == G_PI ? v[t][0] : eval_gate(type, nin, v[t][0], v[t][1], v[t][2], v[t][3]);
+          if ((info[t] >> 8) & 1u) J.good[int64_t(s_node[i].u) * RW + w] = r;
+        }
+        s_val[int64_t(i) * 64 + lane] = r;
+      }
     }
     __syncthreads();
   }
@@ -1803,6 +1841,7 @@ void run_chunks(int chunks, int64_t n, Fn &&fn) {
 struct GoodGraph {
   int64_t R = 0;
   int form = 0;  // 0: one launch per level, 1: level windows
+  int win_levels = 0, win_cap = 0;  // window plan of form 1
   DevBuf<JobDev> job;
   DevBuf<int64_t> off;
   hipGraph_t graph = nullptr;
@@ -1815,7 +1854,9 @@ struct GoodGraph {
 
 // Level-window blocks of one circuit (built on first use).
 struct WindowPlan {
+  int levels = 0, cap = 0;         // window depth asked for, block capacity in nodes
   std::vector<int32_t> win_block;  // window k holds blocks [win_block[k], win_block[k+1])
+  std::vector<int32_t> win_depth;  // levels of window k (cut below `levels` where a cone outgrows `cap`)
   DevBuf<WinBlock> blocks;
   DevBuf<WinNode> nodes;
   int64_t repeated = 0;  // block nodes beyond one per circuit node
@@ -2026,8 +2067,10 @@ struct Engine::Impl {
     if (bytes) LSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
   }
   JobDev job_dev(JobState &j) const;
-  GoodGraph *good_graph(CircuitRes &c, int64_t R, int form);
-  void build_window_plan(CircuitRes &c);
+  GoodGraph *good_graph(CircuitRes &c, int64_t R, int form, const EngineOptions &eo);
+  void build_window_plan(CircuitRes &c, int levels, int cap);
+  int win_cap(const EngineOptions &eo) const;
+  int lds_cu = 65536, lds_block = 65536;  // LDS bytes per CU and per workgroup
   void finish_start(const std::vector<JobState *> &js, double ms);
   int64_t iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo);
   // Autotuner: sweep kernel and dropping word schedule. Per job-set
@@ -2104,6 +2147,8 @@ Engine::Engine(int device) : impl_(new Impl) {
     hipDeviceProp_t prop;
     LSIM_HIP(hipGetDeviceProperties(&prop, device));
     impl_->cus = prop.multiProcessorCount;
+    impl_->lds_cu = int(prop.maxSharedMemoryPerMultiProcessor);
+    impl_->lds_block = int(prop.sharedMemPerBlock);
     LSIM_HIP(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
     impl_->holder.s = impl_->stream;
     // Stream-ordered releases stay in the device's pool up to 32 GiB instead
@@ -2131,6 +2176,8 @@ Engine::Engine(const Engine &host, void *scratch, size_t bytes) : impl_(new Impl
   Impl &m = *impl_;
   m.device = h.device;
   m.cus = h.cus;
+  m.lds_cu = h.lds_cu;
+  m.lds_block = h.lds_block;
   m.stream = h.stream;  // borrowed: holder stays empty
   m.ev[0] = h.ev[0];
   m.ev[1] = h.ev[1];
@@ -2527,14 +2574,18 @@ int64_t good_items(const CircuitRes &c, int64_t R, int l) {
 }
 }  // namespace
 
-// Splits every window of kWinLevels levels into blocks of at most kWinNodes
-// nodes closed under in-window fanins: the window's outputs (nodes with no
-// fanout inside it) are taken in position order and each adds its in-window
-// backward cone to the current block, which is closed when the cone would
-// overflow it. A cone of kWinLevels levels with at most 4 fanins per node has
-// at most 85 nodes, so it always fits one block. A node's first block owns it.
-void Engine::Impl::build_window_plan(CircuitRes &c) {
+// Cuts the levels into windows of at most `levels` levels and splits every
+// window into blocks of at most `cap` nodes closed under in-window fanins: the
+// window's outputs (nodes with no fanout inside it) are taken in position
+// order and each adds its in-window backward cone to the current block, which
+// is closed when the cone would overflow it. A window whose largest cone
+// exceeds `cap` is cut one level shorter until every cone fits (a one-level
+// window's cones are single nodes). A node's first block owns it.
+void Engine::Impl::build_window_plan(CircuitRes &c, int levels, int cap) {
+  const auto t0 = std::chrono::steady_clock::now();
   auto plan = std::make_unique<WindowPlan>();
+  plan->levels = levels;
+  plan->cap = cap;
   const int64_t N = c.N;
   auto fanins = [&](int32_t u, int32_t *f) {
     const uint32_t type = c.h_info[size_t(u)] & 15u, nin = (c.h_info[size_t(u)] >> 4) & 7u;
@@ -2560,11 +2611,10 @@ void Engine::Impl::build_window_plan(CircuitRes &c) {
   std::vector<uint8_t> owned(size_t(N), 0);
   std::vector<int32_t> in_block(size_t(N), -1);  // index in the current block, by block id stamp
   std::vector<int32_t> stamp(size_t(N), -1), cone_stamp(size_t(N), -1);
-  std::vector<int32_t> cur, cone, stack, outs;
+  std::vector<int32_t> cur, stack, outs, cone_nodes, cone_off;
   int32_t block_id = 0, cone_id = 0;
-  auto flush = [&](int l0) {
+  auto flush = [&](int l0, int depth) {
     if (cur.empty()) return;
-    if (cur.size() > size_t(kWinNodes)) throw DeviceError(LSIM_E_DEVICE, "window block above its node capacity");
     std::sort(cur.begin(), cur.end(), [&](int32_t a, int32_t b) {
       const int32_t la = c.level_dp[size_t(a)], lb = c.level_dp[size_t(b)];
       return la != lb ? la < lb : a < b;
@@ -2576,7 +2626,7 @@ void Engine::Impl::build_window_plan(CircuitRes &c) {
     B.seg[0] = 0;
     for (size_t i = 0; i < cur.size(); ++i) {
       const int32_t u = cur[i];
-      while (k < kWinLevels - 1 && c.level_dp[size_t(u)] > l0 + k) B.seg[++k] = int32_t(i);
+      while (k < depth - 1 && c.level_dp[size_t(u)] > l0 + k) B.seg[++k] = int32_t(i);
       WinNode n;
       n.u = u;
       const bool own = !owned[size_t(u)];
@@ -2594,51 +2644,73 @@ void Engine::Impl::build_window_plan(CircuitRes &c) {
       n.fin = make_int4(e[0], e[1], e[2], e[3]);
       nodes.push_back(n);
     }
-    while (k < kWinLevels) B.seg[++k] = int32_t(cur.size());
+    while (k < kWinLevelsMax) B.seg[++k] = int32_t(cur.size());
     blocks.push_back(B);
     cur.clear();
     ++block_id;
   };
-  for (int l0 = 0; l0 <= c.L; l0 += kWinLevels) {
-    const int l1 = std::min(c.L, l0 + kWinLevels - 1);
-    plan->win_block.push_back(int32_t(blocks.size()));
-    outs.clear();
-    for (int l = l0; l <= l1; ++l) {
-      for (int32_t u = c.live_lvl[size_t(l)]; u < c.live_lvl[size_t(l) + 1]; ++u)
-        if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
-      for (int32_t u = c.dead_lvl[size_t(l)]; u < c.dead_lvl[size_t(l) + 1]; ++u)
-        if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
-    }
-    std::sort(outs.begin(), outs.end());
-    for (int32_t o : outs) {
-      // in-window backward cone of o
-      cone.clear();
-      stack.clear();
-      stack.push_back(o);
-      cone_stamp[size_t(o)] = cone_id;
-      while (!stack.empty()) {
-        const int32_t u = stack.back();
-        stack.pop_back();
-        cone.push_back(u);
-        int32_t f[4];
-        const int k = fanins(u, f);
-        for (int q = 0; q < k; ++q)
-          if (c.level_dp[size_t(f[q])] >= l0 && cone_stamp[size_t(f[q])] != cone_id) {
-            cone_stamp[size_t(f[q])] = cone_id;
-            stack.push_back(f[q]);
-          }
+  for (int l0 = 0; l0 <= c.L;) {
+    // The window's outputs and their in-window backward cones; a window
+    // whose largest cone outgrows a block is cut a level shorter (a one-level
+    // window's cones are single nodes).
+    int depth = std::min(levels, c.L - l0 + 1);
+    for (;;) {
+      const int l1 = l0 + depth - 1;
+      outs.clear();
+      for (int l = l0; l <= l1; ++l) {
+        for (int32_t u = c.live_lvl[size_t(l)]; u < c.live_lvl[size_t(l) + 1]; ++u)
+          if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
+        for (int32_t u = c.dead_lvl[size_t(l)]; u < c.dead_lvl[size_t(l) + 1]; ++u)
+          if (min_fo_level[size_t(u)] > l1) outs.push_back(u);
       }
-      ++cone_id;
+      std::sort(outs.begin(), outs.end());
+      cone_nodes.clear();
+      cone_off.assign(1, 0);
+      bool fits = true;
+      for (int32_t o : outs) {
+        stack.clear();
+        stack.push_back(o);
+        cone_stamp[size_t(o)] = cone_id;
+        const size_t first = cone_nodes.size();
+        while (!stack.empty()) {
+          const int32_t u = stack.back();
+          stack.pop_back();
+          cone_nodes.push_back(u);
+          int32_t f[4];
+          const int k = fanins(u, f);
+          for (int q = 0; q < k; ++q)
+            if (c.level_dp[size_t(f[q])] >= l0 && cone_stamp[size_t(f[q])] != cone_id) {
+              cone_stamp[size_t(f[q])] = cone_id;
+              stack.push_back(f[q]);
+            }
+        }
+        ++cone_id;
+        if (cone_nodes.size() - first > size_t(cap)) {
+          fits = false;
+          break;
+        }
+        cone_off.push_back(int32_t(cone_nodes.size()));
+      }
+      if (fits) break;
+      if (depth == 1) throw DeviceError(LSIM_E_DEVICE, "window block capacity below one node");
+      --depth;
+    }
+    plan->win_block.push_back(int32_t(blocks.size()));
+    plan->win_depth.push_back(depth);
+    for (size_t oi = 0; oi + 1 < cone_off.size(); ++oi) {
+      const int32_t *cn = cone_nodes.data() + cone_off[oi];
+      const int64_t cs = cone_off[oi + 1] - cone_off[oi];
       int64_t fresh = 0;
-      for (int32_t u : cone) fresh += stamp[size_t(u)] == block_id ? 0 : 1;
-      if (int64_t(cur.size()) + fresh > kWinNodes) flush(l0);
-      for (int32_t u : cone)
-        if (stamp[size_t(u)] != block_id) {
-          stamp[size_t(u)] = block_id;
-          cur.push_back(u);
+      for (int64_t q = 0; q < cs; ++q) fresh += stamp[size_t(cn[q])] == block_id ? 0 : 1;
+      if (int64_t(cur.size()) + fresh > cap) flush(l0, depth);
+      for (int64_t q = 0; q < cs; ++q)
+        if (stamp[size_t(cn[q])] != block_id) {
+          stamp[size_t(cn[q])] = block_id;
+          cur.push_back(cn[q]);
         }
     }
-    flush(l0);
+    flush(l0, depth);
+    l0 += depth;
   }
   plan->win_block.push_back(int32_t(blocks.size()));
   for (int64_t u = 0; u < N; ++u)
@@ -2649,12 +2721,27 @@ void Engine::Impl::build_window_plan(CircuitRes &c) {
   h2d(plan->blocks.p, blocks.data(), blocks.size() * sizeof(WinBlock));
   h2d(plan->nodes.p, nodes.data(), nodes.size() * sizeof(WinNode));
   LSIM_HIP(hipStreamSynchronize(stream));
+  if (trace_on())
+    std::fprintf(stderr, "[lsim] window plan: %zu windows of up to %d levels, %zu blocks of up to %d nodes (%zu KB LDS), "
+                 "%lld nodes repeated, built in %.1f ms\n", plan->win_depth.size(), levels, blocks.size(), cap,
+                 win_lds_bytes(cap) / 1024, (long long)plan->repeated,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   c.win = std::move(plan);
 }
 
-GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
+int Engine::Impl::win_cap(const EngineOptions &eo) const {
+  // A block's values (cap nodes x 64 words x 8 B) in LDS: half the CU's LDS,
+  // so two blocks are resident per CU, within the workgroup limit.
+  const int per_node = int(win_lds_bytes(1));
+  const int by_lds = std::max(1, std::min(lds_block, lds_cu / 2) / per_node);
+  return eo.win_cap > 0 ? std::min(eo.win_cap, std::max(1, lds_block / per_node)) : by_lds;
+}
+
+GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form, const EngineOptions &eo) {
+  const int wl = form == 1 ? eo.win_levels : 0, wc = form == 1 ? win_cap(eo) : 0;
   for (size_t i = 0; i < c.graphs.size(); ++i)
-    if (c.graphs[i]->R == R && c.graphs[i]->form == form) {
+    if (c.graphs[i]->R == R && c.graphs[i]->form == form && c.graphs[i]->win_levels == wl &&
+        c.graphs[i]->win_cap == wc) {
       std::unique_ptr<GoodGraph> g = std::move(c.graphs[i]);
       c.graphs.erase(c.graphs.begin() + long(i));
       c.graphs.push_back(std::move(g));
@@ -2663,12 +2750,19 @@ GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
   auto g = std::make_unique<GoodGraph>();
   g->R = R;
   g->form = form;
+  g->win_levels = wl;
+  g->win_cap = wc;
   g->job.alloc(1);
   std::vector<int64_t> off(size_t(c.L + 1) * 2, 0);
   for (int l = 0; l <= c.L; ++l) off[size_t(l) * 2 + 1] = good_items(c, R, l);
   g->off.alloc(off.size());
   h2d(g->off.p, off.data(), off.size() * sizeof(int64_t));
-  if (form == 1 && !c.win) build_window_plan(c);
+  const size_t win_lds = win_lds_bytes(wc);
+  if (form == 1) {
+    if (!c.win || c.win->levels != wl || c.win->cap != wc) build_window_plan(c, wl, wc);
+    LSIM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_good_window),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(win_lds)));
+  }
   LSIM_HIP(hipStreamSynchronize(stream));
   LSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   if (form == 1) {
@@ -2677,8 +2771,8 @@ GoodGraph *Engine::Impl::good_graph(CircuitRes &c, int64_t R, int form) {
     for (size_t k = 0; k + 1 < wp.win_block.size(); ++k) {
       const int nb = wp.win_block[k + 1] - wp.win_block[k];
       if (!nb) continue;
-      k_good_window<<<unsigned(int64_t(nb) * wg), 512, 0, stream>>>(g->job.p, wp.blocks.p, wp.nodes.p,
-                                                                   wp.win_block[k], wg);
+      k_good_window<<<unsigned(int64_t(nb) * wg), 512, win_lds, stream>>>(g->job.p, wp.blocks.p, wp.nodes.p,
+                                                                         wp.win_block[k], wg, wp.win_depth[k], wc);
     }
   } else {
     for (int l = 0; l <= c.L; ++l) {
@@ -2711,7 +2805,7 @@ void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo)
   if (js.size() == 1 && eo.good_graph && js[0]->R > 0) {
     // One job: replay its circuit's captured good-machine graph.
     JobState &j = *js[0];
-    GoodGraph *g = m.good_graph(*j.c, j.R, eo.good_form);
+    GoodGraph *g = m.good_graph(*j.c, j.R, eo.good_form, eo);
     const JobDev d = m.job_dev(j);
     m.time_begin();
     m.h2d(g->job.p, &d, sizeof(JobDev));

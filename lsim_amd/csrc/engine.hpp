@@ -72,7 +72,9 @@ struct EngineOptions {
   int block_waves = 8;               // waves per workgroup of the node-wave persistent sweep (1..8)
   int sweep_kernel = 0;              // persistent sweep: 0 = flattened (node, lane) pairs, 1 = one node per wave
   bool good_graph = true;            // a lone starting job replays its circuit's captured good-machine graph
-  int good_form = 0;                 // good machine of a lone job: 0 one launch per level, 1 level windows
+  int good_form = 1;                 // good machine of a lone job: 0 one launch per level, 1 level windows
+  int win_levels = 4;                // level windows: most levels per window (1..8)
+  int win_cap = 0;                   // level windows: nodes per block (0 = from the device's LDS)
   int pair_threads = 0;              // workgroup size of the pair sweep: 512, 256, or 0 = by step density
   int64_t pool_cap = 0;              // pair sweep: at most this many pool words per slot (0 = as memory allows)
   int pair_dense = 24;               // pair sweep: passes averaging this many active lanes per node go one node per wave (0: never)

@@ -202,18 +202,41 @@ def test_pool_overflow_reruns(c1, drop):
         assert np.array_equal(s.detect_matrix(), det)
 
 
-@pytest.mark.parametrize("form", [0, 1], ids=["per_level", "windows"])
-def test_good_machine_forms_agree(c2, form):
-    """The good machine of a lone job, one launch per level or in LDS-staged
-    level windows: identical last-round good values and detection on C2
-    (100k gates, 120 levels, 128 rounds)."""
+@pytest.mark.parametrize("form,levels,cap", [(0, 4, 0), (1, 4, 0), (1, 8, 0), (1, 2, 48), (1, 8, 40)],
+                         ids=["per_level", "4lv_lds", "8lv_lds", "2lv_48", "8lv_40"])
+def test_good_forms_against_oracle_c1(c1, form, levels, cap):
+    """Both good-machine forms: one launch per level, and the partitioned
+    form (LDS-sized level windows; cap 0 sizes the blocks from the device's
+    LDS, small caps force windows cut short): every d(f, r), the report and
+    the last-round good values against the literal restatement (5k gates, 16
+    rounds)."""
+    ckt, flst, ptn, ref = c1
+    s = _sim(ckt, flst, ptn, good_form=form, win_levels=levels, win_cap=cap)
+    s.run(lsim_amd.Simulator.Mode.SEQUENTIAL)
+    assert np.array_equal(s.detect_matrix(), ref["det"])
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    pi, gate, po = s.good_results()
+    assert np.array_equal(pi, ref["g_pi"]) and np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
+
+
+def test_window_form_against_oracle_c2(c2):
+    """100k gates, 120 levels, 128 rounds: the partitioned good machine's
+    last-round good values against one oracle pass over that round, its
+    detections against the per-level form on every fault and round, and a
+    seeded fault sample against the event-driven restatement."""
     ckt, flst, ptn = c2
-    s = _sim(ckt, flst, ptn, good_form=form)
-    s.run(lsim_amd.Simulator.Mode.PARTITIONED)
-    other = _sim(ckt, flst, ptn, good_form=1 - form, good_graph=0)
-    other.run(lsim_amd.Simulator.Mode.PARTITIONED)
-    assert all(np.array_equal(a, b) for a, b in zip(s.good_results(), other.good_results()))
-    assert np.array_equal(s.detect_matrix(), other.detect_matrix())
+    s = _sim(ckt, flst, ptn, good_form=1)
+    s.run(lsim_amd.Simulator.Mode.SEQUENTIAL)
+    det = s.detect_matrix()
+    good = s.good_results()
+    other = _sim(ckt, flst, ptn, good_form=0)
+    other.run(lsim_amd.Simulator.Mode.SEQUENTIAL)
+    assert np.array_equal(det, other.detect_matrix())
+    o = oracle.Oracle(ckt, flst, ptn)
+    want = o.good_round(o.R - 1)
+    assert all(np.array_equal(a, b) for a, b in zip(good, want))
+    sample = np.sort(np.random.default_rng(2012).choice(o.F, size=200, replace=False))
+    assert np.array_equal(det[sample], o.run_event_range(sample, 0, o.R))
 
 
 def test_repeated_runs_keep_inputs_resident(c1):

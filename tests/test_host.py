@@ -182,6 +182,8 @@ def test_accessor_size_checks():
     ("pair_fo_pre", [0, 1], [-1, 2]),
     ("pair_nopend_pct", [0, 45, 100], [-1, 101]),
     ("pair_threads", [0, 256, 512], [128, 1024]),
+    ("win_levels", [1, 4, 8], [0, 9]),
+    ("win_cap", [0, 96, 4096], [-1, 4097]),
 ])
 def test_option_ranges(key, good, bad):
     """Options are checked on the host: values in range are taken, others
