@@ -197,7 +197,8 @@ def _kernel_name(args, tuned=None) -> str:
 def _tuned(st: dict) -> dict:
     t = int(st["tuned"])
     return {"sweep_kernel": "pairs" if (t & 0xff) == 0 else "nodewave", "first_step_words": (t >> 8) & 0xff,
-            "word_growth": (t >> 16) & 0xff}
+            "word_growth": (t >> 16) & 0xff, "pair_threads": ((t >> 24) & 0xff) * 256,
+            "pair_dense": (t >> 32) & 0xff, "max_step_words": (t >> 40) & 0xffff}
 
 
 def _batch_options(batch, args) -> None:

@@ -103,8 +103,14 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            (default 0 = as memory allows); a tile that needs
  *                            more is swept again in a slot that holds any tile.
  *  "block_waves"        waves per workgroup of the one-node-per-wave sweep (default 8).
- *  "autotune"           0/1  successive runs of a job set each try one (kernel,
- *                            word schedule) candidate, then keep the fastest.
+ *  "autotune"           0/1  successive runs of a job set each measure one sweep
+ *                            configuration, chosen by coordinate descent: the word
+ *                            schedule (first step 1 or 2 words, growth x2 or x4),
+ *                            then the pair sweep's workgroup size (by step, 256,
+ *                            512), its dense-pass threshold (24, 16, 32, never),
+ *                            the step cap (128, 64, 256 words) and the sweep kernel,
+ *                            each with the others at the best so far; later runs
+ *                            use the fastest.
  *  "good_graph"         0/1  a lone job replays its circuit's captured
  *                            good-machine graph (default 1).
  *  "good_form"          good machine of a lone job: 0 one launch per level,
@@ -217,10 +223,11 @@ typedef struct {
   int64_t sweep_reruns;    /* tiles swept again after outgrowing their slot's pool */
   int64_t tune_trials;     /* autotuner candidates measured so far for this job set (0: off) */
   int64_t tuned;           /* configuration used: sweep kernel | first step words << 8 |
-                              word growth << 16 */
+                              word growth << 16 | pair workgroup / 256 << 24 (0: by
+                              step) | dense-pass threshold << 32 | step cap words << 40 */
   int64_t engine;          /* sweep kernel that ran: 0 k_sweep_level (per-level form),
                               1 k_sweep_pairs, 2 k_sweep_events (persistent forms) */
-  int64_t tune_candidates; /* autotuner candidates for this job set (0: off) */
+  int64_t tune_candidates; /* configurations the autotuner measures for this job set (0: off) */
   int64_t device_allocs;   /* device and page-locked host allocations the library made
                               during the call (lsim_run, lsim_run_noalloc; counted over
                               all threads of the process) */

@@ -2077,20 +2077,98 @@ struct Engine::Impl {
   // signature (resident circuits, rounds, dropping, injection mode) the first
   // runs each use one untried candidate and time their word steps; later
   // runs use the fastest. Exploration runs are ordinary runs (same results).
+  // Autotuner: one configuration of the sweep per run of a job set, chosen
+  // by coordinate descent over the dimensions below (the first value of each
+  // is the default): each dimension's values are tried with the others held
+  // at the best found so far, then the best value is kept and the next
+  // dimension is explored; after the last one every run uses the best.
   struct TuneChoice {
-    int kernel = 0, first = 1, growth = 2;
+    int kernel = 0, first = 1, growth = 2, threads = 0, dense = 24, max_words = 128;
+    bool operator==(const TuneChoice &o) const {
+      return kernel == o.kernel && first == o.first && growth == o.growth && threads == o.threads &&
+             dense == o.dense && max_words == o.max_words;
+    }
+    int64_t code() const {
+      return int64_t(kernel) | (int64_t(first) << 8) | (int64_t(growth) << 16) | (int64_t(threads / 256) << 24) |
+             (int64_t(dense) << 32) | (int64_t(max_words) << 40);
+    }
   };
   struct TuneState {
-    std::vector<TuneChoice> cands;
-    std::vector<double> ms;  // measured sweep-loop time per candidate, -1 until tried
-    // The next run's candidate: the first untried one, else the fastest.
-    int pick() const {
-      int best = 0;
-      for (size_t i = 0; i < ms.size(); ++i) {
-        if (ms[i] < 0) return int(i);
-        if (ms[i] < ms[size_t(best)]) best = int(i);
+    std::vector<std::vector<TuneChoice>> dims;  // per dimension: the base with that dimension varied (filled lazily)
+    std::vector<int> dim_kind;                  // 0 schedule, 1 threads, 2 dense, 3 max words, 4 kernel
+    size_t dim = 0;                             // dimension being explored
+    std::vector<TuneChoice> cands;              // configurations measured or in flight, in order
+    std::vector<double> ms;                     // their word-step time, -1 until measured
+    TuneChoice base;                            // the defaults, then the best so far
+    int planned = 1;                            // distinct configurations the descent will measure
+    int best() const {
+      int b = -1;
+      for (size_t i = 0; i < ms.size(); ++i)
+        if (ms[i] >= 0 && (b < 0 || ms[i] < ms[size_t(b)])) b = int(i);
+      return b;
+    }
+    // The variants of dimension `kind` around `b`.
+    static std::vector<TuneChoice> variants(int kind, const TuneChoice &b) {
+      std::vector<TuneChoice> v;
+      auto add = [&](TuneChoice c) { v.push_back(c); };
+      if (kind == 0) {
+        const int sched[4][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}};
+        for (auto &sc : sched) {
+          TuneChoice c = b;
+          c.first = sc[0];
+          c.growth = sc[1];
+          add(c);
+        }
+      } else if (kind == 1) {
+        for (int t : {0, 256, 512}) {
+          TuneChoice c = b;
+          c.threads = t;
+          add(c);
+        }
+      } else if (kind == 2) {
+        for (int d : {24, 16, 32, 0}) {
+          TuneChoice c = b;
+          c.dense = d;
+          add(c);
+        }
+      } else if (kind == 3) {
+        for (int mw : {128, 64, 256}) {
+          TuneChoice c = b;
+          c.max_words = mw;
+          add(c);
+        }
+      } else {
+        for (int k : {0, 1}) {
+          TuneChoice c = b;
+          c.kernel = k;
+          add(c);
+        }
       }
-      return best;
+      return v;
+    }
+    // The next run's configuration: an unmeasured variant of the current
+    // dimension, else (moving on to later dimensions) one of theirs, else
+    // the best measured. Returns its index in cands.
+    int pick() {
+      for (size_t i = 0; i < ms.size(); ++i)
+        if (ms[i] < 0) return int(i);  // a candidate whose run was not a fair sample: again
+      while (dim < dim_kind.size()) {
+        const int b = best();
+        if (b >= 0) base = cands[size_t(b)];
+        for (const TuneChoice &c : variants(dim_kind[dim], base)) {
+          if (std::find(cands.begin(), cands.end(), c) != cands.end()) continue;
+          cands.push_back(c);
+          ms.push_back(-1.0);
+          return int(cands.size()) - 1;
+        }
+        ++dim;
+      }
+      if (cands.empty()) {
+        cands.push_back(base);
+        ms.push_back(-1.0);
+        return 0;
+      }
+      return best();
     }
     int tried() const {
       int n = 0;
@@ -3835,17 +3913,21 @@ Engine::Impl::TuneState &Engine::Impl::tune_state(const std::string &key, const 
   auto it = tuned.find(key);
   if (it != tuned.end()) return it->second;
   TuneState st;
-  const int nk = eo.mode == ENGINE_PLAIN ? 1 : 2;
-  const int sched[4][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}};
-  for (int k = 0; k < nk; ++k)
-    for (int s = 0; s < (any_drop ? 4 : 1); ++s) {
-      TuneChoice c;
-      c.kernel = k;
-      c.first = sched[s][0];
-      c.growth = sched[s][1];
-      st.cands.push_back(c);
-      st.ms.push_back(-1.0);
-    }
+  st.base.kernel = eo.sweep_kernel;
+  st.base.threads = eo.pair_threads;
+  st.base.dense = eo.pair_dense;
+  st.base.max_words = eo.max_step_words;
+  // The word schedule and its cap only matter with dropping; the pair
+  // sweep's workgroup size and dense passes and the kernel only for the
+  // persistent engine.
+  if (any_drop) st.dim_kind.push_back(0);
+  if (eo.mode != ENGINE_PLAIN) {
+    st.dim_kind.push_back(1);
+    st.dim_kind.push_back(2);
+  }
+  if (any_drop) st.dim_kind.push_back(3);
+  if (eo.mode != ENGINE_PLAIN) st.dim_kind.push_back(4);
+  for (int k : st.dim_kind) st.planned += int(TuneState::variants(k, st.base).size()) - 1;
   return tuned.emplace(key, std::move(st)).first->second;
 }
 
@@ -3858,12 +3940,22 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   int first = jobs.empty() ? 1 : jobs[0]->Wt, growth = jobs.empty() ? 2 : jobs[0]->Wg;
   Impl::TuneState *ts = nullptr;
   int cand = -1;
+  Impl::TuneChoice tc;
+  tc.kernel = eo.sweep_kernel;
+  tc.first = first;
+  tc.growth = growth;
+  tc.threads = eo.pair_threads;
+  tc.dense = eo.pair_dense;
+  tc.max_words = eo.max_step_words;
   if (eo.autotune && !jobs.empty()) {
     bool any_drop = false;
     ts = &m.tune_state(m.tune_key(jobs, eo, any_drop), eo, any_drop);
     cand = ts->pick();
-    const Impl::TuneChoice &tc = ts->cands[size_t(cand)];
+    tc = ts->cands[size_t(cand)];
     e2.sweep_kernel = tc.kernel;
+    e2.pair_threads = tc.threads;
+    e2.pair_dense = tc.dense;
+    e2.max_step_words = tc.max_words;
     first = tc.first;
     growth = tc.growth;
     for (JobState *j : jobs)
@@ -3884,15 +3976,17 @@ void Engine::run(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
     double &slot = ts->ms[size_t(cand)];
     slot = slot < 0 ? steps_ms : std::min(slot, steps_ms);
     if (trace_on())
-      std::fprintf(stderr, "[lsim] autotune kernel=%d first=%d growth=%d steps %.2f ms (%d of %zu tried)\n",
-                   e2.sweep_kernel, first, growth, steps_ms, ts->tried(), ts->cands.size());
+      std::fprintf(stderr,
+                   "[lsim] autotune kernel=%d first=%d growth=%d threads=%d dense=%d max_words=%d steps %.2f ms "
+                   "(%d of %d tried)\n",
+                   tc.kernel, first, growth, tc.threads, tc.dense, tc.max_words, steps_ms, ts->tried(), ts->planned);
   }
   const double ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
   for (JobState *j : jobs) {
     j->res.stats.t_total_ms = ms;
     j->res.stats.tune_trials = ts ? ts->tried() : 0;
-    j->res.stats.tune_candidates = ts ? int64_t(ts->cands.size()) : 0;
-    j->res.stats.tuned = int64_t(e2.sweep_kernel) | (int64_t(first) << 8) | (int64_t(growth) << 16);
+    j->res.stats.tune_candidates = ts ? int64_t(ts->planned) : 0;
+    j->res.stats.tuned = tc.code();
     j->res.stats.engine = e2.mode == ENGINE_PLAIN ? 0 : (e2.sweep_kernel == 0 ? 1 : 2);
   }
 }

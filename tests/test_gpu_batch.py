@@ -162,31 +162,39 @@ def test_server_rejects_job_above_budget():
     srv.close()
 
 
-def test_autotuned_batch_matches_single_runs():
-    """With the autotuner on, successive batches over the same circuits try
-    every kernel / word-schedule candidate (a run that allocated sweep
-    scratch is not timed, so that candidate runs again) and then keep the
-    fastest; every batch's results are those of the untuned single runs."""
+def test_autotuned_batch_matches_oracle():
+    """With the autotuner on, successive batches over the same circuits each
+    measure one sweep configuration (coordinate descent over word schedule,
+    pair workgroup size, dense-pass threshold, step cap and kernel: 12
+    distinct configurations for dropping jobs; a run that allocated sweep
+    scratch is not timed, so its configuration runs again), then keep the
+    fastest; every batch's first detecting rounds equal the oracle's over all
+    rounds, for every fault."""
     b = lsim_amd.Batch(device=0, autotune=1)
     cids, texts = [], []
     for spec in SPECS[:3]:
         ckt, flst, ptn = _texts(spec)
         cids.append(b.add_circuit(ckt))
         texts.append((ckt, flst, ptn))
-    want = [_single(ck, f, p, True)["first"] for ck, f, p in texts]
-    seen = set()
+    want = []
+    for ck, f, p in texts:
+        o = oracle.Oracle(ck, f, p)
+        want.append(o.first_rounds(np.arange(o.F)))
+    seen = []
     trials = 0
-    for rnd in range(16):
+    for rnd in range(20):
         jobs = [b.add_job(c, f, p, drop=True) for c, (ck, f, p) in zip(cids, texts)]
         b.run()
         st = b.last_run_stats()
-        seen.add(st["tuned"])
+        seen.append(st["tuned"])
         assert trials <= st["tune_trials"] <= trials + 1
         trials = st["tune_trials"]
         for jid, w in zip(jobs, want):
-            assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid)
+            assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid, st["tuned"])
             b.release_job(jid)
-    assert trials == 8 and len(seen) == 8  # 2 kernels x 4 word schedules
+    assert st["tune_candidates"] == 12 and trials == 12
+    assert len(set(seen)) == 12
+    assert len(set(seen[-3:])) == 1  # explored: the fastest is kept
 
 
 def test_server_two_lanes_match_single_runs():
