@@ -228,6 +228,7 @@ typedef struct {
   int64_t engine;          /* sweep kernel that ran: 0 k_sweep_level (per-level form),
                               1 k_sweep_pairs, 2 k_sweep_events (persistent forms) */
   int64_t tune_candidates; /* configurations the autotuner measures for this job set (0: off) */
+  int64_t good_replayed;   /* 1 when the good machine replayed a captured hipGraph */
   int64_t device_allocs;   /* device and page-locked host allocations the library made
                               during the call (lsim_run, lsim_run_noalloc; counted over
                               all threads of the process) */
@@ -273,8 +274,12 @@ LSIM_API int lsim_batch_release_job(lsim_batch *b, int64_t job_id);
  * lane's sweeps take a quarter of the workgroup slots and the other lane the
  * rest. */
 typedef struct lsim_server lsim_server;
-/* job_budget_mb: HBM for admitted jobs (good values, fault state, results);
- * scratch_mb: HBM for sweep tiles (0 = 90 % of free HBM less the job budget and 4 GiB). */
+/* job_budget_mb: HBM for the resident circuits (one device copy serves both
+ * lanes) and the admitted jobs (good values, fault state, results); a job that
+ * cannot fit beside the circuits fails at submit with LSIM_E_NOMEM.
+ * scratch_mb: HBM for sweep tiles (0 = 90 % of free HBM less the job budget and 4 GiB).
+ * When an iteration fails, each of its jobs is run again alone from its
+ * start, so only a job that fails by itself reports the error. */
 LSIM_API int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, lsim_server **out);
 LSIM_API void lsim_server_destroy(lsim_server *s);
 LSIM_API const char *lsim_server_last_error(const lsim_server *s);

@@ -59,6 +59,7 @@ class RunStats(ctypes.Structure):
         ("tuned", ctypes.c_int64),
         ("engine", ctypes.c_int64),
         ("tune_candidates", ctypes.c_int64),
+        ("good_replayed", ctypes.c_int64),
         ("device_allocs", ctypes.c_int64),
     ]
 

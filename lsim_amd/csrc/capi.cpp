@@ -590,6 +590,7 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.sweep_bytes = std::max(b->stats.sweep_bytes, s.sweep_bytes);
       b->stats.tune_trials = s.tune_trials;
       b->stats.tune_candidates = s.tune_candidates;
+      b->stats.good_replayed = s.good_replayed;
       b->stats.sweep_reruns = std::max(b->stats.sweep_reruns, s.sweep_reruns);
       b->stats.tuned = s.tuned;
       b->stats.engine = s.engine;
@@ -704,7 +705,8 @@ struct ServerLane {
 
 struct lsim_server {
   ServerLane lane[2];
-  int64_t budget = 0, used = 0;
+  int64_t budget = 0, used = 0;  // HBM budget; held by resident circuits and admitted jobs
+  int64_t circuit_bytes = 0;     // held by resident circuits (one device copy for both lanes)
   std::mutex mu;  // queues, job table, counters, budget
   std::condition_variable cv_done;
   std::vector<std::unique_ptr<ResidentCircuit>> circuits;
@@ -782,12 +784,28 @@ void lsim_server::loop(int l) {
       std::vector<ServerJob *> still;
       for (ServerJob *j : live) {
         if (rc) {
-          // The iteration failed: its jobs fail, the server keeps serving.
-          j->code = rc;
-          j->error = msg;
-          L.engine->release_job(j->js);
-          j->js = nullptr;
-          failed_now.push_back(j);
+          // The iteration failed. Each of its jobs runs again alone from its
+          // start, so that one job's failure does not fail the others; a job
+          // that fails alone too reports its own error.
+          std::string msg1;
+          const int rc1 = guarded(&msg1, [&] {
+            if (!L.engine->reset_job(j->js, j->opt)) throw lsim::DeviceError(LSIM_E_STATE, msg);
+            lsim::EngineOptions alone = eo;
+            alone.autotune = false;
+            L.engine->run({j->js}, alone);
+          });
+          if (rc1) {
+            j->code = rc1;
+            j->error = msg1;
+            L.engine->release_job(j->js);
+            j->js = nullptr;
+            failed_now.push_back(j);
+          } else {
+            j->res = std::move(L.engine->results(j->js));
+            L.engine->release_job(j->js);
+            j->js = nullptr;
+            done_now.push_back(j);
+          }
         } else if (L.engine->done(j->js)) {
           j->res = std::move(L.engine->results(j->js));
           L.engine->release_job(j->js);
@@ -840,8 +858,9 @@ int lsim_server_create(int device, int64_t job_budget_mb, int64_t scratch_mb, ls
     }
     // Small jobs: 30 % of the scratch and a quarter of the slots while large
     // jobs run; large jobs: 70 % and the other three quarters.
-    s->lane[0].eopt.scratch_mb = std::max<int64_t>(512, scratch * 3 / 10);
-    s->lane[1].eopt.scratch_mb = std::max<int64_t>(512, scratch - s->lane[0].eopt.scratch_mb);
+    const int64_t floor_mb = scratch_mb > 0 ? 32 : 512;  // an explicit scratch size is taken as given
+    s->lane[0].eopt.scratch_mb = std::max<int64_t>(floor_mb, scratch * 3 / 10);
+    s->lane[1].eopt.scratch_mb = std::max<int64_t>(floor_mb, scratch - s->lane[0].eopt.scratch_mb);
     s->lane[0].busy_share = 0.25;
     s->lane[1].busy_share = 0.75;
     s->lane[1].max_running = 1;
@@ -904,14 +923,19 @@ int lsim_server_add_circuit(lsim_server *s, const char *ckt, size_t ckt_len, int
       std::lock_guard<std::mutex> elk(s->lane[0].engine_mu);
       rc_ptr->res = s->lane[0].engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
     }
+    // One device copy serves both lanes (lane 0 owns it).
     std::lock_guard<std::mutex> elk(s->lane[1].engine_mu);
-    rc_ptr->res2 = s->lane[1].engine->add_circuit(rc_ptr->nl, rc_ptr->fs);
+    rc_ptr->res2 = s->lane[1].engine->share_circuit(rc_ptr->res);
   });
   std::lock_guard<std::mutex> lk(s->mu);
   if (rc) {
     s->err = msg;
     return rc;
   }
+  // The resident circuit counts against the HBM budget that admission uses.
+  const int64_t cb = s->lane[0].engine->circuit_bytes(rc_ptr->res);
+  s->used += cb;
+  s->circuit_bytes += cb;
   *circuit_id = int64_t(s->circuits.size());
   s->circuits.push_back(std::move(rc_ptr));
   return LSIM_OK;
@@ -955,8 +979,9 @@ int lsim_server_submit(lsim_server *s, int64_t circuit_id, const char *flst, siz
   j->lane = (s->big_evals > 0 && (evals >= s->big_evals || node_words >= s->big_node_words)) ? 1 : 0;
   ServerJob *jp = j.get();
   s->jobs[j->id] = std::move(j);
-  if (jp->bytes > s->budget) {
-    // Larger than the whole job budget: it could never be admitted.
+  if (jp->bytes > s->budget - s->circuit_bytes) {
+    // Larger than what the budget leaves beside the resident circuits: it
+    // could never be admitted.
     jp->state = 3;
     jp->code = LSIM_E_NOMEM;
     jp->error = "job needs " + std::to_string(jp->bytes) + " bytes, above the server's job budget";

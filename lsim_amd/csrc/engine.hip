@@ -1852,6 +1852,22 @@ struct GoodGraph {
   }
 };
 
+// The batched good machine of a job set (one launch per level index over all
+// jobs), captured once the same set of (circuit, rounds) recurs and replayed
+// with the jobs' records copied in first.
+struct BatchGoodGraph {
+  std::string key;
+  std::vector<const void *> circuits;
+  DevBuf<JobDev> jobs;
+  DevBuf<int64_t> off;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchGoodGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 // Level-window blocks of one circuit (built on first use).
 struct WindowPlan {
   int levels = 0, cap = 0;         // window depth asked for, block capacity in nodes
@@ -2177,6 +2193,10 @@ struct Engine::Impl {
     }
   };
   std::map<std::string, TuneState> tuned;
+  // Batched good-machine graphs (most recent last) and the job-set keys seen
+  // once (a key is captured when it comes again).
+  std::vector<std::unique_ptr<BatchGoodGraph>> batch_graphs;
+  std::vector<std::string> batch_keys_seen;
   uint64_t setup_events = 0;  // sweep scratch (re)allocations: a run that made one is not timed for the tuner
   std::string tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo, bool &any_drop) const;
   TuneState &tune_state(const std::string &key, const EngineOptions &eo, bool any_drop);
@@ -2434,11 +2454,54 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   return m.circuits.back().get();
 }
 
+CircuitRes *Engine::share_circuit(const CircuitRes *src) {
+  Impl &m = *impl_;
+  auto cp = std::make_unique<CircuitRes>();
+  CircuitRes &c = *cp;
+  c.N = src->N;
+  c.NL = src->NL;
+  c.P = src->P;
+  c.G = src->G;
+  c.O = src->O;
+  c.E = src->E;
+  c.L = src->L;
+  c.h_info = src->h_info;
+  c.h_fin = src->h_fin;
+  c.wide = src->wide;
+  c.has_mux = src->has_mux;
+  c.pos2dp = src->pos2dp;
+  c.dp2pos = src->dp2pos;
+  c.level_dp = src->level_dp;
+  c.live_lvl = src->live_lvl;
+  c.dead_lvl = src->dead_lvl;
+  c.head_dp = src->head_dp;
+  c.head_is_po = src->head_is_po;
+  // The device arrays are read only: views of src's, never freed here.
+  auto share = [](auto &dst, const auto &from) { dst.view(from.p, from.n); };
+  share(c.info, src->info);
+  share(c.fin, src->fin);
+  share(c.parent, src->parent);
+  share(c.d_live_lvl, src->d_live_lvl);
+  share(c.d_dead_lvl, src->d_dead_lvl);
+  share(c.fo_off, src->fo_off);
+  share(c.fo, src->fo);
+  share(c.is_po, src->is_po);
+  m.circuits.push_back(std::move(cp));
+  return m.circuits.back().get();
+}
+
 void Engine::remove_circuit(CircuitRes *c) {
   Impl &m = *impl_;
   LSIM_HIP(hipSetDevice(m.device));
   AllocStream alloc_on(m.stream, m.arena);
   LSIM_HIP(hipStreamSynchronize(m.stream));
+  // Batched graphs and remembered job sets that name the circuit go with it.
+  for (size_t i = m.batch_graphs.size(); i-- > 0;) {
+    const auto &cs = m.batch_graphs[i]->circuits;
+    if (std::find(cs.begin(), cs.end(), static_cast<const void *>(c)) != cs.end())
+      m.batch_graphs.erase(m.batch_graphs.begin() + long(i));
+  }
+  m.batch_keys_seen.clear();
   for (auto it = m.circuits.begin(); it != m.circuits.end(); ++it)
     if (it->get() == c) {
       m.circuits.erase(it);
@@ -2890,6 +2953,7 @@ void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo)
     LSIM_HIP(hipGraphLaunch(g->exec, m.stream));
     const double ms = m.time_end();
     m.finish_start(js, ms);
+    j.res.stats.good_replayed = 1;
     return;
   }
   // One launch per level index over all jobs' (node, word) pairs of that level.
@@ -2907,6 +2971,63 @@ void Engine::start(const std::vector<JobState *> &jobs, const EngineOptions &eo)
       if (js[i]->R > 0 && l <= c.L)
         items = int64_t(c.live_lvl[l + 1] - c.live_lvl[l] + c.dead_lvl[l + 1] - c.dead_lvl[l]) * js[i]->R;
       o[i + 1] = o[i] + items;
+    }
+  }
+  if (eo.good_graph) {
+    // The set's key: every job's circuit (with its shape) and rounds, in order.
+    std::string key;
+    for (JobState *j : js) {
+      char buf[96];
+      std::snprintf(buf, sizeof(buf), "%p/%lld/%lld/%d/%lld;", static_cast<const void *>(j->c), (long long)j->c->N,
+                    (long long)j->c->NL, j->c->L, (long long)j->R);
+      key += buf;
+    }
+    BatchGoodGraph *g = nullptr;
+    for (size_t i = 0; i < m.batch_graphs.size(); ++i)
+      if (m.batch_graphs[i]->key == key) {
+        std::unique_ptr<BatchGoodGraph> keep = std::move(m.batch_graphs[i]);
+        m.batch_graphs.erase(m.batch_graphs.begin() + long(i));
+        m.batch_graphs.push_back(std::move(keep));
+        g = m.batch_graphs.back().get();
+        break;
+      }
+    if (!g && std::find(m.batch_keys_seen.begin(), m.batch_keys_seen.end(), key) != m.batch_keys_seen.end()) {
+      auto ng = std::make_unique<BatchGoodGraph>();
+      ng->key = key;
+      for (JobState *j : js) ng->circuits.push_back(j->c);
+      ng->jobs.alloc(nj);
+      ng->off.alloc(off.size());
+      m.h2d(ng->off.p, off.data(), off.size() * sizeof(int64_t));
+      LSIM_HIP(hipStreamSynchronize(m.stream));
+      LSIM_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+      for (int l = 0; l <= Lmax; ++l) {
+        const int64_t total = off[size_t(l) * (nj + 1) + nj];
+        if (!total) continue;
+        k_good_level<<<blocks_for(total, 256), 256, 0, m.stream>>>(ng->jobs.p, nj, ng->off.p + size_t(l) * (nj + 1), l);
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(m.stream, &graph);
+      if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        LSIM_HIP(e);
+      }
+      ng->graph = graph;
+      LSIM_HIP(hipGraphInstantiate(&ng->exec, ng->graph, nullptr, nullptr, 0));
+      if (m.batch_graphs.size() >= kMaxGraphs) m.batch_graphs.erase(m.batch_graphs.begin());
+      m.batch_graphs.push_back(std::move(ng));
+      g = m.batch_graphs.back().get();
+    }
+    if (g) {
+      m.time_begin();
+      m.h2d(g->jobs.p, devs.data(), nj * sizeof(JobDev));
+      LSIM_HIP(hipGraphLaunch(g->exec, m.stream));
+      m.finish_start(js, m.time_end());
+      for (JobState *j : js) j->res.stats.good_replayed = 1;
+      return;
+    }
+    if (std::find(m.batch_keys_seen.begin(), m.batch_keys_seen.end(), key) == m.batch_keys_seen.end()) {
+      if (m.batch_keys_seen.size() >= 16) m.batch_keys_seen.erase(m.batch_keys_seen.begin());
+      m.batch_keys_seen.push_back(key);
     }
   }
   m.d_jobs.alloc(nj);
@@ -3372,9 +3493,16 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   if (pc512 < 1)
     throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
   // Sized for the largest resident circuit, so later jobs on it reuse the
-  // scratch instead of reallocating it.
-  for (const auto &c : circuits) nl_max = std::max(nl_max, c->NL);
-  ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  // scratch instead of reallocating it; sized for this step's circuits when
+  // the budget cannot hold a slot of the largest resident one.
+  int64_t nl_all = nl_max;
+  for (const auto &c : circuits) nl_all = std::max(nl_all, c->NL);
+  try {
+    ensure_pair_scratch(eo, nl_all, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  } catch (const DeviceError &e) {
+    if (e.code != LSIM_E_NOMEM || nl_all == nl_max) throw;
+    ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  }
   // Dense or wide steps use 512-thread workgroups, the others 256-thread
   // ones, twice as many tiles at once. (128-thread groups, 8 tiles per CU,
   // were slower in every word step of the 2M-gate workload.)

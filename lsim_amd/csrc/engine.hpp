@@ -120,6 +120,10 @@ class Engine {
 
   // Makes a circuit resident. The engine owns it until remove_circuit.
   CircuitRes *add_circuit(const Netlist &nl, const FaultStructure &fs);
+  // The same circuit in this engine, reading src's device arrays (another
+  // engine on the same device, which must keep src resident while this one
+  // uses it): no second upload.
+  CircuitRes *share_circuit(const CircuitRes *src);
   void remove_circuit(CircuitRes *c);
   int64_t circuit_bytes(const CircuitRes *c) const;
 

@@ -229,3 +229,64 @@ def test_server_two_lanes_match_single_runs():
     cnt = srv.counters()
     assert cnt["finished"] == len(submitted) and cnt["failed"] == 0
     srv.close()
+
+
+def test_batched_good_machine_graph_replays():
+    """The batched good machine is captured in a graph once a job set
+    recurs and replayed after that; four batches of the same set, with an
+    eager batch of another set on another engine between them, all give the
+    oracle's first detecting rounds, and the graph is replayed from the
+    second batch on."""
+    b = lsim_amd.Batch(device=0)
+    other = lsim_amd.Batch(device=0, good_graph=0)
+    cids, texts, want = [], [], []
+    for spec in SPECS[:4]:
+        ckt, flst, ptn = _texts(spec)
+        cids.append(b.add_circuit(ckt))
+        texts.append((ckt, flst, ptn))
+        o = oracle.Oracle(ckt, flst, ptn)
+        want.append(o.first_rounds(np.arange(o.F)))
+    ocid = other.add_circuit(texts[0][0])
+    replayed = []
+    for rnd in range(4):
+        jobs = [b.add_job(c, f, p, drop=True) for c, (ck, f, p) in zip(cids, texts)]
+        b.run()
+        replayed.append(b.last_run_stats()["good_replayed"])
+        for jid, w in zip(jobs, want):
+            assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid)
+            b.release_job(jid)
+        # eager work of another engine between the replays
+        oj = [other.add_job(ocid, texts[0][1], texts[0][2], drop=False), other.add_job(ocid, texts[0][1], texts[0][2],
+                                                                                       drop=True)]
+        other.run()
+        assert other.last_run_stats()["good_replayed"] == 0
+        assert np.array_equal(other.first_detect_round(oj[1]), want[0])
+        for j in oj:
+            other.release_job(j)
+    assert replayed == [0, 1, 1, 1]
+
+
+def test_server_failure_fails_only_its_job():
+    """A job whose circuit is too large for the lane's sweep scratch (32 MiB:
+    a tile slot of a 600k-gate circuit does not fit) fails with
+    LSIM_E_NOMEM; a small job batched in the same lane at the same time is
+    run again alone and finishes with the single-simulator result."""
+    srv = lsim_amd.Server(device=0, job_budget_mb=16 * 1024, scratch_mb=100)
+    srv.set_option("big_evals", 0)  # one lane for both jobs
+    small = _texts(SPECS[0])
+    big_ckt, big_flst, big_ptn = lsim_amd.generate(seed=77, num_pis=128, num_pos=64, num_gates=600000,
+                                                   num_levels=200, num_patterns=256, fanout_cap=200,
+                                                   fault_fraction=0.001)
+    cs = srv.add_circuit(small[0])
+    cb = srv.add_circuit(big_ckt)
+    jb = srv.submit(cb, big_flst, big_ptn)
+    js = srv.submit(cs, small[1], small[2])
+    with pytest.raises(lsim_amd.LsimError) as e:
+        srv.wait(jb, 120)
+    assert e.value.code == lsim_amd.LSIM_E_NOMEM
+    srv.wait(js, 120)
+    want = _single(small[0], small[1], small[2], True)["first"]
+    assert np.array_equal(srv.first_detect_round(js), want)
+    cnt = srv.counters()
+    assert cnt["failed"] == 1 and cnt["finished"] >= 1
+    srv.close()
