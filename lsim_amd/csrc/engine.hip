@@ -1045,9 +1045,10 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
                 if (kStats) ++n_pairs;
               }
               if (d) {
-                // Dense passes write only the differing lanes: most lanes are
-                // active, so the node's block is mostly written anyway.
-                pool[pb + uint32_t(s_pref[kk[i]]) + lanes_below(act[i])] = r[i];
+                // Dense passes write only the differing lanes, packed: the
+                // node's record then indexes its block by the differing lanes
+                // (its `act` is nm), so readers touch only written words.
+                pool[pb + uint32_t(s_pref[kk[i]]) + lanes_below(nm)] = r[i];
                 if (type[i] == G_PO) atomicOr(&s_acc[lane], (unsigned long long)d);
                 bytes += 8;
               }
@@ -1136,7 +1137,10 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             const int v = s_u[tid];
             PairRec r;
             r.nm = nm;
-            r.act = s_act[tid];
+            // The mask that indexes the node's pool block: its active lanes
+            // (flattened passes write every active lane), or the differing
+            // lanes where a dense pass packed only those.
+            r.act = dense_pass ? nm : s_act[tid];
             r.epoch = epoch;
             r.base = pb + uint32_t(s_pref[tid]);
             r.pad = 0;
