@@ -3508,8 +3508,8 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
     ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
   }
   // Dense or wide steps use 512-thread workgroups, the others 256-thread
-  // ones, twice as many tiles at once. (128-thread groups, 8 tiles per CU,
-  // were slower in every word step of the 2M-gate workload.)
+  // ones, twice as many tiles at once. (128- and 192-thread groups, 8 and 5
+  // tiles per CU, were slower in every word step of the 2M-gate workload.)
   int threads = eo.pair_threads ? eo.pair_threads : (dense ? 512 : 256);
   if (threads == 256 && pc256 < 1) threads = 512;
   const int per_cu = threads == 256 ? pc256 : pc512;
