@@ -104,6 +104,11 @@ struct lsim_sim {
   lsim::FaultStructure fs;
   lsim::CircuitRes *circuit = nullptr;
   lsim::JobState *job = nullptr;  // the inputs' job, kept resident from run to run
+  // Circuits with MUX: the faults whose cone reaches a MUX select input are
+  // injected directly (job2); the others keep stem tracing (job).
+  lsim::JobState *job2 = nullptr;
+  bool job_split = false;
+  std::vector<int32_t> split_stem, split_direct;  // fault indices of job and job2
   bool loaded = false;
   bool ran = false;
   lsim::JobOptions jopt;
@@ -114,22 +119,81 @@ struct lsim_sim {
 
 namespace {
 
+// The reference's MUX reads its select as a whole word (`!s`,
+// fsim/simulation.cpp:1056), which couples the 64 patterns of a word. A fault
+// whose cone reaches no MUX select propagates pattern by pattern (a MUX's data
+// inputs do), so stem tracing stays exact for it; the others are injected
+// directly. Splits the fault indices accordingly (empty lists without MUX).
+void mux_split(const lsim::Netlist &nl, const lsim::FaultStructure &fs, std::vector<int32_t> &stem,
+               std::vector<int32_t> &direct) {
+  stem.clear();
+  direct.clear();
+  if (!nl.has_mux) return;
+  const int64_t N = nl.N;
+  std::vector<uint8_t> reach(size_t(N), 0);  // reaches a MUX select input
+  for (int64_t p = 0; p < N; ++p)
+    if (nl.type[size_t(p)] == lsim::G_MUX && nl.fin_off[p + 1] - nl.fin_off[p] >= 3)
+      reach[size_t(nl.fin[size_t(nl.fin_off[p] + 2)])] = 1;
+  for (int64_t p = N - 1; p >= 0; --p)  // fanouts sit at higher positions
+    for (int32_t e = fs.fo_off[p]; e < fs.fo_off[p + 1] && !reach[size_t(p)]; ++e)
+      reach[size_t(p)] = reach[size_t(fs.fo[e])];
+  for (size_t f = 0; f < nl.fault_site.size(); ++f)
+    (reach[size_t(nl.fault_site[f])] ? direct : stem).push_back(int32_t(f));
+}
+
+void release_jobs(lsim_sim *s) {
+  for (lsim::JobState **j : {&s->job, &s->job2})
+    if (s->engine && *j) {
+      s->engine->release_job(*j);
+      *j = nullptr;
+    }
+  s->job_split = false;
+}
+
+// Results of the split jobs in the fault order of the inputs.
+lsim::RunResults merge_split(lsim::RunResults &a, const std::vector<int32_t> &ia, lsim::RunResults &b,
+                             const std::vector<int32_t> &ib, int64_t F) {
+  lsim::RunResults out;
+  out.good_last = std::move(a.good_last);
+  out.RB = a.RB;
+  out.has_matrix = a.has_matrix;
+  out.has_last = a.has_last;
+  out.first_round.resize(size_t(F));
+  out.last_round.resize(size_t(F));
+  if (out.has_matrix) out.det_bits.assign(size_t(F * out.RB), 0);
+  for (int part = 0; part < 2; ++part) {
+    const lsim::RunResults &r = part ? b : a;
+    const std::vector<int32_t> &idx = part ? ib : ia;
+    for (size_t k = 0; k < idx.size(); ++k) {
+      const size_t f = size_t(idx[k]);
+      out.first_round[f] = r.first_round[k];
+      out.last_round[f] = r.has_last ? r.last_round[k] : uint8_t(0);
+      if (out.has_matrix)
+        std::memcpy(out.det_bits.data() + f * size_t(out.RB), r.det_bits.data() + k * size_t(r.RB),
+                    size_t(out.RB) * sizeof(uint64_t));
+    }
+  }
+  out.stats = a.stats;  // the two ran together: shared device times
+  out.stats.live_faults += b.stats.live_faults;
+  out.stats.stems += b.stats.stems;
+  out.stats.sweep_bytes = std::max(a.stats.sweep_bytes, b.stats.sweep_bytes);
+  return out;
+}
+
 int load_texts(lsim_sim *s, const char *ckt, size_t ckt_len, const char *flst, size_t flst_len, const char *ptn,
                size_t ptn_len) {
   return guarded(&s->err, [&] {
     s->loaded = false;
     s->ran = false;
     s->res = lsim::RunResults();
-    if (s->engine && s->job) {
-      s->engine->release_job(s->job);
-      s->job = nullptr;
-    }
+    release_jobs(s);
     if (s->engine && s->circuit) {
       s->engine->remove_circuit(s->circuit);
       s->circuit = nullptr;
     }
     lsim::read_netlist(s->nl, ckt, ckt_len, flst, flst_len, ptn, ptn_len);
     lsim::build_fault_structure(s->nl, s->fs);
+    mux_split(s->nl, s->fs, s->split_stem, s->split_direct);
     if (s->engine) s->circuit = s->engine->add_circuit(s->nl, s->fs);
     s->loaded = true;
   });
@@ -291,21 +355,62 @@ int lsim_run(lsim_sim *s, int mode, size_t num_threads) {
     eo.mode = m;
     lsim::JobOptions jo = s->jopt;
     jo.keep_good_last = true;
-    // The fault list and patterns stay resident between runs of the same
-    // inputs; only a change of dropping or last-round reporting rebuilds them.
-    if (s->job && !s->engine->reset_job(s->job, jo)) {
-      s->engine->release_job(s->job);
-      s->job = nullptr;
+    // In a circuit with MUX the faults whose cone reaches no MUX select keep
+    // stem tracing; the others are injected directly (mux_split).
+    const bool split = !jo.direct && !s->split_stem.empty() && !s->split_direct.empty();
+    if (split != s->job_split) release_jobs(s);
+    if (s->nl.has_mux && s->split_direct.empty()) jo.mux_safe = true;
+    lsim::JobOptions jd = jo;
+    if (split) {
+      jo.mux_safe = true;
+      jd.direct = true;
     }
-    if (!s->job)
-      s->job = s->engine->add_job(s->circuit, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns, s->nl.R,
-                                  s->nl.patterns, jo);
+    // The fault lists and patterns stay resident between runs of the same
+    // inputs; only a change of dropping or last-round reporting rebuilds them.
+    for (lsim::JobState **j : {&s->job, &s->job2})
+      if (*j && !s->engine->reset_job(*j, j == &s->job ? jo : jd)) {
+        s->engine->release_job(*j);
+        *j = nullptr;
+      }
+    auto subset = [&](const std::vector<int32_t> &idx, std::vector<int32_t> &site, std::vector<uint8_t> &sa) {
+      site.resize(idx.size());
+      sa.resize(idx.size());
+      for (size_t k = 0; k < idx.size(); ++k) {
+        site[k] = s->nl.fault_site[size_t(idx[k])];
+        sa[k] = s->nl.fault_sa[size_t(idx[k])];
+      }
+    };
+    if (!s->job) {
+      if (split) {
+        std::vector<int32_t> site;
+        std::vector<uint8_t> sa;
+        subset(s->split_stem, site, sa);
+        s->job = s->engine->add_job(s->circuit, std::move(site), std::move(sa), s->nl.num_patterns, s->nl.R,
+                                    s->nl.patterns, jo);
+      } else {
+        s->job = s->engine->add_job(s->circuit, s->nl.fault_site, s->nl.fault_sa, s->nl.num_patterns, s->nl.R,
+                                    s->nl.patterns, jo);
+      }
+    }
+    if (split && !s->job2) {
+      std::vector<int32_t> site;
+      std::vector<uint8_t> sa;
+      subset(s->split_direct, site, sa);
+      s->job2 = s->engine->add_job(s->circuit, std::move(site), std::move(sa), s->nl.num_patterns, s->nl.R,
+                                   s->nl.patterns, jd);
+    }
+    s->job_split = split;
     try {
-      s->engine->run({s->job}, eo);
-      s->res = std::move(s->engine->results(s->job));
+      if (split) {
+        s->engine->run({s->job, s->job2}, eo);
+        s->res = merge_split(s->engine->results(s->job), s->split_stem, s->engine->results(s->job2),
+                             s->split_direct, int64_t(s->nl.fault_site.size()));
+      } else {
+        s->engine->run({s->job}, eo);
+        s->res = std::move(s->engine->results(s->job));
+      }
     } catch (...) {
-      s->engine->release_job(s->job);
-      s->job = nullptr;
+      release_jobs(s);
       throw;
     }
     s->res.stats.device_allocs = lsim::device_allocations() - allocs0;

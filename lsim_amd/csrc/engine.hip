@@ -2554,7 +2554,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   JobState &j = *jp;
   j.c = c;
   j.opt = opt;
-  j.direct = opt.direct || c->has_mux;
+  j.direct = opt.direct || (c->has_mux && !opt.mux_safe);
   j.F = int64_t(fault_site.size());
   j.R = R;
   j.num_patterns = num_patterns;
@@ -2653,7 +2653,7 @@ bool Engine::reset_job(JobState *j, const JobOptions &opt) {
   LSIM_HIP(hipSetDevice(m.device));
   AllocStream alloc_on(m.stream, m.arena);
   j->opt = opt;
-  j->direct = opt.direct || j->c->has_mux;
+  j->direct = opt.direct || (j->c->has_mux && !opt.mux_safe);
   j->started = j->done = j->last_done = false;
   j->w0 = 0;
   j->Wt = std::max(1, std::min(64, opt.first_words));

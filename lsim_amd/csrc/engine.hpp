@@ -55,6 +55,8 @@ struct JobOptions {
   bool report_last_round = true;  // with dropping: evaluate the last round for every fault
   bool direct = false;            // inject every fault directly (forced for circuits with MUX)
   bool keep_good_last = false;    // gather the last round's good values (reference accessors)
+  bool mux_safe = false;          // no fault's cone reaches a MUX select input: stem tracing stays
+                                  // valid in a circuit with MUX (the caller has checked)
   int first_words = 1;            // with dropping: words of the first step
   int word_growth = 2;            // with dropping: factor of words per step after each step (up to 64)
 };

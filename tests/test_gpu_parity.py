@@ -264,3 +264,67 @@ def test_repeated_runs_keep_inputs_resident(c1):
     s.set_option("drop", 0)
     s.run(lsim_amd.Simulator.Mode.PLAIN)
     assert np.array_equal(s.detect_matrix(), det)
+
+
+def _with_muxes(ckt: bytes, seed: int, count: int) -> bytes:
+    """The circuit with `count` 2-input gates turned into MUXes: the gate's
+    pins 1 and 2 become the data inputs and a new wire from a PI drives the
+    select (pin 5). PIs sit on level 0, so the order stays acyclic."""
+    import random
+
+    toks = ckt.split()
+    P, O, G, W = (int(x) for x in toks[:4])
+    rng = random.Random(seed)
+    i, recs, pins = 4, [], {}
+    for _ in range(W):
+        src, py, k = int(toks[i]), int(toks[i + 1]), int(toks[i + 2])
+        outs = [(int(toks[i + 3 + 2 * j]), int(toks[i + 4 + 2 * j])) for j in range(k)]
+        recs.append((src, py, outs))
+        for d, p in outs:
+            pins.setdefault(d, []).append(p)
+        i += 3 + 2 * k
+    types = [int(x) for x in toks[i:i + G]]
+    two = [g for g in range(G) if types[g] in (1, 2, 3, 4, 5, 6) and sorted(pins.get(P + O + g, [])) == [1, 2]]
+    for g in rng.sample(two, count):
+        types[g] = 7
+        recs.append((rng.randrange(P), 0, [(P + O + g, 5)]))
+    lines = [f"{P} {O} {G} {len(recs)}"]
+    lines += [f"{s_} {py} {len(outs)} " + " ".join(f"{d} {p}" for d, p in outs) for s_, py, outs in recs]
+    lines.append(" ".join(map(str, types)))
+    return ("\n".join(lines) + "\n").encode()
+
+
+@pytest.mark.parametrize("drop", [0, 1], ids=["matrix", "dropping"])
+def test_mux_circuit_splits_faults_against_oracle(drop):
+    """A 5,000-gate circuit with 40 MUXes: faults whose cone reaches a MUX
+    select are injected directly, the others keep stem tracing (their
+    patterns propagate independently through MUX data inputs). Every
+    d(f, r) (or first round with dropping) and the report equal the literal
+    restatement, and stem tracing is used."""
+    ckt, flst, ptn = lsim_amd.generate(seed=4242, num_pis=64, num_pos=32, num_gates=5000, num_levels=40,
+                                       num_patterns=64 * 6 + 13, fanout_cap=40)
+    ckt = _with_muxes(ckt, 7, 40)
+    # PI faults too: a PI that drives a select reaches it, so both kinds occur
+    lines = flst.decode().split()
+    pairs = lines[1:] + [x for i in range(64) for x in (str(i), str(i % 2))]
+    flst = (f"{len(pairs) // 2}\n" + "\n".join(f"{pairs[2 * k]} {pairs[2 * k + 1]}" for k in range(len(pairs) // 2))
+            + "\n").encode()
+    ref = oracle.Oracle(ckt, flst, ptn).run_literal(threads=0)
+    det = ref["det"]
+    s = _sim(ckt, flst, ptn, drop=drop)
+    s.run(lsim_amd.Simulator.Mode.SEQUENTIAL)
+    st = s.last_run_stats()
+    assert st["stems"] > 0
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    if drop:
+        assert np.array_equal(s.first_detect_round(), np.where(det.any(1), det.argmax(1), -1))
+    else:
+        assert np.array_equal(s.detect_matrix(), det)
+    pi, gate, po = s.good_results()
+    assert np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
+    # all faults injected directly: the same answer
+    s.set_option("direct", 1)
+    s.run(lsim_amd.Simulator.Mode.SEQUENTIAL)
+    assert s.last_run_stats()["stems"] == 0
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    s.close()
