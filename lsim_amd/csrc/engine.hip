@@ -993,7 +993,10 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           // node is the count of the node's active lanes below it (mbcnt), so
           // no pair has to be located by search.
           const int wave = tid >> 6;
-          const int w_l = s_lword[lane];
+          const int64_t w_l = s_lword[lane];
+          const int site_l = s_lsite[lane];
+          const uint32_t kind_l = uint32_t(s_lkind[lane]);
+          const uint64_t mask_l = s_lmask[lane];
 #pragma unroll 1
           for (int k0 = wave; k0 < n; k0 += kDenseNodes * (CHUNK / 64)) {
             int kk[kDenseNodes], v[kDenseNodes];
@@ -1028,16 +1031,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             for (int i = 0; i < kDenseNodes; ++i)
               r[i] = (on[i] && type[i] != G_PI) ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], vals[i][2], vals[i][3])
                                                 : gu[i];
-            // The lane's injection, read from LDS here rather than held in
-            // registers across the loads above.
-            const int site_l = s_lsite[lane];
 #pragma unroll
             for (int i = 0; i < kDenseNodes; ++i) {
               if (!has[i]) continue;
-              if (on[i] && site_l == v[i]) {
-                const uint32_t kind_l = uint32_t(s_lkind[lane]);
-                r[i] = kind_l == K_FLIP ? (gu[i] ^ s_lmask[lane]) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
-              }
+              if (on[i] && site_l == v[i])
+                r[i] = kind_l == K_FLIP ? (gu[i] ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
               const uint64_t d = on[i] ? (r[i] ^ gu[i]) : 0;
               const uint64_t nm = __ballot(d != 0);
               if (on[i]) {
