@@ -139,6 +139,39 @@ def cpu_baseline(ckt: bytes, flst: bytes, ptn: bytes, preset: dict, budget_s: fl
                          "split": "faults over OpenMP threads"}}
 
 
+def plain_form_sample(ckt: bytes, flst: bytes, ptn: bytes, stride: int) -> dict:
+    """Every `stride`-th fault of the workload through the plain per-level
+    form (LSIM_MODE_PLAIN: one launch per level for the good machine and for
+    every sweep) and through the persistent form, both with dropping, each
+    timed on its second run: evals/s of both on the same sample."""
+    import lsim_amd
+
+    lines = flst.split(b"\n")
+    F = int(lines[0])
+    pick = list(range(0, F, stride))
+    sub = f"{len(pick)}\n".encode() + b"\n".join(lines[1 + i] for i in pick) + b"\n"
+    n = int(ptn.split(b"\n", 1)[0])
+    sim = lsim_amd.Simulator(device=0)
+    sim.set_option("drop", 1)
+    sim.set_option("report_last_round", 0)
+    sim.read_text(ckt, sub, ptn)
+    timed = {}
+    for name, mode in (("plain", lsim_amd.Simulator.Mode.PLAIN), ("persistent", lsim_amd.Simulator.Mode.SEQUENTIAL)):
+        sim.run(mode)  # warm: scratch allocation
+        t0 = time.perf_counter()
+        sim.run(mode)
+        timed[name] = (time.perf_counter() - t0, sim.last_run_stats())
+    sim.close()
+    secs, st = timed["plain"]
+    v = len(pick) * n / secs
+    vp = len(pick) * n / timed["persistent"][0]
+    return {"value": v, "unit": "fault*pattern evals/s", "engine": "k_sweep_level" if st["engine"] == 0 else st["engine"],
+            "sample": f"every {stride}th fault ({len(pick)} faults), all {n} patterns, dropping on",
+            "seconds": round(secs, 3), "sweep_launches": st["sweep_launches"],
+            "persistent_value": vp, "persistent_seconds": round(timed["persistent"][0], 4),
+            "persistent_over_plain": vp / v}
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as fh:
@@ -580,6 +613,16 @@ def main() -> int:
         "breakdown_ms": {k: agg[k] / args.steps for k in ("t_good_ms", "t_sweep_ms", "t_detect_ms", "t_total_ms")},
     }
     sim.close()
+    if not args.no_legs and args.mode == "partitioned":
+        # The plain per-level form on a 1 % fault sample of the same workload
+        # (its cost grows with the faults swept): the ratio the persistent
+        # form reaches over it.
+        t0 = time.perf_counter()
+        try:
+            out["plain_form"] = plain_form_sample(ckt, flst, ptn, stride=400)
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["plain_form"] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] plain-form sample in {time.perf_counter() - t0:.1f}s")
     if not args.no_legs and args.config == "c3":
         # The two systems forms on their own workloads, in the same run.
         for name, fn in (("batching", lambda: batching_leg(args, args.c4_steps, args.c4_warmup)),
