@@ -140,26 +140,40 @@ struct JobDev {
   int64_t lane_unit0;        // first entry of the job's units in the step's {position, kind} list
 };
 
+// Device memory pointers in the global address space: loads through them are
+// global loads, which the LDS wait counter does not track. A pointer read out
+// of the job table is otherwise a generic one, loaded with flat instructions
+// that count against both wait counters, so every wait for an LDS read or a
+// cross-lane shuffle would also wait for them.
+#define LSIM_GLOBAL __attribute__((address_space(1)))
+// int4 / int2 / 2 x uint64 as the compiler's own vector types, which (unlike
+// HIP's int4 / int2 classes) load from any address space.
+typedef int gi4 __attribute__((ext_vector_type(4)));
+typedef int gi2 __attribute__((ext_vector_type(2)));
+typedef unsigned long long gu2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int4 to_int4(gi4 v) { return make_int4(v.x, v.y, v.z, v.w); }
+__device__ __forceinline__ int2 to_int2(gi2 v) { return make_int2(v.x, v.y); }
+
 // The fields a sweep reads, copied out of the job table once per tile so
 // they stay in registers (stores to sweep state cannot alias them).
 struct SweepView {
-  const uint32_t *__restrict__ info;
-  const int4 *__restrict__ fin;
-  const uint64_t *__restrict__ good;
-  const int32_t *__restrict__ live_lvl;
-  const int32_t *__restrict__ fo_off;
-  const int2 *__restrict__ fo;
+  const LSIM_GLOBAL uint32_t *__restrict__ info;
+  const LSIM_GLOBAL gi4 *__restrict__ fin;
+  const LSIM_GLOBAL uint64_t *__restrict__ good;
+  const LSIM_GLOBAL int32_t *__restrict__ live_lvl;
+  const LSIM_GLOBAL int32_t *__restrict__ fo_off;
+  const LSIM_GLOBAL gi2 *__restrict__ fo;
   int64_t RW;
 };
 
 __device__ __forceinline__ SweepView sweep_view(const JobDev *jobs, int j) {
   SweepView v;
-  v.info = jobs[j].info;
-  v.fin = jobs[j].fin;
-  v.good = jobs[j].good;
-  v.live_lvl = jobs[j].live_lvl;
-  v.fo_off = jobs[j].fo_off;
-  v.fo = jobs[j].fo;
+  v.info = (const LSIM_GLOBAL uint32_t *)jobs[j].info;
+  v.fin = (const LSIM_GLOBAL gi4 *)jobs[j].fin;
+  v.good = (const LSIM_GLOBAL uint64_t *)jobs[j].good;
+  v.live_lvl = (const LSIM_GLOBAL int32_t *)jobs[j].live_lvl;
+  v.fo_off = (const LSIM_GLOBAL int32_t *)jobs[j].fo_off;
+  v.fo = (const LSIM_GLOBAL gi2 *)jobs[j].fo;
   v.RW = jobs[j].RW;
   return v;
 }
@@ -233,7 +247,7 @@ __global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const i
   const uint32_t inf = J.info[u];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
   const int4 f = J.fin[u];
-  const uint64_t *good = J.good;
+  const auto good = J.good;
   uint64_t r;
   if (type == G_PI) {
     r = J.pi_words[int64_t(f.x) * RW + w];
@@ -291,7 +305,7 @@ __global__ void __launch_bounds__(512) k_good_window(const JobDev *__restrict__ 
   const int64_t RW = J.RW;
   const int64_t w = int64_t(g) * 64 + lane;
   const bool wok = w < RW;
-  const uint64_t *good = J.good;
+  const auto good = J.good;
   // The block's node records, once, in one coalesced pass.
   const int nb = B.seg[depth];
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s_node[i] = nodes[B.node0 + i];
@@ -418,7 +432,7 @@ __device__ __forceinline__ void sweep_node(const SweepView &J, int u, int base, 
   u = __builtin_amdgcn_readfirstlane(u);
   const uint32_t inf = J.info[u];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-  const int4 f = J.fin[u];
+  const int4 f = to_int4(J.fin[u]);
   uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
   if (type != G_PI) {
     if (f.x >= base) m0 = masks[f.x];
@@ -434,7 +448,7 @@ __device__ __forceinline__ void sweep_node(const SweepView &J, int u, int base, 
     return;
   }
   const int64_t RW = J.RW;
-  const uint64_t *good = J.good;
+  const auto good = J.good;
   // Lanes without a differing fanin and without an injection keep the good
   // value and load nothing.
   const bool act = (((m0 | m1 | m2 | m3) >> lane) & 1) || inj;
@@ -518,7 +532,7 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
   uint64_t m[NB][4], gu[NB], r[NB];
   bool act[NB];
   const int64_t RW = J.RW;
-  const uint64_t *good = J.good;
+  const auto good = J.good;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const NodeStage &n = st[k < cnt ? k : 0];
@@ -576,11 +590,11 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
       }
       const int e0 = __builtin_amdgcn_readfirstlane(st[k].fo0), e1 = __builtin_amdgcn_readfirstlane(st[k].fo1);
       for (int e = e0 + lane; e < e1; e += 64) {
-        const int2 v = J.fo[e];
+        const int2 v = to_int2(J.fo[e]);
         pend[v.x] = epoch;
         atomicAdd(&s_lvlcnt[v.y], 1);
       }
-      if (e1 > e0 && lane == 0) atomicMax(s_maxl, J.fo[e1 - 1].y);
+      if (e1 > e0 && lane == 0) atomicMax(s_maxl, to_int2(J.fo[e1 - 1]).y);
       bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
     }
     if (type[k] == G_PO) acc |= d;
@@ -652,7 +666,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
           if (p) {
             NodeStage &n = s_stage[base + __popcll(b & ((1ull << lane) - 1ull))];
             const uint32_t inf = J.info[u];
-            const int4 f = J.fin[u];
+            const int4 f = to_int4(J.fin[u]);
             const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
             n.u = u;
             n.info = inf;
@@ -849,7 +863,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
   const int lane = tid & 63;
   PairRec *rec = a.rec + int64_t(blockIdx.x) * a.rec_stride;
   uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.rec_stride;
-  uint64_t *pool = a.pool + int64_t(blockIdx.x) * a.pool_words;
+  LSIM_GLOBAL uint64_t *pool = (LSIM_GLOBAL uint64_t *)(a.pool + int64_t(blockIdx.x) * a.pool_words);
   for (int i = tid; i < a.lvl_entries; i += CHUNK) s_lvl[i] = 0;
   if (tid == 0) {
     s_cnt2 = 0;
@@ -865,7 +879,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
     const uint32_t epoch = a.epoch_base + uint32_t(q) + 1u;
     const int64_t RW = J.RW;
-    const uint64_t *good = J.good;
+    const auto good = J.good;
     uint32_t bytes = 0;  // per thread and tile
     uint32_t n_lv = 0, n_pass = 0, n_staged = 0, n_written = 0, n_pairs = 0;
     if (tid < 64) {
@@ -916,7 +930,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         if (in) {
           if (!a.nopend) pf = pend[u];
           inf = J.info[u];
-          f = J.fin[u];
+          f = to_int4(J.fin[u]);
           fo0 = J.fo_off[u];
           fo1 = J.fo_off[u + 1];
         }
@@ -929,22 +943,41 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         int2 fp = make_int2(0, 0);
         if (cand) {
           if (pre && fo1 > fo0) {
-            fp = J.fo[fo0];
+            fp = to_int2(J.fo[fo0]);
             bytes += 8;
           }
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-          if (type != G_PI) {
-            const int fi[4] = {f.x, f.y, f.z, f.w};
+          // The fanins' whole records are loaded together, unconditionally
+          // for the first two (record 0 stands in for a missing fanin), and
+          // masked afterwards by fanin count and epoch, so a node's records
+          // cost one round trip. (Under a per-fanin or per-epoch branch the
+          // compiler loads them one fanin after another.)
+          const int nf = type != G_PI ? int(nin) : 0;
+          const int fi[4] = {f.x, f.y, f.z, f.w};
+          gu2 ma[4];
+          gi2 eb[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-              if (k < int(nin)) {
-                const PairRec r = rec[fi[k]];
-                if (r.epoch == epoch) {
-                  m[k] = r.nm;
-                  fa[k] = r.act;
-                  fb[k] = r.base;
-                }
-              }
+          for (int k = 0; k < 2; ++k) {
+            const PairRec *rp = rec + (k < nf ? fi[k] : 0);
+            ma[k] = *reinterpret_cast<const gu2 *>(rp);
+            eb[k] = *reinterpret_cast<const gi2 *>(&rp->epoch);
+          }
+          ma[2] = ma[3] = gu2{0, 0};
+          eb[2] = eb[3] = gi2{0, 0};
+          if (nf > 2) {
+#pragma unroll
+            for (int k = 2; k < 4; ++k) {
+              const PairRec *rp = rec + (k < nf ? fi[k] : 0);
+              ma[k] = *reinterpret_cast<const gu2 *>(rp);
+              eb[k] = *reinterpret_cast<const gi2 *>(&rp->epoch);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint64_t keep = (k < nf && uint32_t(eb[k].x) == epoch) ? ~uint64_t(0) : uint64_t(0);
+            m[k] = ma[k].x & keep;
+            fa[k] = ma[k].y & keep;
+            fb[k] = uint32_t(eb[k].y) & uint32_t(keep);
           }
           act = m[0] | m[1] | m[2] | m[3] | inj_lanes(s_lsite, u);
           bytes += 20 + (type != G_PI ? 24 * nin : 0);
@@ -997,13 +1030,34 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           const int site_l = s_lsite[lane];
           const uint32_t kind_l = uint32_t(s_lkind[lane]);
           const uint64_t mask_l = s_lmask[lane];
+          // Node k (position v) of the pass evaluated to r in this lane, gu
+          // being its good word: the lane's injection replaces r at the
+          // lane's site; the differing lanes are written, packed.
+          auto dense_out = [&](int k, int v, uint32_t type, uint32_t nin, bool on, uint64_t gu, uint64_t r) {
+            if (on && site_l == v)
+              r = kind_l == K_FLIP ? (gu ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+            const uint64_t d = on ? (r ^ gu) : 0;
+            const uint64_t nm = __ballot(d != 0);
+            if (on) {
+              bytes += 8 * (type == G_PI ? 1 : nin + 1);
+              if (kStats) ++n_pairs;
+            }
+            if (d) {
+              // Dense passes write only the differing lanes, packed: the
+              // node's record then indexes its block by the differing lanes
+              // (its `act` is nm), so readers touch only written words.
+              pool[pb + uint32_t(s_pref[k]) + lanes_below(nm)] = r;
+              if (type == G_PO) atomicOr(&s_acc[lane], (unsigned long long)d);
+              bytes += 8;
+            }
+            if (lane == 0) s_nm[k] = nm;
+          };
 #pragma unroll 1
           for (int k0 = wave; k0 < n; k0 += kDenseNodes * (CHUNK / 64)) {
             int kk[kDenseNodes], v[kDenseNodes];
-            bool has[kDenseNodes], on[kDenseNodes];
+            bool has[kDenseNodes];
             uint32_t type[kDenseNodes], nin[kDenseNodes];
-            uint64_t act[kDenseNodes], gu[kDenseNodes], r[kDenseNodes], vals[kDenseNodes][4];
-            // Every load of the group's nodes first, then the evaluations.
+            bool wide = false;
 #pragma unroll
             for (int i = 0; i < kDenseNodes; ++i) {
               const int ki = k0 + i * (CHUNK / 64);
@@ -1013,58 +1067,78 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               const uint32_t vi = __builtin_amdgcn_readfirstlane(s_info[kk[i]]);
               type[i] = vi & 15u;
               nin[i] = (vi >> 4) & 7u;
-              act[i] = s_act[kk[i]];
-              on[i] = has[i] && ((act[i] >> lane) & 1);
-              gu[i] = on[i] ? good[int64_t(v[i]) * RW + w_l] : 0;
-              const int4 fv = s_fin[kk[i]];
-              const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
-#pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                vals[i][j] = 0;
-                if (on[i] && type[i] != G_PI && j < int(nin[i]))
-                  vals[i][j] = ((s_m[kk[i]][j] >> lane) & 1)
-                                   ? pool[s_fb[kk[i]][j] + lanes_below(s_fa[kk[i]][j])]
-                                   : good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
-              }
+              if (has[i] && nin[i] > 2) wide = true;
             }
+            if (!wide) {
+              // Gates of at most two inputs (the group's choice is uniform):
+              // every load of the group's nodes first, then the evaluations.
+              bool on[kDenseNodes];
+              uint64_t gu[kDenseNodes], vals[kDenseNodes][2];
 #pragma unroll
-            for (int i = 0; i < kDenseNodes; ++i)
-              r[i] = (on[i] && type[i] != G_PI) ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], vals[i][2], vals[i][3])
-                                                : gu[i];
+              for (int i = 0; i < kDenseNodes; ++i) {
+                on[i] = has[i] && ((s_act[kk[i]] >> lane) & 1);
+                gu[i] = 0;
+                if (on[i]) gu[i] = good[int64_t(v[i]) * RW + w_l];
+                const int2 fv = *reinterpret_cast<const int2 *>(&s_fin[kk[i]]);
+                const int fi[2] = {fv.x, fv.y};
 #pragma unroll
-            for (int i = 0; i < kDenseNodes; ++i) {
-              if (!has[i]) continue;
-              if (on[i] && site_l == v[i])
-                r[i] = kind_l == K_FLIP ? (gu[i] ^ mask_l) : (kind_l == K_SA1 ? ~uint64_t(0) : uint64_t(0));
-              const uint64_t d = on[i] ? (r[i] ^ gu[i]) : 0;
-              const uint64_t nm = __ballot(d != 0);
-              if (on[i]) {
-                bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
-                if (kStats) ++n_pairs;
+                for (int j = 0; j < 2; ++j) {
+                  // Both addresses first, then one load from the selected one
+                  // (a load in each arm of a branch makes the compiler wait).
+                  const auto pa = &pool[s_fb[kk[i]][j] + lanes_below(s_fa[kk[i]][j])];
+                  const auto ga = &good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
+                  const auto src = ((s_m[kk[i]][j] >> lane) & 1) ? pa : ga;
+                  vals[i][j] = 0;
+                  if (on[i] && type[i] != G_PI && j < int(nin[i])) vals[i][j] = *src;
+                }
               }
-              if (d) {
-                // Dense passes write only the differing lanes, packed: the
-                // node's record then indexes its block by the differing lanes
-                // (its `act` is nm), so readers touch only written words.
-                pool[pb + uint32_t(s_pref[kk[i]]) + lanes_below(nm)] = r[i];
-                if (type[i] == G_PO) atomicOr(&s_acc[lane], (unsigned long long)d);
-                bytes += 8;
+#pragma unroll
+              for (int i = 0; i < kDenseNodes; ++i) {
+                if (!has[i]) continue;
+                const uint64_t r = (on[i] && type[i] != G_PI)
+                                       ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], 0, 0)
+                                       : gu[i];
+                dense_out(kk[i], v[i], type[i], nin[i], on[i], gu[i], r);
               }
-              if (lane == 0) s_nm[kk[i]] = nm;
+            } else {
+              // The group holds a gate of more than two inputs: its nodes go
+              // one at a time, with four fanin slots.
+              const int k_end = min(n, k0 + kDenseNodes * (CHUNK / 64));
+#pragma unroll 1
+              for (int k1 = k0; k1 < k_end; k1 += CHUNK / 64) {
+                const int v1 = __builtin_amdgcn_readfirstlane(s_u[k1]);
+                const uint32_t vi = __builtin_amdgcn_readfirstlane(s_info[k1]);
+                const uint32_t ty = vi & 15u, ni = (vi >> 4) & 7u;
+                const bool on1 = (s_act[k1] >> lane) & 1;
+                uint64_t gu1 = 0;
+                if (on1) gu1 = good[int64_t(v1) * RW + w_l];
+                const int4 fv = s_fin[k1];
+                const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
+                uint64_t v4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  const auto pa = &pool[s_fb[k1][j] + lanes_below(s_fa[k1][j])];
+                  const auto ga = &good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
+                  const auto src = ((s_m[k1][j] >> lane) & 1) ? pa : ga;
+                  v4[j] = 0;
+                  if (on1 && ty != G_PI && j < int(ni)) v4[j] = *src;
+                }
+                const uint64_t r = (on1 && ty != G_PI) ? eval_gate(ty, ni, v4[0], v4[1], v4[2], v4[3]) : gu1;
+                dense_out(k1, v1, ty, ni, on1, gu1, r);
+              }
             }
           }
         } else {
           // B. (node, active lane) pairs, kPairsPerThread per thread at a time
-          // with their loads interleaved. Every pair writes its pool word
-          // (readers use only the lanes that differ): the pass's words then
-          // fill one contiguous block, written in whole lines.
+          // with their loads interleaved: the pairs' good words and fanin
+          // words are all loaded before any is used. Every pair writes its
+          // pool word (readers use only the lanes that differ): the pass's
+          // words then fill one contiguous block, written in whole lines.
   #pragma unroll 1
           for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
-            int k[kPairsPerThread], ln[kPairsPerThread], v[kPairsPerThread], rk[kPairsPerThread];
-            uint32_t type[kPairsPerThread], nin[kPairsPerThread];
-            int4 fv[kPairsPerThread];
-            int64_t w[kPairsPerThread];
-            uint64_t gu[kPairsPerThread], r[kPairsPerThread];
+            int k[kPairsPerThread], ln[kPairsPerThread], rk[kPairsPerThread];
+            uint32_t vi[kPairsPerThread];
+            uint64_t gu[kPairsPerThread], vals[kPairsPerThread][4];
             bool ok[kPairsPerThread];
   #pragma unroll
             for (int i = 0; i < kPairsPerThread; ++i) {
@@ -1074,45 +1148,43 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               k[i] = find_range(s_pref, n, qe);
               rk[i] = qe - s_pref[k[i]];
               ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
-              v[i] = s_u[k[i]];
-              const uint32_t vi = s_info[k[i]];
-              type[i] = vi & 15u;
-              nin[i] = (vi >> 4) & 7u;
-              fv[i] = s_fin[k[i]];
-              w[i] = s_lword[ln[i]];
-              gu[i] = ok[i] ? good[int64_t(v[i]) * RW + w[i]] : 0;
-            }
+              vi[i] = s_info[k[i]];
+              const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
+              const int4 fv = s_fin[k[i]];
+              const int w = s_lword[ln[i]];
+              const int b = ln[i];
+              const uint64_t below = (uint64_t(1) << b) - 1;
+              const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
+              gu[i] = 0;
+              if (ok[i]) gu[i] = good[int64_t(s_u[k[i]]) * RW + w];
   #pragma unroll
-            for (int i = 0; i < kPairsPerThread; ++i) {
-              r[i] = gu[i];
-              if (ok[i] && type[i] != G_PI) {
-                const int b = ln[i];
-                const uint64_t below = (uint64_t(1) << b) - 1;
-                const int fi[4] = {fv[i].x, fv[i].y, fv[i].z, fv[i].w};
-                uint64_t vals[4] = {0, 0, 0, 0};
-  #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                  if (j < int(nin[i]))
-                    vals[j] = ((s_m[k[i]][j] >> b) & 1)
-                                  ? pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))]
-                                  : good[int64_t(fi[j]) * RW + w[i]];
-                r[i] = eval_gate(type[i], nin[i], vals[0], vals[1], vals[2], vals[3]);
+              for (int j = 0; j < 4; ++j) {
+                // Both addresses first, then one load from the selected one
+                // (a load in each arm of a branch makes the compiler wait).
+                const auto pa = &pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))];
+                const auto ga = &good[int64_t(fi[j]) * RW + w];
+                const auto src = ((s_m[k[i]][j] >> b) & 1) ? pa : ga;
+                vals[i][j] = 0;
+                if (ok[i] && type != G_PI && j < int(nin)) vals[i][j] = *src;
               }
             }
   #pragma unroll
             for (int i = 0; i < kPairsPerThread; ++i) {
               if (!ok[i]) continue;
+              const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
               const int b = ln[i];
-              if (s_lsite[b] == v[i]) {
+              const int v = s_u[k[i]];
+              uint64_t r = type != G_PI ? eval_gate(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]) : gu[i];
+              if (s_lsite[b] == v) {
                 const uint32_t kind = uint32_t(s_lkind[b]);
-                r[i] = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
+                r = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
               }
-              const uint64_t d = r[i] ^ gu[i];
-              bytes += 8 * (type[i] == G_PI ? 1 : nin[i] + 1);
-              pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r[i];
+              const uint64_t d = r ^ gu[i];
+              bytes += 8 * (type == G_PI ? 1 : nin + 1);
+              pool[pb + uint32_t(s_pref[k[i]] + rk[i])] = r;
               if (d) {
                 atomicOr(&s_nm[k[i]], 1ull << b);
-                if (type[i] == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
+                if (type == G_PO) atomicOr(&s_acc[b], (unsigned long long)d);
                 bytes += 8;
               }
               if (kStats) ++n_pairs;
@@ -1182,7 +1254,9 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               if (__shfl(excl, k + step) <= e) k += step;
             const int src0 = __shfl(e0, k), srcx = __shfl(excl, k);
             ok[i] = e < Ew;
-            fo[i] = ok[i] ? J.fo[src0 + (e - srcx)] : make_int2(0, 0);
+            // Loaded whatever ok is (entry 0 exists: Ew > 0), so the loads of
+            // the lane's fanouts are in flight together.
+            fo[i] = to_int2(J.fo[ok[i] ? src0 + (e - srcx) : 0]);
           }
 #pragma unroll
           for (int i = 0; i < kFanoutsPerLane; ++i)
@@ -1274,7 +1348,7 @@ __device__ __forceinline__ uint64_t sens(const JobDev &J, int g, int x, int64_t 
   const uint32_t inf = J.info[g];
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
   const int4 f = J.fin[g];
-  const uint64_t *good = J.good;
+  const auto good = J.good;
   const uint64_t v0 = good[int64_t(f.x) * J.RW + w];
   const uint64_t v1 = nin > 1 ? good[int64_t(f.y) * J.RW + w] : 0;
   const uint64_t v2 = nin > 2 ? good[int64_t(f.z) * J.RW + w] : 0;
