@@ -154,6 +154,21 @@ typedef unsigned long long gu2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int4 to_int4(gi4 v) { return make_int4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ int2 to_int2(gi2 v) { return make_int2(v.x, v.y); }
 
+// First live positions of levels l and l + 1, read with a scalar load: the
+// table is read-only while a kernel runs, and a scalar load is waited for on
+// the LDS/scalar counter, so it does not wait for the vector stores still in
+// flight (a vector load would, its counter being in order with them).
+__device__ __forceinline__ int2 level_range(const LSIM_GLOBAL int32_t *lvl, int l) {
+  const uint64_t p = reinterpret_cast<uint64_t>(lvl);
+  // readfirstlane returns a signed int: each half goes through uint32_t so
+  // that the low half is not sign-extended into the high one.
+  const uint64_t q = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(p))))) |
+                     (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(p >> 32))))) << 32);
+  const __attribute__((address_space(4))) int32_t *c = reinterpret_cast<const __attribute__((address_space(4))) int32_t *>(q);
+  const int i = __builtin_amdgcn_readfirstlane(l);
+  return make_int2(c[i], c[i + 1]);
+}
+
 // The fields a sweep reads, copied out of the job table once per tile so
 // they stay in registers (stores to sweep state cannot alias them).
 struct SweepView {
@@ -912,7 +927,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       l_end = l;
       if (!a.nopend && s_lvl[l] == 0) continue;
       ++n_lv;
-      const int lo = J.live_lvl[l], hi = J.live_lvl[l + 1];
+      const int2 lr = level_range(J.live_lvl, l);
+      const int lo = lr.x, hi = lr.y;
       for (int c0 = lo; c0 < hi; c0 += CHUNK) {
         ++n_pass;
         if (kStats) cy_t = clock64();
@@ -924,16 +940,13 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         const bool pre = a.fo_pre && !a.nopend;
         const int u = c0 + tid;
         const bool in = u < hi;
-        uint32_t pf = 0, inf = 0;
-        int4 f = make_int4(0, 0, 0, 0);
-        int fo0 = 0, fo1 = 0;
-        if (in) {
-          if (!a.nopend) pf = pend[u];
-          inf = J.info[u];
-          f = to_int4(J.fin[u]);
-          fo0 = J.fo_off[u];
-          fo1 = J.fo_off[u + 1];
-        }
+        // Threads past the level's end load node lo's data and ignore it, so
+        // the loads are unconditional.
+        const int uc = in ? u : lo;
+        const uint32_t pf = a.nopend ? 0u : pend[uc];
+        const uint32_t inf = J.info[uc];
+        const int4 f = to_int4(J.fin[uc]);
+        const int fo0 = J.fo_off[uc], fo1 = J.fo_off[uc + 1];
         // Without pending flags every node of the level is a candidate and is
         // staged when some fanin differs in some lane or it injects.
         const bool cand = in && (a.nopend || pf == epoch);
