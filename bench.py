@@ -40,9 +40,10 @@ def log(msg: str) -> None:
 
 def profiled_value(config: str, kernel: str, key: str = "traffic_bytes_per_launch"):
     """A per-launch figure of `kernel` on `config` from its committed rocprofv3
-    summary: `traffic_bytes_per_launch` (HBM bytes, FETCH_SIZE and WRITE_SIZE
-    passes, copy-calibrated) or `transactions_per_launch` (TCC_EA0_RDREQ +
-    TCC_EA0_WRREQ requests); None when absent."""
+    summary: `traffic_bytes_per_launch` (HBM bytes from the read and write
+    request counts by size, TCC_EA0_RDREQ_128B/64B/32B and TCC_EA0_WRREQ_64B)
+    or `transactions_per_launch` (TCC_EA0_RDREQ + TCC_EA0_WRREQ requests);
+    None when absent."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{config}.md")
     if not os.path.exists(path):
         return None
