@@ -19,19 +19,21 @@
 //    so it flips h on U_h alone and (stem, word) pairs with U_h = 0 are not
 //    swept at all. Gates are bitwise per pattern, so this is exact. The
 //    reference's MUX uses the word-level `!s` (fsim/simulation.cpp:1056),
-//    which couples the 64 patterns of a word; circuits with a MUX inject
-//    every fault directly (full word), skipping words where the site's good
-//    value already equals the stuck value.
+//    which couples the 64 patterns of a word through the select only: a
+//    fault whose cone reaches a MUX select is injected directly (full
+//    word), skipping words where the site's good value already equals the
+//    stuck value; every other fault keeps stem tracing.
 //  * Sweeps: a tile holds up to 64 lanes of one job, each lane one (injection
-//    site, word, injected bits). A wave evaluates one node for all lanes of a
-//    tile; per node the tile keeps a 64-bit mask of the lanes whose value
-//    differs from the good machine and a 64-word row of faulty values.
-//      - plain form: one launch per level, every live node of the level
-//        visited by every tile of the batch (one job at a time);
-//      - event form: one persistent launch over the tiles of all active jobs;
-//        a workgroup owns a tile at a time and visits only nodes with a
-//        differing fanin, staged per level in LDS, ending at the deepest
-//        pending level.
+//    site, word, injected bits).
+//      - plain form (k_sweep_level): one launch per level, every live node
+//        of the level visited by every tile of the batch, a wave per node;
+//      - persistent forms: one launch over the tiles of all active jobs; a
+//        workgroup owns a tile at a time, walks the levels from its lowest
+//        injection to the deepest pending node and visits only nodes with a
+//        differing fanin. k_sweep_events evaluates a node per wave;
+//        k_sweep_pairs (the default) stages a level's nodes in LDS and
+//        evaluates their (node, active lane) pairs, with each differing
+//        node's faulty words packed into a per-tile pool.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
