@@ -236,7 +236,7 @@ def _tuned(st: dict) -> dict:
 
 
 def _batch_options(batch, args) -> None:
-    for key in ("pair_dense", "pair_nopend", "pair_fo_pre", "pair_nopend_pct", "max_step_words"):
+    for key in ("pair_dense", "pair_nopend", "pair_fo_pre", "pair_nopend_pct", "max_step_words", "device_steps"):
         v = getattr(args, key)
         if v >= 0:
             batch.set_option(key, v)
@@ -351,6 +351,8 @@ def serving_leg(args, n_jobs: int) -> dict:
     t0 = time.perf_counter()
     srv = lsim_amd.Server(device=0, job_budget_mb=64 * 1024)
     srv.set_option("word_growth", args.c5_growth)
+    if args.device_steps >= 0:
+        srv.set_option("device_steps", args.device_steps)
     cids = []
     for i, sp in enumerate(circuits):
         cids.append(srv.add_circuit(workloads.circuit_text(sp)))
@@ -481,6 +483,8 @@ def main() -> int:
                          "engine default)")
     ap.add_argument("--max-step-words", type=int, default=-1,
                     help="with fault dropping: most words of one step (-1: the engine default)")
+    ap.add_argument("--device-steps", type=int, default=-1, choices=[-1, 0, 1],
+                    help="plan word steps on the device (1) or the host (0); -1: the library default")
     ap.add_argument("--first-words", type=int, default=0,
                     help="with fault dropping: words of the first step (0: the engine default)")
     ap.add_argument("--word-growth", type=int, default=0,
@@ -552,6 +556,8 @@ def main() -> int:
         sim.set_option("pair_nopend_pct", args.pair_nopend_pct)
     if args.max_step_words >= 0:
         sim.set_option("max_step_words", args.max_step_words)
+    if args.device_steps >= 0:
+        sim.set_option("device_steps", args.device_steps)
     if args.first_words > 0:
         sim.set_option("first_words", args.first_words)
     if args.word_growth > 0:

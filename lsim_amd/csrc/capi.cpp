@@ -241,6 +241,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
     eo.pair_dense = int(value);
   } else if (k == "autotune") {
     eo.autotune = value != 0;
+  } else if (k == "device_steps") {
+    if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "device_steps must be 0 or 1");
+    eo.device_steps = value != 0;
   } else if (k == "good_form") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "good_form must be 0 or 1");
     eo.good_form = int(value);

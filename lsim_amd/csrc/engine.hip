@@ -139,7 +139,8 @@ struct JobDev {
   int direct;
   int64_t item0;             // first detection item of the job
   int64_t word0;             // first need/dout word of the job's units
-  int64_t lane_unit0;        // first entry of the job's units in the step's {position, kind} list
+  int64_t lane_unit0;        // first entry of the job's units in the step's unit list
+  const int32_t *units;      // the step's units (positions), in order
 };
 
 // Device memory pointers in the global address space: loads through them are
@@ -834,6 +835,15 @@ struct alignas(32) PairRec {  // 32 B: a record never straddles a memory sector
   unsigned long long pad;
 };
 
+// Sweep state of a step planned on the device (written by k_ds_tiles).
+struct SweepDyn {
+  int ntiles;           // tiles of the step
+  int nopend;           // sweep without pending flags
+  int form;             // workgroup size of the pair sweep that runs the step (256 or 512)
+  uint32_t epoch_base;  // the step's tiles take epochs epoch_base + 1 ...
+  double share;         // staged share of the nodes walked by the last sweep that walked any
+};
+
 struct PairArgs {
   const JobDev *__restrict__ jobs;
   const Tile *__restrict__ tiles;
@@ -856,6 +866,13 @@ struct PairArgs {
   int nopend;                  // no pending flags: every node of a walked level is a candidate
   int fo_pre;                  // with pending flags: a staged node's first live fanout is loaded in the scan
   unsigned long long *tstats;  // as SweepArgs::tstats, or null
+  // A step planned on the device: the tile count, pending-flag mode,
+  // workgroup size and epochs come from *dyn (the launch of the other
+  // workgroup size returns at once). With ntiles_p set, the launch sweeps the
+  // tiles the first launch gave up: their count is *ntiles_p and their epochs
+  // follow the first launch's.
+  const SweepDyn *dyn;
+  const int *ntiles_p;
 };
 
 template <bool kStats, int CHUNK>
@@ -878,6 +895,20 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
   extern __shared__ uint8_t s_lvl[];  // per level: some node of the level is pending
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  int ntiles = a.ntiles, nopend = a.nopend;
+  uint32_t epoch_base = a.epoch_base;
+  if (a.dyn) {
+    const SweepDyn D = *a.dyn;
+    if (a.ntiles_p) {
+      ntiles = *a.ntiles_p;
+      epoch_base = D.epoch_base + uint32_t(D.ntiles);
+    } else {
+      if (D.form != CHUNK) return;
+      ntiles = D.ntiles;
+      epoch_base = D.epoch_base;
+    }
+    nopend = D.nopend;
+  }
   PairRec *rec = a.rec + int64_t(blockIdx.x) * a.rec_stride;
   uint32_t *pend = a.pend + int64_t(blockIdx.x) * a.rec_stride;
   LSIM_GLOBAL uint64_t *pool = (LSIM_GLOBAL uint64_t *)(a.pool + int64_t(blockIdx.x) * a.pool_words);
@@ -890,11 +921,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     if (tid == 0) s_tile = atomicAdd(a.queue, 1);
     __syncthreads();
     const int q = __builtin_amdgcn_readfirstlane(s_tile);
-    if (q >= a.ntiles) return;
+    if (q >= ntiles) return;
     const int t = a.tile_list ? a.tile_list[q] : q;
     const Tile T = a.tiles[t];
     const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
-    const uint32_t epoch = a.epoch_base + uint32_t(q) + 1u;
+    const uint32_t epoch = epoch_base + uint32_t(q) + 1u;
     const int64_t RW = J.RW;
     const auto good = J.good;
     uint32_t bytes = 0;  // per thread and tile
@@ -909,7 +940,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       s_lmask[lane] = L.mask;
       s_acc[lane] = 0;
       if (valid) {
-        if (!a.nopend) pend[site] = epoch;
+        if (!nopend) pend[site] = epoch;
         s_lvl[L.level] = 1;
       }
     }
@@ -927,7 +958,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     bool dense_pass = false;
     for (int l = T.start_level; l <= s_maxl && !aborted; ++l) {
       l_end = l;
-      if (!a.nopend && s_lvl[l] == 0) continue;
+      if (!nopend && s_lvl[l] == 0) continue;
       ++n_lv;
       const int2 lr = level_range(J.live_lvl, l);
       const int lo = lr.x, hi = lr.y;
@@ -939,19 +970,19 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         // costs two dependent loads: these, then its fanins' records. With
         // fo_pre its first live fanout is loaded beside the records, so that a
         // written node with one live fanout marks it without a further load.
-        const bool pre = a.fo_pre && !a.nopend;
+        const bool pre = a.fo_pre && !nopend;
         const int u = c0 + tid;
         const bool in = u < hi;
         // Threads past the level's end load node lo's data and ignore it, so
         // the loads are unconditional.
         const int uc = in ? u : lo;
-        const uint32_t pf = a.nopend ? 0u : pend[uc];
+        const uint32_t pf = nopend ? 0u : pend[uc];
         const uint32_t inf = J.info[uc];
         const int4 f = to_int4(J.fin[uc]);
         const int fo0 = J.fo_off[uc], fo1 = J.fo_off[uc + 1];
         // Without pending flags every node of the level is a candidate and is
         // staged when some fanin differs in some lane or it injects.
-        const bool cand = in && (a.nopend || pf == epoch);
+        const bool cand = in && (nopend || pf == epoch);
         uint64_t act = 0;
         uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
         uint32_t fb[4] = {0, 0, 0, 0};
@@ -997,8 +1028,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
           act = m[0] | m[1] | m[2] | m[3] | inj_lanes(s_lsite, u);
           bytes += 20 + (type != G_PI ? 24 * nin : 0);
         }
-        if (!a.nopend) bytes += 4;
-        const bool p = cand && (!a.nopend || act != 0);
+        if (!nopend) bytes += 4;
+        const bool p = cand && (!nopend || act != 0);
         const uint64_t slot = wave_claim(p ? ((uint64_t(1) << 32) | uint64_t(__popcll(act))) : 0, lane, &s_cnt2);
         if (p) {
           const int k = int(slot >> 32);
@@ -1245,13 +1276,13 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         }
-        int incl = a.nopend ? 0 : deg;
+        int incl = nopend ? 0 : deg;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
           const int y = __shfl_up(incl, o);
           if (lane >= o) incl += y;
         }
-        const int excl = incl - (a.nopend ? 0 : deg);
+        const int excl = incl - (nopend ? 0 : deg);
         const int Ew = __shfl(incl, 63);  // 0 without pending flags: no fanout is marked
         // D. kFanoutsPerLane fanouts per lane at a time, loads interleaved.
         // The fanout e belongs to the last lane whose prefix is <= e.
@@ -1395,9 +1426,7 @@ struct ItemArgs {
 // ends at a PO is decided here, otherwise its bits join its stem's need.
 // Direct injection: a word needs a lane iff the stuck value differs from the
 // site's good value somewhere in the full word.
-__global__ void k_local(ItemArgs a) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= a.item_off[a.njobs]) return;
+__device__ __forceinline__ void local_item(const ItemArgs &a, int64_t t) {
   const int jb = find_job(a.item_off, a.njobs, t);
   const JobDev &J = a.jobs[jb];
   const int64_t lt = t - a.item_off[jb];
@@ -1419,10 +1448,13 @@ __global__ void k_local(ItemArgs a) {
   }
 }
 
-// Phase 2: stem tracing d = L & D_stem; direct injection d = PO difference.
-__global__ void k_detect(ItemArgs a) {
+__global__ void k_local(ItemArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= a.item_off[a.njobs]) return;
+  if (t < a.item_off[a.njobs]) local_item(a, t);
+}
+
+// Phase 2: stem tracing d = L & D_stem; direct injection d = PO difference.
+__device__ __forceinline__ void detect_item(const ItemArgs &a, int64_t t) {
   const int jb = find_job(a.item_off, a.njobs, t);
   const JobDev &J = a.jobs[jb];
   const int64_t lt = t - a.item_off[jb];
@@ -1439,6 +1471,22 @@ __global__ void k_detect(ItemArgs a) {
   const uint64_t D = a.dout[J.word0 + int64_t(J.head_slot[h]) * J.Wn + j];
   if (!D) return;
   record(J, f, w, local_term(J, f, w) & D);
+}
+
+__global__ void k_detect(ItemArgs a) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < a.item_off[a.njobs]) detect_item(a, t);
+}
+
+// The same over a fixed grid: the item count is read from the device.
+__global__ void k_local_gs(ItemArgs a) {
+  const int64_t n = a.item_off[a.njobs], stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) local_item(a, t);
+}
+
+__global__ void k_detect_gs(ItemArgs a) {
+  const int64_t n = a.item_off[a.njobs], stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) detect_item(a, t);
 }
 
 // First detecting rounds of the step's faults, packed per dropping job at
@@ -1548,21 +1596,18 @@ __global__ void k_lane_bounds(const JobDev *__restrict__ jobs, int nj, const uin
   lane_off[t] = o < words ? int64_t(pos[o]) : int64_t(pos[words - 1]) + (need[words - 1] != 0 ? 1 : 0);
 }
 
-// unit_pos[lane_unit0 + k] = position of unit k of a job. Its injection is a
-// flip for stem tracing, else the stuck value of step fault k (a direct job's
-// step faults are in unit order).
-__global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
-                            const int32_t *__restrict__ pos, const int32_t *__restrict__ unit_pos, int64_t words,
-                            Lane *__restrict__ lanes) {
-  const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (o >= words) return;
+// units[k] = position of unit k of a job. Its injection is a flip for stem
+// tracing, else the stuck value of step fault k (a direct job's step faults
+// are in unit order).
+__device__ __forceinline__ void lane_fill_word(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
+                                               const int32_t *__restrict__ pos, int64_t o, Lane *__restrict__ lanes) {
   const uint64_t m = need[o];
   if (!m) return;
   const JobDev &J = jobs[job_of_word(jobs, nj, o)];
   const int64_t rel = o - J.word0;
   const int64_t k = rel / J.Wn;
   const int jw = int(rel - k * J.Wn);
-  const int32_t site = unit_pos[J.lane_unit0 + k];
+  const int32_t site = J.units[k];
   Lane ln;
   ln.mask = m;
   ln.site = site;
@@ -1572,6 +1617,12 @@ __global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint6
   ln.level = level_of(J.live_lvl, J.max_level, site);
   ln.pad = 0;
   lanes[pos[o]] = ln;
+}
+
+__global__ void k_lane_fill(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
+                            const int32_t *__restrict__ pos, int64_t words, Lane *__restrict__ lanes) {
+  const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (o < words) lane_fill_word(jobs, nj, need, pos, o, lanes);
 }
 
 // Tile t of the step: job by tile_off (largest i with tile_off[i] <= t),
@@ -1609,6 +1660,444 @@ __global__ void k_gather_tiles(const Tile *__restrict__ in, const int *__restric
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- word steps planned on the device
+//
+// A step's fault lists, units, detection items, lanes and tiles are counted
+// and laid out by kernels: prefix sums over fixed grids that read their
+// lengths from device memory, one-workgroup kernels for the per-job offsets,
+// and a counting sort of the tiles by depth. The host enqueues a step
+// without waiting for the one before it, and every launch of a step has a
+// fixed shape.
+
+// One job of a device-planned step.
+struct StepJob {
+  const int32_t *src;    // candidate faults: the job's previous step list, or its live list
+  const int32_t *src_n;  // their count on the device, or null (then src_count)
+  int32_t src_count;
+  int32_t filter;        // keep only candidates not detected yet (first_round == INT32_MAX)
+  int32_t *dst;          // the step's fault list (unit order)
+  int32_t *dst_n;        // its count (written by k_ds_plan)
+  int32_t *units;        // the step's units
+  int64_t fbase;         // first entry of the job in the step's fault space
+  int32_t wide;          // the circuit's levels average more than 256 live nodes
+  int32_t pad;
+};
+
+// Lengths of a device-planned step.
+struct StepLen {
+  int64_t items;  // detection items
+  int64_t words;  // need / dout words
+  int64_t lanes;
+  int64_t tiles;
+};
+
+constexpr int kScanBlocks = 256;   // workgroups of a prefix sum's first and last phase
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;      // consecutive entries per thread in the last phase
+constexpr int kGsBlocks = 1024;    // workgroups of the fixed-grid kernels (256 threads)
+constexpr int kPlanThreads = 1024; // the one-workgroup planning kernels
+constexpr int kDsStat = 8;         // step figures before the per-job ones
+
+// Exclusive prefix sum of v over the BS threads of a workgroup; total = the
+// sum. s_w holds BS / 64 entries of LDS. Every thread calls it.
+template <int BS>
+__device__ __forceinline__ int64_t block_excl_sum(int64_t v, int64_t &total, int64_t *s_w) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  int64_t before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i) {
+    const int64_t t = s_w[i];
+    before += i < w ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return before + int64_t(x) - v;
+}
+
+// Flag sources of the prefix sums.
+struct FlagI32 {
+  const int32_t *p;
+  __device__ int operator()(int64_t i) const { return p[i]; }
+};
+struct FlagWord {
+  const uint64_t *p;
+  __device__ int operator()(int64_t i) const { return p[i] != 0 ? 1 : 0; }
+};
+
+__device__ __forceinline__ int64_t scan_len(const int64_t *n_p, int64_t n) { return n_p ? *n_p : n; }
+
+// Entries of each workgroup's range: a multiple of one last-phase tile.
+__device__ __forceinline__ int64_t scan_chunk(int64_t n) {
+  constexpr int64_t tile = int64_t(kScanThreads) * kScanItems;
+  const int64_t per = (n + kScanBlocks - 1) / kScanBlocks;
+  return (per + tile - 1) / tile * tile;
+}
+
+// Prefix sum, phase 1: the flag count of each workgroup's range.
+template <class Flag>
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(Flag flag, const int64_t *n_p, int64_t n_h, int64_t *bsum) {
+  __shared__ int64_t s_w[kScanThreads / 64];
+  const int64_t n = scan_len(n_p, n_h), chunk = scan_chunk(n);
+  const int64_t lo = min(n, int64_t(blockIdx.x) * chunk), hi = min(n, lo + chunk);
+  int64_t v = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) v += flag(i);
+  int64_t tot;
+  (void)block_excl_sum<kScanThreads>(v, tot, s_w);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Phase 2 (one workgroup of kScanBlocks threads): range counts to range
+// offsets, in place; out[n] = the total.
+__global__ void __launch_bounds__(kScanBlocks) k_scan_offsets(int64_t *bsum, const int64_t *n_p, int64_t n_h,
+                                                              int32_t *out) {
+  __shared__ int64_t s_w[kScanBlocks / 64];
+  const int64_t v = bsum[threadIdx.x];
+  int64_t tot;
+  const int64_t ex = block_excl_sum<kScanBlocks>(v, tot, s_w);
+  bsum[threadIdx.x] = ex;
+  if (threadIdx.x == 0) out[scan_len(n_p, n_h)] = int32_t(tot);
+}
+
+// Phase 3: out[i] = the flags before entry i.
+template <class Flag>
+__global__ void __launch_bounds__(kScanThreads) k_scan_write(Flag flag, const int64_t *n_p, int64_t n_h,
+                                                             const int64_t *boff, int32_t *out) {
+  __shared__ int64_t s_w[kScanThreads / 64];
+  const int64_t n = scan_len(n_p, n_h), chunk = scan_chunk(n);
+  const int64_t lo = min(n, int64_t(blockIdx.x) * chunk), hi = min(n, lo + chunk);
+  int64_t run = boff[blockIdx.x];
+  for (int64_t b = lo; b < hi; b += int64_t(kScanThreads) * kScanItems) {
+    const int64_t i0 = b + int64_t(threadIdx.x) * kScanItems;
+    int f[kScanItems];
+    int64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      f[k] = i0 + k < hi ? flag(i0 + k) : 0;
+      v += f[k];
+    }
+    int64_t tot;
+    int64_t ex = run + block_excl_sum<kScanThreads>(v, tot, s_w);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (i0 + k < hi) out[i0 + k] = int32_t(ex);
+      ex += f[k];
+    }
+    run += tot;
+  }
+}
+
+// Job holding entry t of the step's fault space: the largest i with fbase <= t.
+__device__ __forceinline__ int job_of_entry(const StepJob *__restrict__ sj, int nj, int64_t t) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sj[mid].fbase <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// End of job j's range in a fault space of n entries.
+__device__ __forceinline__ int64_t entry_end(const StepJob *__restrict__ sj, int nj, int j, int64_t n) {
+  return j + 1 < nj ? sj[j + 1].fbase : n;
+}
+
+// flag[t] = 1 when entry t of the step's fault space is a fault of the
+// step: candidate k = t - fbase of its job exists and, when the job filters,
+// is not detected yet.
+__global__ void k_ds_keep(const StepJob *__restrict__ sj, const JobDev *__restrict__ jobs, int nj, int64_t n,
+                          int32_t *__restrict__ flag) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int j = job_of_entry(sj, nj, t);
+    const StepJob &S = sj[j];
+    const int64_t k = t - S.fbase;
+    const int64_t cnt = S.src_n ? int64_t(*S.src_n) : int64_t(S.src_count);
+    int keep = 0;
+    if (k < cnt) keep = (!S.filter || jobs[j].first_round[S.src[k]] == INT32_MAX) ? 1 : 0;
+    flag[t] = keep;
+  }
+}
+
+// The kept candidates, in their order, into each job's step list.
+__global__ void k_ds_compact(const StepJob *__restrict__ sj, int nj, int64_t n, const int32_t *__restrict__ flag,
+                             const int32_t *__restrict__ pos) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    if (!flag[t]) continue;
+    const StepJob &S = sj[job_of_entry(sj, nj, t)];
+    S.dst[pos[t] - pos[S.fbase]] = S.src[t - S.fbase];
+  }
+}
+
+// flag[t] = 1 when step fault k = t - fbase opens a unit: every fault under
+// direct injection; under stem tracing the first fault of a region whose
+// head is not a PO (a job's faults are ordered by region head).
+__global__ void k_ds_unit_flag(const StepJob *__restrict__ sj, const JobDev *__restrict__ jobs, int nj, int64_t n,
+                               const int32_t *__restrict__ pos, int32_t *__restrict__ flag) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int j = job_of_entry(sj, nj, t);
+    const StepJob &S = sj[j];
+    const JobDev &J = jobs[j];
+    const int64_t k = t - S.fbase;
+    const int64_t nf = int64_t(pos[entry_end(sj, nj, j, n)]) - int64_t(pos[S.fbase]);
+    int f = 0;
+    if (k < nf) {
+      if (J.direct) {
+        f = 1;
+      } else {
+        const int32_t h = J.head[S.dst[k]];
+        f = (!J.is_po[h] && (k == 0 || J.head[S.dst[k - 1]] != h)) ? 1 : 0;
+      }
+    }
+    flag[t] = f;
+  }
+}
+
+// Unit positions: the region head, or the fault site under direct injection.
+__global__ void k_ds_units(const StepJob *__restrict__ sj, const JobDev *__restrict__ jobs, int nj, int64_t n,
+                           const int32_t *__restrict__ flag, const int32_t *__restrict__ upos) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    if (!flag[t]) continue;
+    const int j = job_of_entry(sj, nj, t);
+    const StepJob &S = sj[j];
+    const JobDev &J = jobs[j];
+    const int32_t x = S.dst[t - S.fbase];
+    S.units[upos[t] - upos[S.fbase]] = J.direct ? J.site[x] : J.head[x];
+  }
+}
+
+// Per job: the step's faults and units, its first detection item and need
+// word (written into the job table), and the step's lengths; stat gets
+// {faults, units} per job. One workgroup.
+__global__ void __launch_bounds__(kPlanThreads) k_ds_plan(const StepJob *__restrict__ sj, JobDev *__restrict__ jobs,
+                                                          int nj, int64_t n, const int32_t *__restrict__ pos,
+                                                          const int32_t *__restrict__ upos, int64_t *__restrict__ item_off,
+                                                          StepLen *__restrict__ len, int64_t *__restrict__ stat) {
+  __shared__ int64_t s_w[kPlanThreads / 64];
+  int64_t ci = 0, cw = 0;
+  for (int j0 = 0; j0 < nj; j0 += kPlanThreads) {
+    const int j = j0 + int(threadIdx.x);
+    int64_t nf = 0, nu = 0, items = 0, words = 0;
+    if (j < nj) {
+      const int64_t b = sj[j].fbase, e = entry_end(sj, nj, j, n);
+      nf = int64_t(pos[e]) - int64_t(pos[b]);
+      nu = int64_t(upos[e]) - int64_t(upos[b]);
+      items = nf * jobs[j].Wn;
+      words = nu * jobs[j].Wn;
+    }
+    int64_t ti, tw;
+    const int64_t ei = block_excl_sum<kPlanThreads>(items, ti, s_w);
+    const int64_t ew = block_excl_sum<kPlanThreads>(words, tw, s_w);
+    if (j < nj) {
+      JobDev &J = jobs[j];
+      J.nf = nf;
+      J.item0 = ci + ei;
+      J.word0 = cw + ew;
+      item_off[j] = ci + ei;
+      *sj[j].dst_n = int32_t(nf);
+      stat[2 * j] = nf;
+      stat[2 * j + 1] = nu;
+    }
+    ci += ti;
+    cw += tw;
+  }
+  if (threadIdx.x == 0) {
+    item_off[nj] = ci;
+    len->items = ci;
+    len->words = cw;
+  }
+}
+
+__global__ void k_ds_zero(const StepLen *__restrict__ len, uint64_t *__restrict__ need, uint64_t *__restrict__ dout) {
+  const int64_t n = len->words, stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    need[t] = 0;
+    dout[t] = 0;
+  }
+}
+
+// Stem tracing: head_slot of each unit of the step = its index in the job's
+// units (clear: back to -1).
+__global__ void k_ds_slots(const StepJob *__restrict__ sj, const JobDev *__restrict__ jobs, int nj, int64_t n,
+                           const int32_t *__restrict__ upos, int clear) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int j = job_of_entry(sj, nj, t);
+    const JobDev &J = jobs[j];
+    if (J.direct) continue;
+    const StepJob &S = sj[j];
+    const int64_t k = t - S.fbase;
+    const int64_t nu = int64_t(upos[entry_end(sj, nj, j, n)]) - int64_t(upos[S.fbase]);
+    if (k < nu) J.head_slot[S.units[k]] = clear ? -1 : int32_t(k);
+  }
+}
+
+__global__ void k_lane_fill_gs(const JobDev *__restrict__ jobs, int nj, const uint64_t *__restrict__ need,
+                               const int32_t *__restrict__ pos, const StepLen *__restrict__ len,
+                               Lane *__restrict__ lanes) {
+  const int64_t n = len->words, stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < n; o += stride)
+    lane_fill_word(jobs, nj, need, pos, o, lanes);
+}
+
+struct TilePlanArgs {
+  const StepJob *sj;
+  const JobDev *jobs;
+  int nj;
+  const int32_t *pos;           // lane position of each need word (words + 1 entries)
+  StepLen *len;
+  int64_t *lane_off, *tile_off; // nj + 1 entries each
+  int32_t *hist;                // depth histogram of nkeys entries, cleared here
+  int nkeys;
+  SweepDyn *dyn;
+  unsigned long long *bytes;    // sweep counters: bytes, staged nodes, walked nodes
+  int *counters;                // tile queue of the first launch and of the rerun, tiles each gave up
+  uint32_t epoch_base;
+  int dense;                    // a dropping job's first step or a full-matrix step
+  int threads;                  // workgroup size asked for (0: chosen here)
+  int allow256;                 // the 256-thread form can be resident
+  int nopend_mode, nopend_pct;  // as EngineOptions::pair_nopend, pair_nopend_pct
+};
+
+// Per job: lanes and tiles of 64 lanes; the step's sweep configuration,
+// chosen as the host chooses it for a step it plans itself: 512-thread
+// workgroups when most tiles walk levels wider than 256 nodes, in dense
+// steps and outside a middle staged share (0.2-0.75) of the last sweep;
+// no pending flags in dense steps and after a staged share above
+// nopend_pct percent. One workgroup.
+__global__ void __launch_bounds__(kPlanThreads) k_ds_tiles(TilePlanArgs a) {
+  __shared__ int64_t s_w[kPlanThreads / 64];
+  const int64_t words = a.len->words;
+  int64_t ct = 0, cwide = 0;
+  for (int j0 = 0; j0 < a.nj; j0 += kPlanThreads) {
+    const int j = j0 + int(threadIdx.x);
+    int64_t lanes = 0, tiles = 0, lo = 0;
+    if (j < a.nj) {
+      const int64_t w0 = a.jobs[j].word0, w1 = j + 1 < a.nj ? a.jobs[j + 1].word0 : words;
+      lo = a.pos[w0];
+      lanes = int64_t(a.pos[w1]) - lo;
+      tiles = (lanes + 63) / 64;
+    }
+    int64_t tt, tw;
+    const int64_t et = block_excl_sum<kPlanThreads>(tiles, tt, s_w);
+    (void)block_excl_sum<kPlanThreads>((j < a.nj && a.sj[j].wide) ? tiles : 0, tw, s_w);
+    if (j < a.nj) {
+      a.lane_off[j] = lo;
+      a.tile_off[j] = ct + et;
+    }
+    ct += tt;
+    cwide += tw;
+  }
+  for (int i = int(threadIdx.x); i < a.nkeys; i += kPlanThreads) a.hist[i] = 0;
+  if (threadIdx.x == 0) {
+    const int64_t lanes = a.pos[words];
+    a.lane_off[a.nj] = lanes;
+    a.tile_off[a.nj] = ct;
+    a.len->lanes = lanes;
+    a.len->tiles = ct;
+    SweepDyn D = *a.dyn;
+    if (a.bytes[2]) D.share = double(a.bytes[1]) / double(a.bytes[2]);
+    const bool mid = D.share >= 0.2 && D.share <= 0.75;
+    const bool big = 2 * cwide > ct && (a.dense || !mid);
+    int threads = a.threads ? a.threads : (big ? 512 : 256);
+    if (threads == 256 && !a.allow256) threads = 512;
+    D.form = threads;
+    D.nopend = (a.nopend_mode == 2 || (a.nopend_mode == 1 && (a.dense || 100.0 * D.share > double(a.nopend_pct)))) ? 1 : 0;
+    D.ntiles = int(ct);
+    D.epoch_base = a.epoch_base;
+    *a.dyn = D;
+    a.bytes[0] = 0;
+    a.bytes[1] = 0;
+    a.bytes[2] = 0;
+    for (int i = 0; i < 4; ++i) a.counters[i] = 0;
+  }
+}
+
+struct TileSortArgs {
+  const JobDev *jobs;
+  int nj;
+  const Lane *lanes;
+  const int64_t *lane_off, *tile_off;
+  const StepLen *len;
+  Tile *raw;
+  int32_t *keys;
+  int32_t *hist;
+  Tile *out;
+};
+
+// Tile t of the step (its job by tile_off), with its depth key (the levels it
+// may walk) counted in the histogram.
+__global__ void k_ds_tile_hist(TileSortArgs a) {
+  const int64_t n = a.len->tiles, stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) {
+    int lo = 0, hi = a.nj - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.tile_off[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int64_t l0 = a.lane_off[lo] + 64 * (t - a.tile_off[lo]);
+    const int nl = int(min(int64_t(64), a.lane_off[lo + 1] - l0));
+    Tile T;
+    T.lane0 = int32_t(l0);
+    T.nlanes = nl;
+    T.start_level = a.lanes[l0].level;
+    T.end_level = a.lanes[l0 + nl - 1].level;
+    T.job = lo;
+    const int key = a.jobs[lo].max_level - T.start_level;
+    a.raw[t] = T;
+    a.keys[t] = key;
+    atomicAdd(&a.hist[key], 1);
+  }
+}
+
+// Depth buckets, deepest first: hist[key] becomes the first queue position of
+// the key's tiles. One workgroup.
+__global__ void __launch_bounds__(kPlanThreads) k_ds_hist_scan(int32_t *hist, int nkeys) {
+  __shared__ int64_t s_w[kPlanThreads / 64];
+  int64_t carry = 0;
+  for (int i0 = 0; i0 < nkeys; i0 += kPlanThreads) {
+    const int i = i0 + int(threadIdx.x);
+    const int key = nkeys - 1 - i;
+    const int64_t v = i < nkeys ? hist[key] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_sum<kPlanThreads>(v, tot, s_w);
+    if (i < nkeys) hist[key] = int32_t(carry + ex);
+    carry += tot;
+  }
+}
+
+// Tiles into the queue, deepest first (in any order within a depth).
+__global__ void k_ds_tile_scatter(TileSortArgs a) {
+  const int64_t n = a.len->tiles, stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride)
+    a.out[atomicAdd(&a.hist[a.keys[t]], 1)] = a.raw[t];
+}
+
+// The step's figures for the host: sweep bytes, lanes, tiles, tiles the
+// first sweep launch gave up and tiles the rerun gave up, items, words.
+__global__ void k_ds_stats(const StepLen *__restrict__ len, const unsigned long long *__restrict__ bytes,
+                           const int *__restrict__ counters, int64_t *__restrict__ stat) {
+  stat[0] = int64_t(bytes[0]);
+  stat[1] = len->lanes;
+  stat[2] = len->tiles;
+  stat[3] = counters[2];
+  stat[4] = counters[3];
+  stat[5] = len->items;
+  stat[6] = len->words;
+  stat[7] = 0;
 }
 
 // Stream on which device buffers allocated by the calling thread are
@@ -2018,6 +2507,14 @@ struct JobState {
   int64_t step_w0 = 0;
   int step_Wn = 0;
   RunResults res;
+  // Steps planned on the device: the live list, two step lists (a step's list
+  // is filtered from the one before), the step's units, and the counts of the
+  // two step lists.
+  bool dev = false;         // this run's steps are planned on the device
+  int dev_steps = 0;        // device steps taken (the parity picks the step list)
+  int64_t dev_alive = -1;   // faults of the last step whose figures came back (-1: none yet)
+  int64_t stems_cap = 0;    // regions with a non-PO head among the live faults
+  DevBuf<int32_t> d_live, d_fl0, d_fl1, d_units, d_cnt;
 };
 
 // Owns the engine's stream; declared first in Engine::Impl so that it is
@@ -2148,9 +2645,13 @@ struct Engine::Impl {
   bool borrowed = false;
 
   ~Impl() {
-    if (borrowed) (void)hipStreamSynchronize(stream);
+    // In-flight steps may still copy into the slots' page-locked buffers.
+    if (stream) (void)hipStreamSynchronize(stream);
     jobs.clear();
     circuits.clear();
+    for (DsSlot &sl : ds)
+      for (hipEvent_t &e : sl.ev)
+        if (e) (void)hipEventDestroy(e);
     if (!borrowed) {
       if (ev[0]) (void)hipEventDestroy(ev[0]);
       if (ev[1]) (void)hipEventDestroy(ev[1]);
@@ -2291,7 +2792,36 @@ struct Engine::Impl {
   uint64_t setup_events = 0;  // sweep scratch (re)allocations: a run that made one is not timed for the tuner
   std::string tune_key(const std::vector<JobState *> &jobs, const EngineOptions &eo, bool &any_drop) const;
   TuneState &tune_state(const std::string &key, const EngineOptions &eo, bool any_drop);
-  bool plan(JobState &j);
+  bool plan(JobState &j, bool dev_ok);
+  // Word steps planned on the device (persistent pair sweep): a ring of
+  // in-flight steps, each with its page-locked tables and figures and four
+  // events (step start, sweep start, sweep end, step end). A slot is reused
+  // after its step's figures have been read back (ds_resolve).
+  struct DsSlot {
+    bool busy = false;
+    std::vector<JobState *> jobs;
+    HostBuf<JobDev> h_jobs;
+    HostBuf<StepJob> h_sj;
+    HostBuf<int64_t> h_stat;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  };
+  static constexpr int kDsRing = 2;
+  DsSlot ds[kDsRing];
+  uint64_t ds_next = 0;
+  DevBuf<StepJob> d_sjobs;
+  DevBuf<int32_t> d_fflag, d_fpos, d_upos, d_hist;
+  DevBuf<int64_t> d_scan_sum, d_ds_stat;
+  DevBuf<StepLen> d_len;
+  DevBuf<SweepDyn> d_dyn;
+  DevBuf<int> d_counters;
+  bool device_steps_ok(const EngineOptions &eo) const {
+    return eo.device_steps && eo.mode != ENGINE_PLAIN && eo.sweep_kernel == 0 && !arena;
+  }
+  void step_device(const std::vector<JobState *> &act, const EngineOptions &eo);
+  void ds_resolve(DsSlot &sl);
+  void ds_drain();
+  template <class Flag>
+  void scan(Flag flag, const int64_t *n_p, int64_t n, int32_t *out);
   std::vector<JobState *> to_finalize;  // jobs plan() found finished
   HostBuf<uint8_t> h_final;
   // Finished jobs whose result copies are in flight (finalize_begin) and not
@@ -2615,6 +3145,14 @@ void order_live(JobState &j) {
   for (int32_t f : j.live) sorted[size_t(cnt[size_t(key[size_t(f)])]++)] = f;
   j.live.swap(sorted);
   j.live_by_site = j.direct;
+  // Units of a step over every live fault: one per region with a non-PO head.
+  j.stems_cap = 0;
+  int32_t prev = -1;
+  for (int32_t f : j.live) {
+    const int32_t h = j.head[size_t(f)];
+    if (h != prev && !j.c->head_is_po[size_t(h)]) ++j.stems_cap;
+    prev = h;
+  }
 }
 }  // namespace
 
@@ -2627,7 +3165,7 @@ int64_t Engine::noalloc_min_bytes(const CircuitRes *c, int64_t faults, int64_t R
 
 int64_t Engine::job_bytes(const CircuitRes *c, int64_t faults, int64_t R, bool drop) {
   const int64_t RB = (R + 63) / 64;
-  return 8 * c->N * R + 8 * c->P * R + 4 * std::max<int64_t>(c->NL, 1) + faults * (4 + 4 + 1 + 4 + 4 + 1) +
+  return 8 * c->N * R + 8 * c->P * R + 4 * std::max<int64_t>(c->NL, 1) + faults * (4 + 4 + 1 + 4 + 4 + 1 + 16) +
          (drop ? 0 : 8 * faults * RB);
 }
 
@@ -2671,9 +3209,12 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
     const size_t n_good = size_t(std::max<int64_t>(c->N * R, 1)), n_hs = size_t(std::max<int64_t>(c->NL, 1));
     const size_t n_det = opt.drop ? 0 : size_t(std::max<int64_t>(j.F * j.RB, 1));
     const bool with_last = opt.drop && opt.report_last_round;
+    const size_t L1 = std::max<size_t>(j.live.size(), 1);
     const size_t o_good = take(n_good * 8), o_piw = take(piw.size() * 8), o_site = take(size_t(F1) * 4),
                  o_head = take(size_t(F1) * 4), o_sa = take(size_t(F1)), o_first = take(size_t(F1) * 4),
-                 o_hs = take(n_hs * 4), o_det = take(n_det * 8), o_last = take(with_last ? size_t(F1) : 0);
+                 o_hs = take(n_hs * 4), o_det = take(n_det * 8), o_last = take(with_last ? size_t(F1) : 0),
+                 o_live = take(L1 * 4), o_fl0 = take(L1 * 4), o_fl1 = take(L1 * 4), o_units = take(L1 * 4),
+                 o_cnt = take(2 * 4);
     if (!m.arenas.take(total, j.arena)) {
       try {
         j.arena.alloc(total);
@@ -2698,6 +3239,11 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
     j.head_slot.view(reinterpret_cast<int32_t *>(b + o_hs), n_hs);
     if (n_det) j.detbits.view(reinterpret_cast<uint64_t *>(b + o_det), n_det);
     if (with_last) j.last.view(b + o_last, size_t(F1));
+    j.d_live.view(reinterpret_cast<int32_t *>(b + o_live), L1);
+    j.d_fl0.view(reinterpret_cast<int32_t *>(b + o_fl0), L1);
+    j.d_fl1.view(reinterpret_cast<int32_t *>(b + o_fl1), L1);
+    j.d_units.view(reinterpret_cast<int32_t *>(b + o_units), L1);
+    j.d_cnt.view(reinterpret_cast<int32_t *>(b + o_cnt), 2);
   }
   m.h2d(j.pi_words.p, piw.data(), piw.size() * sizeof(uint64_t));
   m.h2d(j.d_site.p, j.site.data(), j.F * sizeof(int32_t));
@@ -2710,6 +3256,7 @@ JobState *Engine::add_job(CircuitRes *c, std::vector<int32_t> fault_site, std::v
   if (j.detbits.p) LSIM_HIP(hipMemsetAsync(j.detbits.p, 0, j.detbits.bytes(), m.stream));
   if (j.last.p) LSIM_HIP(hipMemsetAsync(j.last.p, 0, j.last.bytes(), m.stream));
   order_live(j);
+  m.h2d(j.d_live.p, j.live.data(), j.live.size() * sizeof(int32_t));
   j.alive = j.live;
   j.Wt = std::max(1, std::min(64, opt.first_words));
   j.Wg = std::max(1, std::min(64, opt.word_growth));
@@ -2726,6 +3273,7 @@ void Engine::release_job(JobState *j) {
   // engine's cache, where only work queued later on the same stream reuses
   // it, so no wait.
   AllocStream alloc_on(m.stream, m.arena);
+  m.ds_drain();
   for (auto it = m.jobs.begin(); it != m.jobs.end(); ++it)
     if (it->get() == j) {
       m.arenas.put(j->arena);
@@ -2750,7 +3298,14 @@ bool Engine::reset_job(JobState *j, const JobOptions &opt) {
   j->units.clear();
   j->step_w0 = 0;
   j->step_Wn = 0;
-  if (j->direct != j->live_by_site) order_live(*j);
+  m.ds_drain();
+  j->dev = false;
+  j->dev_steps = 0;
+  j->dev_alive = -1;
+  if (j->direct != j->live_by_site) {
+    order_live(*j);
+    m.h2d(j->d_live.p, j->live.data(), j->live.size() * sizeof(int32_t));
+  }
   j->alive = j->live;
   j->res = RunResults();
   j->res.stats.live_nodes = j->c->NL;
@@ -3164,8 +3719,10 @@ void Engine::Impl::finish_start(const std::vector<JobState *> &js, double ms) {
 
 // Decides the job's next step; false (and the job queued for finalize_begin)
 // when none is left.
-bool Engine::Impl::plan(JobState &j) {
+bool Engine::Impl::plan(JobState &j, bool dev_ok) {
   if (j.done) return false;
+  // A run's steps are all planned on the host or all on the device.
+  if (j.res.stats.steps == 0) j.dev = dev_ok;
   if (j.R == 0 || j.live.empty()) {
     to_finalize.push_back(&j);
     return false;
@@ -3176,16 +3733,20 @@ bool Engine::Impl::plan(JobState &j) {
       return false;
     }
     j.kind = JobState::MATRIX;
-    j.step_faults = j.live;
+    if (!j.dev) j.step_faults = j.live;
     j.step_w0 = j.w0;
     j.step_Wn = int(std::min<int64_t>(64, j.R - j.w0));
     return true;
   }
-  if (j.w0 < j.R && !j.alive.empty()) {
+  // With steps planned on the device the host learns a step's fault count a
+  // step or two later; until it reads 0 the job keeps stepping (an empty step
+  // sweeps nothing).
+  const bool left = j.dev ? j.dev_alive != 0 : !j.alive.empty();
+  if (j.w0 < j.R && left) {
     // Words per step grow geometrically: faults left after the first rounds
     // are the hard ones and are dropped less often.
     j.kind = JobState::DROP;
-    j.step_faults = j.alive;
+    if (!j.dev) j.step_faults = j.alive;
     j.step_w0 = j.w0;
     j.step_Wn = int(std::min<int64_t>(j.Wt, j.R - j.w0));
     // A remainder of at most a quarter of this step joins it: a launch for a
@@ -3196,7 +3757,7 @@ bool Engine::Impl::plan(JobState &j) {
   }
   if (j.opt.report_last_round && !j.last_done) {
     j.kind = JobState::LAST;
-    j.step_faults = j.live;
+    if (!j.dev) j.step_faults = j.live;
     j.step_w0 = j.R - 1;
     j.step_Wn = 1;
     return true;
@@ -3632,6 +4193,8 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   a.dense_lanes = eo.pair_dense;
   a.fo_pre = eo.pair_fo_pre ? 1 : 0;
   a.nopend = nopend ? 1 : 0;
+  a.dyn = nullptr;
+  a.ntiles_p = nullptr;
   // A share of the device's resident workgroups (another engine on the same
   // device may hold the rest).
   const double share = std::min(1.0, std::max(0.0, eo.slot_share));
@@ -3679,6 +4242,315 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   d2h(&again, d_ovf_count.p, sizeof(int));
   LSIM_HIP(hipStreamSynchronize(stream));
   if (again) throw DeviceError(LSIM_E_DEVICE, "a tile outgrew a pool of 64 words per live node");
+}
+
+template <class Flag>
+void Engine::Impl::scan(Flag flag, const int64_t *n_p, int64_t n, int32_t *out) {
+  k_scan_sums<Flag><<<kScanBlocks, kScanThreads, 0, stream>>>(flag, n_p, n, d_scan_sum.p);
+  k_scan_offsets<<<1, kScanBlocks, 0, stream>>>(d_scan_sum.p, n_p, n, out);
+  k_scan_write<Flag><<<kScanBlocks, kScanThreads, 0, stream>>>(flag, n_p, n, d_scan_sum.p, out);
+  LSIM_HIP(hipGetLastError());
+}
+
+// One word step of every job in `act`, planned on the device. Every buffer is
+// sized by the step's capacities (each job's live faults, its units times the
+// step's words), which the host knows without reading anything back; the
+// kernels read the actual counts from device memory.
+void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineOptions &eo) {
+  const int nj = int(act.size());
+  DsSlot &sl = ds[ds_next % kDsRing];
+  ds_resolve(sl);
+  for (hipEvent_t &e : sl.ev)
+    if (!e) LSIM_HIP(hipEventCreate(&e));
+  sl.h_jobs.alloc(size_t(nj));
+  sl.h_sj.alloc(size_t(nj));
+  sl.h_stat.alloc(size_t(kDsStat + 2 * nj));
+  int64_t F = 0, W = 0, T = 0, nl_max = 1;
+  int L = 0;
+  bool dense = false;
+  for (int i = 0; i < nj; ++i) {
+    JobState &j = *act[i];
+    const CircuitRes &c = *j.c;
+    const int64_t L1 = std::max<int64_t>(int64_t(j.live.size()), 1);
+    const int64_t U1 = j.direct ? L1 : std::max<int64_t>(1, std::min<int64_t>(L1, j.stems_cap));
+    JobDev d = job_dev(j);
+    d.w0 = j.step_w0;
+    d.Wn = j.step_Wn;
+    d.first_round = (j.kind == JobState::LAST) ? nullptr : j.first.p;
+    d.det_bits = (j.kind == JobState::MATRIX) ? j.detbits.p : nullptr;
+    d.last_flag = (j.kind == JobState::LAST) ? j.last.p : nullptr;
+    const int cur = j.dev_steps & 1, prev = cur ^ 1;
+    int32_t *dst = cur ? j.d_fl1.p : j.d_fl0.p;
+    d.flist = dst;
+    d.units = j.d_units.p;
+    sl.h_jobs.p[i] = d;
+    StepJob S;
+    std::memset(&S, 0, sizeof(S));
+    if (j.kind == JobState::DROP && j.dev_steps > 0) {
+      // The faults of the job's previous step that are still undetected.
+      S.src = prev ? j.d_fl1.p : j.d_fl0.p;
+      S.src_n = j.d_cnt.p + prev;
+      S.filter = 1;
+    } else {
+      S.src = j.d_live.p;
+      S.src_count = int32_t(j.live.size());
+      S.filter = j.kind == JobState::DROP ? 1 : 0;
+    }
+    S.dst = dst;
+    S.dst_n = j.d_cnt.p + cur;
+    S.units = j.d_units.p;
+    S.fbase = F;
+    S.wide = c.wide ? 1 : 0;
+    sl.h_sj.p[i] = S;
+    F += L1;
+    W += U1 * j.step_Wn;
+    T += (U1 * j.step_Wn + 63) / 64;
+    L = std::max(L, c.L);
+    nl_max = std::max(nl_max, c.NL);
+    dense = dense || j.kind == JobState::MATRIX || (j.kind == JobState::DROP && j.w0 == 0);
+  }
+  if (F >= INT32_MAX || W >= INT32_MAX) throw DeviceError(LSIM_E_DEVICE, "too many faults or unit words in one step");
+  // Tile epochs are unique over the engine's lifetime (a step's tiles and
+  // their reruns take at most 2 T + 2); before the 32-bit counter wraps, the
+  // records are cleared and numbering restarts.
+  if (uint64_t(epoch_base) + 2 * uint64_t(T) + 4 >= (uint64_t(1) << 31)) {
+    LSIM_HIP(hipMemsetAsync(d_prec.p, 0, d_prec.bytes(), stream));
+    LSIM_HIP(hipMemsetAsync(d_ppend.p, 0, d_ppend.bytes(), stream));
+    epoch_base = 0;
+  }
+  const uint32_t epoch = epoch_base;
+  epoch_base += uint32_t(2 * T + 2);
+  // Pair sweep scratch, as for a step planned on the host.
+  const int lvl_entries = L + 1;
+  const size_t lds = size_t(lvl_entries);
+  int pc256 = 0, pc512 = 0;
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false), 256, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false), 512, lds));
+  if (pc512 < 1)
+    throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
+  int64_t nl_all = nl_max;
+  for (const auto &c : circuits) nl_all = std::max(nl_all, c->NL);
+  try {
+    ensure_pair_scratch(eo, nl_all, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  } catch (const DeviceError &e) {
+    if (e.code != LSIM_E_NOMEM || nl_all == nl_max) throw;
+    ensure_pair_scratch(eo, nl_max, int64_t(cus) * std::max(pc256, pc512), int64_t(cus) * pc512);
+  }
+  // Step buffers at capacity (grow only).
+  d_fflag.alloc(size_t(F + 1));
+  d_fpos.alloc(size_t(F + 1));
+  d_upos.alloc(size_t(F + 1));
+  d_need.alloc(size_t(std::max<int64_t>(W, 1)));
+  d_dout.alloc(size_t(std::max<int64_t>(W, 1)));
+  d_pos.alloc(size_t(W + 1));
+  d_lanes.alloc(size_t(std::max<int64_t>(W, 1)));
+  d_tiles_raw.alloc(size_t(std::max<int64_t>(T, 1)));
+  d_tiles.alloc(size_t(std::max<int64_t>(T, 1)));
+  d_tile_key.alloc(size_t(std::max<int64_t>(T, 1)));
+  d_ovf_list.alloc(size_t(std::max<int64_t>(T, 1)));
+  d_ovf_list2.alloc(size_t(std::max<int64_t>(T, 1)));
+  d_hist.alloc(size_t(lvl_entries));
+  d_jobs.alloc(size_t(nj));
+  d_sjobs.alloc(size_t(nj));
+  d_item_off.alloc(size_t(nj + 1));
+  d_lane_off.alloc(size_t(nj + 1));
+  d_tile_off.alloc(size_t(nj + 1));
+  d_scan_sum.alloc(kScanBlocks);
+  d_len.alloc(1);
+  d_counters.alloc(4);
+  d_ds_stat.alloc(size_t(kDsStat + 2 * nj));
+  if (!d_dyn.p) {
+    d_dyn.alloc(1);
+    LSIM_HIP(hipMemsetAsync(d_dyn.p, 0, sizeof(SweepDyn), stream));
+  }
+  h2d(d_jobs.p, sl.h_jobs.p, size_t(nj) * sizeof(JobDev));
+  h2d(d_sjobs.p, sl.h_sj.p, size_t(nj) * sizeof(StepJob));
+  LSIM_HIP(hipEventRecord(sl.ev[0], stream));
+  const unsigned gF = std::max(1u, std::min<unsigned>(kGsBlocks, blocks_for(F, 256)));
+  // The step's faults: the previous list filtered by detection (or the live
+  // list), kept in unit order; then its units.
+  k_ds_keep<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p);
+  LSIM_HIP(hipGetLastError());
+  scan(FlagI32{d_fflag.p}, nullptr, F, d_fpos.p);
+  k_ds_compact<<<gF, 256, 0, stream>>>(d_sjobs.p, nj, F, d_fflag.p, d_fpos.p);
+  k_ds_unit_flag<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_fflag.p);
+  LSIM_HIP(hipGetLastError());
+  scan(FlagI32{d_fflag.p}, nullptr, F, d_upos.p);
+  k_ds_units<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p, d_upos.p);
+  k_ds_plan<<<1, kPlanThreads, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_upos.p, d_item_off.p, d_len.p,
+                                             d_ds_stat.p + kDsStat);
+  k_ds_zero<<<kGsBlocks, 256, 0, stream>>>(d_len.p, d_need.p, d_dout.p);
+  k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 0);
+  LSIM_HIP(hipGetLastError());
+  // Local terms and the stems' needed bits, then lanes and tiles.
+  ItemArgs ia;
+  ia.jobs = d_jobs.p;
+  ia.njobs = nj;
+  ia.item_off = d_item_off.p;
+  ia.need = d_need.p;
+  ia.dout = d_dout.p;
+  k_local_gs<<<kGsBlocks, 256, 0, stream>>>(ia);
+  LSIM_HIP(hipGetLastError());
+  scan(FlagWord{d_need.p}, &d_len.p->words, 0, d_pos.p);
+  k_lane_fill_gs<<<kGsBlocks, 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_len.p, d_lanes.p);
+  LSIM_HIP(hipGetLastError());
+  // Which workgroup sizes may run: the one asked for, or both (the device
+  // picks one; the other returns at once).
+  const bool allow256 = pc256 >= 1;
+  const bool run512 = eo.pair_threads != 256 || !allow256;
+  const bool run256 = allow256 && eo.pair_threads != 512;
+  TilePlanArgs tp;
+  tp.sj = d_sjobs.p;
+  tp.jobs = d_jobs.p;
+  tp.nj = nj;
+  tp.pos = d_pos.p;
+  tp.len = d_len.p;
+  tp.lane_off = d_lane_off.p;
+  tp.tile_off = d_tile_off.p;
+  tp.hist = d_hist.p;
+  tp.nkeys = lvl_entries;
+  tp.dyn = d_dyn.p;
+  tp.bytes = d_bytes.p;
+  tp.counters = d_counters.p;
+  tp.epoch_base = epoch;
+  tp.dense = dense ? 1 : 0;
+  tp.threads = eo.pair_threads;
+  tp.allow256 = allow256 ? 1 : 0;
+  tp.nopend_mode = eo.pair_nopend;
+  tp.nopend_pct = eo.pair_nopend_pct;
+  k_ds_tiles<<<1, kPlanThreads, 0, stream>>>(tp);
+  TileSortArgs ts;
+  ts.jobs = d_jobs.p;
+  ts.nj = nj;
+  ts.lanes = d_lanes.p;
+  ts.lane_off = d_lane_off.p;
+  ts.tile_off = d_tile_off.p;
+  ts.len = d_len.p;
+  ts.raw = d_tiles_raw.p;
+  ts.keys = d_tile_key.p;
+  ts.hist = d_hist.p;
+  ts.out = d_tiles.p;
+  k_ds_tile_hist<<<kGsBlocks, 256, 0, stream>>>(ts);
+  k_ds_hist_scan<<<1, kPlanThreads, 0, stream>>>(d_hist.p, lvl_entries);
+  k_ds_tile_scatter<<<kGsBlocks, 256, 0, stream>>>(ts);
+  LSIM_HIP(hipGetLastError());
+  // The sweep: the chosen workgroup size runs the step's tiles; tiles that
+  // outgrow their slot's pool are swept again in slots that hold any tile.
+  LSIM_HIP(hipEventRecord(sl.ev[1], stream));
+  const double share = std::min(1.0, std::max(0.0, eo.slot_share));
+  const int64_t full = pair_nl * 64;
+  const int64_t pool_all = int64_t(d_pool.n);
+  PairArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.jobs = d_jobs.p;
+  a.tiles = d_tiles.p;
+  a.tile_list = nullptr;
+  a.lvl_entries = lvl_entries;
+  a.lanes = d_lanes.p;
+  a.out = d_dout.p;
+  a.bytes = d_bytes.p;
+  a.rec = d_prec.p;
+  a.pend = d_ppend.p;
+  a.rec_stride = pair_nl;
+  a.pool = d_pool.p;
+  a.dense_lanes = eo.pair_dense;
+  a.fo_pre = eo.pair_fo_pre ? 1 : 0;
+  a.tstats = nullptr;
+  a.dyn = d_dyn.p;
+  a.ntiles_p = nullptr;
+  bool may_overflow = false;
+  for (int form : {512, 256}) {
+    if ((form == 512 && !run512) || (form == 256 && !run256)) continue;
+    const int per_cu = form == 256 ? pc256 : pc512;
+    const int64_t resident = std::max<int64_t>(1, int64_t(double(int64_t(cus) * per_cu) * share));
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(resident, pair_slots));
+    a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / grid), int64_t(UINT32_MAX));
+    if (eo.pool_cap > 0) a.pool_words = std::min<int64_t>(a.pool_words, eo.pool_cap);
+    may_overflow = may_overflow || a.pool_words < full;
+    a.queue = d_counters.p + 0;
+    a.ovf_count = d_counters.p + 2;
+    a.ovf_list = d_ovf_list.p;
+    hipLaunchKernelGGL(pairs_kernel(form, false), dim3(unsigned(grid)), dim3(form), lds, stream, a);
+    LSIM_HIP(hipGetLastError());
+  }
+  if (may_overflow) {
+    const int64_t resident = std::max<int64_t>(1, int64_t(double(int64_t(cus) * pc512) * share));
+    const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(resident, std::min(pair_slots, pool_all / full)));
+    a.tile_list = d_ovf_list.p;
+    a.ntiles_p = d_counters.p + 2;
+    a.queue = d_counters.p + 1;
+    a.ovf_count = d_counters.p + 3;
+    a.ovf_list = d_ovf_list2.p;
+    a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / g2), int64_t(UINT32_MAX));
+    hipLaunchKernelGGL(pairs_kernel(512, false), dim3(unsigned(g2)), dim3(512), lds, stream, a);
+    LSIM_HIP(hipGetLastError());
+  }
+  LSIM_HIP(hipEventRecord(sl.ev[2], stream));
+  finalize_end();  // finished jobs' results unpacked while the sweep runs
+  // Detection and the step's figures.
+  k_detect_gs<<<kGsBlocks, 256, 0, stream>>>(ia);
+  k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 1);
+  k_ds_stats<<<1, 1, 0, stream>>>(d_len.p, d_bytes.p, d_counters.p, d_ds_stat.p);
+  LSIM_HIP(hipGetLastError());
+  d2h(sl.h_stat.p, d_ds_stat.p, size_t(kDsStat + 2 * nj) * sizeof(int64_t));
+  LSIM_HIP(hipEventRecord(sl.ev[3], stream));
+  sl.busy = true;
+  sl.jobs = act;
+  ++ds_next;
+  for (JobState *jp : act) {
+    JobState &j = *jp;
+    j.res.stats.steps++;
+    j.dev_steps++;
+    switch (j.kind) {
+      case JobState::MATRIX: j.w0 += j.step_Wn; break;
+      case JobState::DROP:
+        j.w0 += j.step_Wn;
+        j.Wt = std::min(std::max(1, eo.max_step_words), j.Wt * j.Wg);
+        break;
+      case JobState::LAST: j.last_done = true; break;
+    }
+  }
+}
+
+// Reads back a device step's figures (waiting for the step) and adds them to
+// its jobs' statistics; a tile that outgrew even a full pool is an error.
+void Engine::Impl::ds_resolve(DsSlot &sl) {
+  if (!sl.busy) return;
+  sl.busy = false;
+  const std::vector<JobState *> js = std::move(sl.jobs);
+  sl.jobs.clear();
+  LSIM_HIP(hipEventSynchronize(sl.ev[3]));
+  float t_pre = 0, t_sweep = 0, t_post = 0;
+  LSIM_HIP(hipEventElapsedTime(&t_pre, sl.ev[0], sl.ev[1]));
+  LSIM_HIP(hipEventElapsedTime(&t_sweep, sl.ev[1], sl.ev[2]));
+  LSIM_HIP(hipEventElapsedTime(&t_post, sl.ev[2], sl.ev[3]));
+  const int64_t *st = sl.h_stat.p;
+  const int64_t launches = 1 + (st[3] > 0 ? 1 : 0);
+  for (size_t i = 0; i < js.size(); ++i) {
+    JobState &j = *js[i];
+    lsim_run_stats &r = j.res.stats;
+    r.t_sweep_ms += t_sweep;
+    r.t_detect_ms += t_pre + t_post;
+    r.sweep_launches += launches;
+    r.sweep_tiles += st[2];
+    r.sweep_bytes += st[0];
+    r.sweep_reruns += st[3];
+    const int64_t nf = st[kDsStat + 2 * i], nu = st[kDsStat + 2 * i + 1];
+    if (!j.direct) r.stems += nu;
+    j.dev_alive = nf;
+  }
+  if (trace_on())
+    std::fprintf(stderr,
+                 "[lsim] device step jobs=%zu items=%lld words=%lld lanes=%lld tiles=%lld reruns=%lld | plan+local %.2f "
+                 "sweep %.2f detect %.2f ms\n",
+                 js.size(), (long long)st[5], (long long)st[6], (long long)st[1], (long long)st[2], (long long)st[3],
+                 t_pre, t_sweep, t_post);
+  if (st[4]) throw DeviceError(LSIM_E_DEVICE, "a tile outgrew a pool of 64 words per live node");
+}
+
+// Every in-flight device step, oldest first.
+void Engine::Impl::ds_drain() {
+  for (int k = 0; k < kDsRing; ++k) ds_resolve(ds[(ds_next + uint64_t(k)) % kDsRing]);
 }
 
 void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOptions &eo) {
@@ -3789,7 +4661,10 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     d_unit_pos.alloc(size_t(nu_all));
     h2d(d_unit_pos.p, h_unit_pos.p, size_t(nu_all) * sizeof(int32_t));
   }
-  for (int i = 0; i < nj; ++i) devs[i].flist = d_flist.p + flist_off[i];
+  for (int i = 0; i < nj; ++i) {
+    devs[i].flist = d_flist.p + flist_off[i];
+    devs[i].units = d_unit_pos.p + lane_unit_off[i];
+  }
   h2d(d_flist.p, h_flist.p, size_t(nf_all) * sizeof(int32_t));
   d_jobs.alloc(nj);
   d_item_off.alloc(nj + 1);
@@ -3838,8 +4713,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
       LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flags, d_pos.p, int(nw), stream));
       d_cub.alloc(tmp);
       LSIM_HIP(hipcub::DeviceScan::ExclusiveSum(d_cub.p, tmp, flags, d_pos.p, int(nw), stream));
-      k_lane_fill<<<blocks_for(nw, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_unit_pos.p, nw,
-                                                          d_lanes.p);
+      k_lane_fill<<<blocks_for(nw, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, nw, d_lanes.p);
       LSIM_HIP(hipGetLastError());
     }
     k_lane_bounds<<<blocks_for(nj + 1, 256), 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, nw, d_lane_off.p);
@@ -4077,26 +4951,39 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
 int64_t Engine::Impl::iterate_jobs(const std::vector<JobState *> &jobs, const EngineOptions &eo) {
   std::vector<JobState *> act;
   to_finalize.clear();
+  const bool dev_ok = device_steps_ok(eo);
   for (JobState *j : jobs)
-    if (j->started && !j->done && plan(*j)) act.push_back(j);
+    if (j->started && !j->done && plan(*j, dev_ok)) act.push_back(j);
+  // Finishing jobs take the figures of their device steps first.
+  if (!to_finalize.empty()) ds_drain();
   finalize_begin(to_finalize);
   to_finalize.clear();
   try {
-    if (!act.empty()) {
+    std::vector<JobState *> on_dev, on_host;
+    for (JobState *j : act) (j->dev ? on_dev : on_host).push_back(j);
+    if (!on_host.empty()) {
       if (eo.mode == ENGINE_PLAIN) {
-        for (JobState *j : act) step_batch({j}, eo);  // the per-level form runs one job at a time
+        for (JobState *j : on_host) step_batch({j}, eo);  // the per-level form runs one job at a time
       } else {
-        step_batch(act, eo);
+        step_batch(on_host, eo);
       }
     }
+    if (!on_dev.empty()) step_device(on_dev, eo);
     finalize_end();
   } catch (...) {
     // Jobs left unpacked stay unfinished: the next plan() queues them again.
     // Their result copies may still be in flight into h_final / h_stage:
-    // wait for them before the buffers can be reused.
+    // wait for them before the buffers can be reused. In-flight device steps
+    // are waited for and their figures dropped.
     if (!fin_js.empty()) (void)hipEventSynchronize(fin_ev);
     fin_js.clear();
     fin_scatter.clear();
+    for (DsSlot &sl : ds)
+      if (sl.busy) {
+        if (sl.ev[3]) (void)hipEventSynchronize(sl.ev[3]);
+        sl.busy = false;
+        sl.jobs.clear();
+      }
     throw;
   }
   int64_t left = 0;

@@ -86,6 +86,7 @@ struct EngineOptions {
   int max_step_words = 128;          // with dropping: most words of one step (the growth stops there)
   double slot_share = 1.0;           // pair sweep: use at most this share of the resident workgroup slots
   bool autotune = false;             // run(): pick sweep kernel and word schedule by timed probes (cached)
+  bool device_steps = true;          // persistent pair sweep: plan each word step on the device (no host round trip)
 };
 
 struct RunResults {
