@@ -232,6 +232,7 @@ typedef struct {
   int64_t device_allocs;   /* device and page-locked host allocations the library made
                               during the call (lsim_run, lsim_run_noalloc; counted over
                               all threads of the process) */
+  int64_t steps_replayed;  /* word steps whose launches replayed captured hipGraphs */
 } lsim_run_stats;
 LSIM_API int lsim_last_run_stats(const lsim_sim *s, lsim_run_stats *out);
 

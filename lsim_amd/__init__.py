@@ -61,6 +61,7 @@ class RunStats(ctypes.Structure):
         ("tune_candidates", ctypes.c_int64),
         ("good_replayed", ctypes.c_int64),
         ("device_allocs", ctypes.c_int64),
+        ("steps_replayed", ctypes.c_int64),
     ]
 
 

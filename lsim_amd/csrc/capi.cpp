@@ -241,6 +241,8 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
     eo.pair_dense = int(value);
   } else if (k == "autotune") {
     eo.autotune = value != 0;
+  } else if (k == "step_graphs") {
+    eo.step_graphs = value != 0;
   } else if (k == "device_steps") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "device_steps must be 0 or 1");
     eo.device_steps = value != 0;
@@ -699,6 +701,7 @@ int lsim_batch_run(lsim_batch *b) {
       b->stats.tune_trials = s.tune_trials;
       b->stats.tune_candidates = s.tune_candidates;
       b->stats.good_replayed = s.good_replayed;
+      b->stats.steps_replayed = std::max(b->stats.steps_replayed, s.steps_replayed);
       b->stats.sweep_reruns = std::max(b->stats.sweep_reruns, s.sweep_reruns);
       b->stats.tuned = s.tuned;
       b->stats.engine = s.engine;

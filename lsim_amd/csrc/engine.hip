@@ -1697,6 +1697,7 @@ constexpr int kScanBlocks = 256;   // workgroups of a prefix sum's first and las
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;      // consecutive entries per thread in the last phase
 constexpr int kGsBlocks = 1024;    // workgroups of the fixed-grid kernels (256 threads)
+constexpr int kItemBlocks = 8192;  // at most, for the detection-item kernels
 constexpr int kPlanThreads = 1024; // the one-workgroup planning kernels
 constexpr int kDsStat = 8;         // step figures before the per-job ones
 
@@ -1965,7 +1966,7 @@ struct TilePlanArgs {
   SweepDyn *dyn;
   unsigned long long *bytes;    // sweep counters: bytes, staged nodes, walked nodes
   int *counters;                // tile queue of the first launch and of the rerun, tiles each gave up
-  uint32_t epoch_base;
+  const uint32_t *epoch;        // the step's first tile epoch
   int dense;                    // a dropping job's first step or a full-matrix step
   int threads;                  // workgroup size asked for (0: chosen here)
   int allow256;                 // the 256-thread form can be resident
@@ -2017,7 +2018,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_ds_tiles(TilePlanArgs a) {
     D.form = threads;
     D.nopend = (a.nopend_mode == 2 || (a.nopend_mode == 1 && (a.dense || 100.0 * D.share > double(a.nopend_pct)))) ? 1 : 0;
     D.ntiles = int(ct);
-    D.epoch_base = a.epoch_base;
+    D.epoch_base = *a.epoch;
     *a.dyn = D;
     a.bytes[0] = 0;
     a.bytes[1] = 0;
@@ -2803,8 +2804,53 @@ struct Engine::Impl {
     HostBuf<JobDev> h_jobs;
     HostBuf<StepJob> h_sj;
     HostBuf<int64_t> h_stat;
+    HostBuf<uint32_t> h_epoch;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool replayed = false;  // the step's launches replayed captured graphs
   };
+  // Captured launches of a device-planned step (plan, sweep, detection), by
+  // launch signature: every host-side kernel argument of the step. A
+  // signature is captured when it comes a second time (as in repeated runs
+  // and batches of one job set) and replayed from then on. Kernel nodes only.
+  struct StepGraph {
+    std::vector<uint64_t> sig;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    ~StepGraph() {
+      for (int k = 0; k < 3; ++k) {
+        if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+        if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      }
+    }
+  };
+  static constexpr size_t kStepGraphs = 32, kStepSeen = 128;
+  std::vector<std::unique_ptr<StepGraph>> step_graphs;  // most recent last
+  std::vector<uint64_t> step_seen;                      // signature hashes seen once
+  static uint64_t sig_hash(const std::vector<uint64_t> &sig) {
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t w : sig) h = (h ^ w) * 1099511628211ull;
+    return h;
+  }
+  StepGraph *find_step_graph(const std::vector<uint64_t> &sig) {
+    for (size_t i = 0; i < step_graphs.size(); ++i)
+      if (step_graphs[i]->sig == sig) {
+        std::unique_ptr<StepGraph> keep = std::move(step_graphs[i]);
+        step_graphs.erase(step_graphs.begin() + long(i));
+        step_graphs.push_back(std::move(keep));
+        return step_graphs.back().get();
+      }
+    return nullptr;
+  }
+  template <class Fn>
+  void capture_segment(Fn &&fn, hipGraph_t &graph, hipGraphExec_t &exec) {
+    LSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    fn();
+    const hipError_t le = hipGetLastError();
+    const hipError_t e = hipStreamEndCapture(stream, &graph);
+    LSIM_HIP(le);
+    LSIM_HIP(e);
+    LSIM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
   static constexpr int kDsRing = 2;
   DsSlot ds[kDsRing];
   uint64_t ds_next = 0;
@@ -2814,6 +2860,7 @@ struct Engine::Impl {
   DevBuf<StepLen> d_len;
   DevBuf<SweepDyn> d_dyn;
   DevBuf<int> d_counters;
+  DevBuf<uint32_t> d_epoch;
   bool device_steps_ok(const EngineOptions &eo) const {
     return eo.device_steps && eo.mode != ENGINE_PLAIN && eo.sweep_kernel == 0 && !arena;
   }
@@ -4249,7 +4296,6 @@ void Engine::Impl::scan(Flag flag, const int64_t *n_p, int64_t n, int32_t *out) 
   k_scan_sums<Flag><<<kScanBlocks, kScanThreads, 0, stream>>>(flag, n_p, n, d_scan_sum.p);
   k_scan_offsets<<<1, kScanBlocks, 0, stream>>>(d_scan_sum.p, n_p, n, out);
   k_scan_write<Flag><<<kScanBlocks, kScanThreads, 0, stream>>>(flag, n_p, n, d_scan_sum.p, out);
-  LSIM_HIP(hipGetLastError());
 }
 
 // One word step of every job in `act`, planned on the device. Every buffer is
@@ -4265,7 +4311,8 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   sl.h_jobs.alloc(size_t(nj));
   sl.h_sj.alloc(size_t(nj));
   sl.h_stat.alloc(size_t(kDsStat + 2 * nj));
-  int64_t F = 0, W = 0, T = 0, nl_max = 1;
+  sl.h_epoch.alloc(1);
+  int64_t F = 0, W = 0, T = 0, I = 0, nl_max = 1;
   int L = 0;
   bool dense = false;
   for (int i = 0; i < nj; ++i) {
@@ -4303,6 +4350,7 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
     S.wide = c.wide ? 1 : 0;
     sl.h_sj.p[i] = S;
     F += L1;
+    I += L1 * j.step_Wn;
     W += U1 * j.step_Wn;
     T += (U1 * j.step_Wn + 63) / 64;
     L = std::max(L, c.L);
@@ -4363,43 +4411,30 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
     d_dyn.alloc(1);
     LSIM_HIP(hipMemsetAsync(d_dyn.p, 0, sizeof(SweepDyn), stream));
   }
+  if (!d_epoch.p) d_epoch.alloc(1);
+  *sl.h_epoch.p = epoch;
   h2d(d_jobs.p, sl.h_jobs.p, size_t(nj) * sizeof(JobDev));
   h2d(d_sjobs.p, sl.h_sj.p, size_t(nj) * sizeof(StepJob));
-  LSIM_HIP(hipEventRecord(sl.ev[0], stream));
+  h2d(d_epoch.p, sl.h_epoch.p, sizeof(uint32_t));
   const unsigned gF = std::max(1u, std::min<unsigned>(kGsBlocks, blocks_for(F, 256)));
-  // The step's faults: the previous list filtered by detection (or the live
-  // list), kept in unit order; then its units.
-  k_ds_keep<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p);
-  LSIM_HIP(hipGetLastError());
-  scan(FlagI32{d_fflag.p}, nullptr, F, d_fpos.p);
-  k_ds_compact<<<gF, 256, 0, stream>>>(d_sjobs.p, nj, F, d_fflag.p, d_fpos.p);
-  k_ds_unit_flag<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_fflag.p);
-  LSIM_HIP(hipGetLastError());
-  scan(FlagI32{d_fflag.p}, nullptr, F, d_upos.p);
-  k_ds_units<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p, d_upos.p);
-  k_ds_plan<<<1, kPlanThreads, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_upos.p, d_item_off.p, d_len.p,
-                                             d_ds_stat.p + kDsStat);
-  k_ds_zero<<<kGsBlocks, 256, 0, stream>>>(d_len.p, d_need.p, d_dout.p);
-  k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 0);
-  LSIM_HIP(hipGetLastError());
-  // Local terms and the stems' needed bits, then lanes and tiles.
+  // Detection items: up to every step fault times the step's words; the
+  // grid covers them up to kItemBlocks workgroups (the device's resident
+  // limit several times over: the path walks are chains of dependent loads).
+  const unsigned gI = std::max(1u, std::min<unsigned>(kItemBlocks, blocks_for(I, 256)));
   ItemArgs ia;
+  std::memset(&ia, 0, sizeof(ia));  // padding too: the launch signature reads every byte
   ia.jobs = d_jobs.p;
   ia.njobs = nj;
   ia.item_off = d_item_off.p;
   ia.need = d_need.p;
   ia.dout = d_dout.p;
-  k_local_gs<<<kGsBlocks, 256, 0, stream>>>(ia);
-  LSIM_HIP(hipGetLastError());
-  scan(FlagWord{d_need.p}, &d_len.p->words, 0, d_pos.p);
-  k_lane_fill_gs<<<kGsBlocks, 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_len.p, d_lanes.p);
-  LSIM_HIP(hipGetLastError());
   // Which workgroup sizes may run: the one asked for, or both (the device
   // picks one; the other returns at once).
   const bool allow256 = pc256 >= 1;
   const bool run512 = eo.pair_threads != 256 || !allow256;
   const bool run256 = allow256 && eo.pair_threads != 512;
   TilePlanArgs tp;
+  std::memset(&tp, 0, sizeof(tp));
   tp.sj = d_sjobs.p;
   tp.jobs = d_jobs.p;
   tp.nj = nj;
@@ -4412,14 +4447,14 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   tp.dyn = d_dyn.p;
   tp.bytes = d_bytes.p;
   tp.counters = d_counters.p;
-  tp.epoch_base = epoch;
+  tp.epoch = d_epoch.p;
   tp.dense = dense ? 1 : 0;
   tp.threads = eo.pair_threads;
   tp.allow256 = allow256 ? 1 : 0;
   tp.nopend_mode = eo.pair_nopend;
   tp.nopend_pct = eo.pair_nopend_pct;
-  k_ds_tiles<<<1, kPlanThreads, 0, stream>>>(tp);
   TileSortArgs ts;
+  std::memset(&ts, 0, sizeof(ts));
   ts.jobs = d_jobs.p;
   ts.nj = nj;
   ts.lanes = d_lanes.p;
@@ -4430,13 +4465,9 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   ts.keys = d_tile_key.p;
   ts.hist = d_hist.p;
   ts.out = d_tiles.p;
-  k_ds_tile_hist<<<kGsBlocks, 256, 0, stream>>>(ts);
-  k_ds_hist_scan<<<1, kPlanThreads, 0, stream>>>(d_hist.p, lvl_entries);
-  k_ds_tile_scatter<<<kGsBlocks, 256, 0, stream>>>(ts);
-  LSIM_HIP(hipGetLastError());
-  // The sweep: the chosen workgroup size runs the step's tiles; tiles that
-  // outgrow their slot's pool are swept again in slots that hold any tile.
-  LSIM_HIP(hipEventRecord(sl.ev[1], stream));
+  // The sweep launches: the chosen workgroup size runs the step's tiles;
+  // tiles that outgrow their slot's pool are swept again in slots that hold
+  // any tile.
   const double share = std::min(1.0, std::max(0.0, eo.slot_share));
   const int64_t full = pair_nl * 64;
   const int64_t pool_all = int64_t(d_pool.n);
@@ -4444,7 +4475,6 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   std::memset(&a, 0, sizeof(a));
   a.jobs = d_jobs.p;
   a.tiles = d_tiles.p;
-  a.tile_list = nullptr;
   a.lvl_entries = lvl_entries;
   a.lanes = d_lanes.p;
   a.out = d_dout.p;
@@ -4455,45 +4485,126 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   a.pool = d_pool.p;
   a.dense_lanes = eo.pair_dense;
   a.fo_pre = eo.pair_fo_pre ? 1 : 0;
-  a.tstats = nullptr;
   a.dyn = d_dyn.p;
-  a.ntiles_p = nullptr;
+  a.queue = d_counters.p + 0;
+  a.ovf_count = d_counters.p + 2;
+  a.ovf_list = d_ovf_list.p;
+  struct SweepLaunch {
+    int form;
+    int64_t grid;
+    PairArgs args;
+  };
+  std::vector<SweepLaunch> launches;
   bool may_overflow = false;
   for (int form : {512, 256}) {
     if ((form == 512 && !run512) || (form == 256 && !run256)) continue;
     const int per_cu = form == 256 ? pc256 : pc512;
     const int64_t resident = std::max<int64_t>(1, int64_t(double(int64_t(cus) * per_cu) * share));
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(resident, pair_slots));
-    a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / grid), int64_t(UINT32_MAX));
-    if (eo.pool_cap > 0) a.pool_words = std::min<int64_t>(a.pool_words, eo.pool_cap);
-    may_overflow = may_overflow || a.pool_words < full;
-    a.queue = d_counters.p + 0;
-    a.ovf_count = d_counters.p + 2;
-    a.ovf_list = d_ovf_list.p;
-    hipLaunchKernelGGL(pairs_kernel(form, false), dim3(unsigned(grid)), dim3(form), lds, stream, a);
-    LSIM_HIP(hipGetLastError());
+    PairArgs f = a;
+    f.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / grid), int64_t(UINT32_MAX));
+    if (eo.pool_cap > 0) f.pool_words = std::min<int64_t>(f.pool_words, eo.pool_cap);
+    may_overflow = may_overflow || f.pool_words < full;
+    launches.push_back({form, grid, f});
   }
   if (may_overflow) {
     const int64_t resident = std::max<int64_t>(1, int64_t(double(int64_t(cus) * pc512) * share));
     const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(resident, std::min(pair_slots, pool_all / full)));
-    a.tile_list = d_ovf_list.p;
-    a.ntiles_p = d_counters.p + 2;
-    a.queue = d_counters.p + 1;
-    a.ovf_count = d_counters.p + 3;
-    a.ovf_list = d_ovf_list2.p;
-    a.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / g2), int64_t(UINT32_MAX));
-    hipLaunchKernelGGL(pairs_kernel(512, false), dim3(unsigned(g2)), dim3(512), lds, stream, a);
-    LSIM_HIP(hipGetLastError());
+    PairArgs r = a;
+    r.tile_list = d_ovf_list.p;
+    r.ntiles_p = d_counters.p + 2;
+    r.queue = d_counters.p + 1;
+    r.ovf_count = d_counters.p + 3;
+    r.ovf_list = d_ovf_list2.p;
+    r.pool_words = std::min<int64_t>(std::min<int64_t>(full, pool_all / g2), int64_t(UINT32_MAX));
+    launches.push_back({512, g2, r});
   }
+  // Three segments of launches, each one graph when the step recurs: plan
+  // (fault lists, units, local terms, lanes, tiles), sweep, detection.
+  auto seg_plan = [&] {
+    k_ds_keep<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p);
+    scan(FlagI32{d_fflag.p}, nullptr, F, d_fpos.p);
+    k_ds_compact<<<gF, 256, 0, stream>>>(d_sjobs.p, nj, F, d_fflag.p, d_fpos.p);
+    k_ds_unit_flag<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_fflag.p);
+    scan(FlagI32{d_fflag.p}, nullptr, F, d_upos.p);
+    k_ds_units<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fflag.p, d_upos.p);
+    k_ds_plan<<<1, kPlanThreads, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_fpos.p, d_upos.p, d_item_off.p, d_len.p,
+                                               d_ds_stat.p + kDsStat);
+    k_ds_zero<<<kGsBlocks, 256, 0, stream>>>(d_len.p, d_need.p, d_dout.p);
+    k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 0);
+    k_local_gs<<<gI, 256, 0, stream>>>(ia);
+    scan(FlagWord{d_need.p}, &d_len.p->words, 0, d_pos.p);
+    k_lane_fill_gs<<<kGsBlocks, 256, 0, stream>>>(d_jobs.p, nj, d_need.p, d_pos.p, d_len.p, d_lanes.p);
+    k_ds_tiles<<<1, kPlanThreads, 0, stream>>>(tp);
+    k_ds_tile_hist<<<kGsBlocks, 256, 0, stream>>>(ts);
+    k_ds_hist_scan<<<1, kPlanThreads, 0, stream>>>(d_hist.p, lvl_entries);
+    k_ds_tile_scatter<<<kGsBlocks, 256, 0, stream>>>(ts);
+  };
+  auto seg_sweep = [&] {
+    for (const SweepLaunch &l : launches)
+      hipLaunchKernelGGL(pairs_kernel(l.form, false), dim3(unsigned(l.grid)), dim3(l.form), lds, stream, l.args);
+  };
+  auto seg_detect = [&] {
+    k_detect_gs<<<gI, 256, 0, stream>>>(ia);
+    k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 1);
+    k_ds_stats<<<1, 1, 0, stream>>>(d_len.p, d_bytes.p, d_counters.p, d_ds_stat.p);
+  };
+  // The launch signature: every argument the host gives the step's kernels
+  // (the rest is read from device memory, uploaded above).
+  std::vector<uint64_t> sig = {uint64_t(nj), uint64_t(F), uint64_t(gF), uint64_t(gI), uint64_t(lvl_entries)};
+  auto add_bytes = [&](const void *p, size_t n) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (size_t o = 0; o < n; o += 8) {
+      uint64_t w = 0;
+      std::memcpy(&w, b + o, std::min<size_t>(8, n - o));
+      sig.push_back(w);
+    }
+  };
+  for (const void *q : {static_cast<const void *>(d_sjobs.p), static_cast<const void *>(d_fflag.p),
+                        static_cast<const void *>(d_fpos.p), static_cast<const void *>(d_upos.p),
+                        static_cast<const void *>(d_scan_sum.p), static_cast<const void *>(d_ds_stat.p)})
+    sig.push_back(uint64_t(reinterpret_cast<uintptr_t>(q)));
+  add_bytes(&ia, sizeof(ia));
+  add_bytes(&tp, sizeof(tp));
+  add_bytes(&ts, sizeof(ts));
+  for (const SweepLaunch &l : launches) {
+    sig.push_back(uint64_t(l.form));
+    sig.push_back(uint64_t(l.grid));
+    add_bytes(&l.args, sizeof(PairArgs));
+  }
+  StepGraph *g = eo.step_graphs ? find_step_graph(sig) : nullptr;
+  bool capture = false;
+  if (!g && eo.step_graphs) {
+    const uint64_t h = sig_hash(sig);
+    capture = std::find(step_seen.begin(), step_seen.end(), h) != step_seen.end();
+    if (!capture) {
+      if (step_seen.size() >= kStepSeen) step_seen.erase(step_seen.begin());
+      step_seen.push_back(h);
+    }
+  }
+  if (capture) {
+    auto ng = std::make_unique<StepGraph>();
+    ng->sig = sig;
+    capture_segment(seg_plan, ng->graph[0], ng->exec[0]);
+    capture_segment(seg_sweep, ng->graph[1], ng->exec[1]);
+    capture_segment(seg_detect, ng->graph[2], ng->exec[2]);
+    if (step_graphs.size() >= kStepGraphs) step_graphs.erase(step_graphs.begin());
+    step_graphs.push_back(std::move(ng));
+    g = step_graphs.back().get();
+  }
+  LSIM_HIP(hipEventRecord(sl.ev[0], stream));
+  if (g) LSIM_HIP(hipGraphLaunch(g->exec[0], stream)); else seg_plan();
+  LSIM_HIP(hipGetLastError());
+  LSIM_HIP(hipEventRecord(sl.ev[1], stream));
+  if (g) LSIM_HIP(hipGraphLaunch(g->exec[1], stream)); else seg_sweep();
+  LSIM_HIP(hipGetLastError());
   LSIM_HIP(hipEventRecord(sl.ev[2], stream));
   finalize_end();  // finished jobs' results unpacked while the sweep runs
-  // Detection and the step's figures.
-  k_detect_gs<<<kGsBlocks, 256, 0, stream>>>(ia);
-  k_ds_slots<<<gF, 256, 0, stream>>>(d_sjobs.p, d_jobs.p, nj, F, d_upos.p, 1);
-  k_ds_stats<<<1, 1, 0, stream>>>(d_len.p, d_bytes.p, d_counters.p, d_ds_stat.p);
+  if (g) LSIM_HIP(hipGraphLaunch(g->exec[2], stream)); else seg_detect();
   LSIM_HIP(hipGetLastError());
   d2h(sl.h_stat.p, d_ds_stat.p, size_t(kDsStat + 2 * nj) * sizeof(int64_t));
   LSIM_HIP(hipEventRecord(sl.ev[3], stream));
+  sl.replayed = g != nullptr;
   sl.busy = true;
   sl.jobs = act;
   ++ds_next;
@@ -4535,6 +4646,7 @@ void Engine::Impl::ds_resolve(DsSlot &sl) {
     r.sweep_tiles += st[2];
     r.sweep_bytes += st[0];
     r.sweep_reruns += st[3];
+    r.steps_replayed += sl.replayed ? 1 : 0;
     const int64_t nf = st[kDsStat + 2 * i], nu = st[kDsStat + 2 * i + 1];
     if (!j.direct) r.stems += nu;
     j.dev_alive = nf;

@@ -87,6 +87,7 @@ struct EngineOptions {
   double slot_share = 1.0;           // pair sweep: use at most this share of the resident workgroup slots
   bool autotune = false;             // run(): pick sweep kernel and word schedule by timed probes (cached)
   bool device_steps = true;          // persistent pair sweep: plan each word step on the device (no host round trip)
+  bool step_graphs = true;           // device-planned steps: capture recurring steps' launches in graphs and replay them
 };
 
 struct RunResults {

@@ -233,10 +233,14 @@ def test_server_two_lanes_match_single_runs():
 
 def test_batched_good_machine_graph_replays():
     """The batched good machine is captured in a graph once a job set
-    recurs and replayed after that; four batches of the same set, with an
-    eager batch of another set on another engine between them, all give the
-    oracle's first detecting rounds, and the graph is replayed from the
-    second batch on."""
+    recurs and replayed after that, and so are the word steps' launches
+    (plan, sweep, detection) once a step's launch signature recurs; four
+    batches of the same set, with an eager batch of another set on another
+    engine between them, all give the oracle's first detecting rounds, the
+    good-machine graph is replayed from the second batch on, and every word
+    step of the third and fourth batches replays its captured graphs (the
+    step buffers grow during the first batch, so the second one's launch
+    signatures are partly new)."""
     b = lsim_amd.Batch(device=0)
     other = lsim_amd.Batch(device=0, good_graph=0)
     cids, texts, want = [], [], []
@@ -247,11 +251,13 @@ def test_batched_good_machine_graph_replays():
         o = oracle.Oracle(ckt, flst, ptn)
         want.append(o.first_rounds(np.arange(o.F)))
     ocid = other.add_circuit(texts[0][0])
-    replayed = []
+    replayed, steps = [], []
     for rnd in range(4):
         jobs = [b.add_job(c, f, p, drop=True) for c, (ck, f, p) in zip(cids, texts)]
         b.run()
-        replayed.append(b.last_run_stats()["good_replayed"])
+        st = b.last_run_stats()
+        replayed.append(st["good_replayed"])
+        steps.append((st["steps_replayed"], st["steps"]))
         for jid, w in zip(jobs, want):
             assert np.array_equal(b.first_detect_round(jid), w), (rnd, jid)
             b.release_job(jid)
@@ -264,6 +270,7 @@ def test_batched_good_machine_graph_replays():
         for j in oj:
             other.release_job(j)
     assert replayed == [0, 1, 1, 1]
+    assert steps[0][0] < steps[0][1] and all(r == n and n > 0 for r, n in steps[2:]), steps
 
 
 def test_server_failure_fails_only_its_job():
