@@ -120,7 +120,14 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            a window whose deepest cone outgrows a block is
  *                            cut shorter.
  *  "win_cap"            level windows: nodes per block, 0..4096 (default 0 = half
- *                            the CU's LDS at 544 B per node: 150 on gfx950). */
+ *                            the CU's LDS at 544 B per node: 150 on gfx950).
+ *  "device_steps"       0/1  pair sweep: each word step is planned by kernels
+ *                            (fault filtering, units, lanes, tiles) with no host
+ *                            round trip between steps (default 1).
+ *  "step_graphs"        0/1  device-planned steps: a step whose launch signature
+ *                            recurs replays its captured hipGraphs (default 1).
+ *  "slot_pct"           1..100: the pair sweep uses at most this percentage of
+ *                            the device's resident workgroup slots (default 100). */
 LSIM_API int lsim_set_option(lsim_sim *s, const char *key, int64_t value);
 
 /* Replaces Simulator::run(Mode, size_t) (fsim/simulation.cpp:65-92).
@@ -247,7 +254,8 @@ LSIM_API void lsim_batch_destroy(lsim_batch *b);
 LSIM_API const char *lsim_batch_last_error(const lsim_batch *b);
 /* Options as lsim_set_option (engine options: "scratch_mb", "block_waves", "sweep_kernel", "pair_threads",
  * "pair_dense", "pair_nopend", "pair_nopend_pct", "pair_fo_pre", "max_step_words", "pool_cap", "autotune", "good_graph",
- * "good_form", "win_levels", "win_cap") and "mode" (0 plain, 1 persistent). */
+ * "good_form", "win_levels", "win_cap", "device_steps", "step_graphs", "slot_pct") and "mode" (0 plain,
+ * 1 persistent). */
 LSIM_API int lsim_batch_set_option(lsim_batch *b, const char *key, int64_t value);
 /* Makes a circuit (.ckt text) resident; returns its id. */
 LSIM_API int lsim_batch_add_circuit(lsim_batch *b, const char *ckt, size_t ckt_len, int64_t *circuit_id);

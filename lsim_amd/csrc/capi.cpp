@@ -260,6 +260,9 @@ int set_engine_option(lsim::EngineOptions &eo, std::string *err, const std::stri
   } else if (k == "sweep_kernel") {
     if (value < 0 || value > 1) return fail(err, LSIM_E_ARG, "sweep_kernel must be 0 or 1");
     eo.sweep_kernel = int(value);
+  } else if (k == "slot_pct") {
+    if (value < 1 || value > 100) return fail(err, LSIM_E_ARG, "slot_pct must be 1..100");
+    eo.slot_share = double(value) / 100.0;
   } else {
     known = false;
   }

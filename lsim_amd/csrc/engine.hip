@@ -156,6 +156,10 @@ typedef int gi2 __attribute__((ext_vector_type(2)));
 typedef unsigned long long gu2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int4 to_int4(gi4 v) { return make_int4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ int2 to_int2(gi2 v) { return make_int2(v.x, v.y); }
+// A live-fanout entry is {position, level | pins << 24}: `pins` has bit k set
+// when the fanout reads the node on its input pin k.
+__device__ __forceinline__ int fo_level(int2 f) { return f.y & 0xffffff; }
+__device__ __forceinline__ uint32_t fo_pins(int2 f) { return uint32_t(f.y) >> 24; }
 
 // First live positions of levels l and l + 1, read with a scalar load: the
 // table is read-only while a kernel runs, and a scalar load is waited for on
@@ -610,9 +614,9 @@ __device__ __forceinline__ void event_nodes(const SweepView &J, const NodeStage 
       for (int e = e0 + lane; e < e1; e += 64) {
         const int2 v = to_int2(J.fo[e]);
         pend[v.x] = epoch;
-        atomicAdd(&s_lvlcnt[v.y], 1);
+        atomicAdd(&s_lvlcnt[fo_level(v)], 1);
       }
-      if (e1 > e0 && lane == 0) atomicMax(s_maxl, to_int2(J.fo[e1 - 1]).y);
+      if (e1 > e0 && lane == 0) atomicMax(s_maxl, fo_level(to_int2(J.fo[e1 - 1])));
       bytes += 8 * uint64_t(__popcll(nm)) + 16 + 12 * uint64_t(e1 - e0);
     }
     if (type[k] == G_PO) acc |= d;
@@ -754,10 +758,23 @@ constexpr int kTileStats = 13;  // sweep statistics counters (LSIM_TRACE)
 constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
 constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
+constexpr int kDenseNodesOcc = 2;   // the same at more than four waves per SIMD
+constexpr int kPairsPerThreadOcc = 1;
 
 // Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Marks the fanout of live-fanout entry f pending on the pins that read the
+// written node: a byte store of the tile's tag per pin (pins != 0).
+__device__ __forceinline__ void mark_pins(uint32_t *pend, int2 f, uint32_t tag) {
+  LSIM_GLOBAL uint8_t *b = (LSIM_GLOBAL uint8_t *)(pend + f.x);
+  uint32_t pins = fo_pins(f);
+  do {
+    b[__builtin_ctz(pins)] = uint8_t(tag);
+    pins &= pins - 1u;
+  } while (pins);
 }
 
 // Position of the r-th set bit (r < popcount) of m.
@@ -875,15 +892,24 @@ struct PairArgs {
   const int *ntiles_p;
 };
 
-template <bool kStats, int CHUNK>
-__global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4))) k_sweep_pairs(PairArgs a) {
+// NIN is the most fanins of any gate of the launch's circuits (2, or 4 as the
+// format allows): circuits of 1- and 2-input gates stage half the fanin data,
+// which leaves LDS and registers for more tiles in flight per CU (WAVES is
+// the waves per SIMD the kernel is compiled for).
+template <bool kStats, int CHUNK, int NIN, int WAVES>
+__global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVES))) k_sweep_pairs(PairArgs a) {
+  static_assert(NIN == 2 || NIN == 4, "fanin slots per node: 2 or 4");
+  // Nodes per wave in flight in dense passes and pairs per thread in
+  // flattened passes: fewer where more waves share a SIMD's registers.
+  constexpr int DN = WAVES > 4 ? kDenseNodesOcc : kDenseNodes;
+  constexpr int PPT = WAVES > 4 ? kPairsPerThreadOcc : kPairsPerThread;
   __shared__ unsigned long long s_acc[64], s_lmask[64];
   __shared__ int s_lsite[64], s_lword[64], s_lkind[64];
   __shared__ int s_u[CHUNK];
   __shared__ uint32_t s_info[CHUNK];
-  __shared__ int4 s_fin[CHUNK];
-  __shared__ unsigned long long s_m[CHUNK][4], s_fa[CHUNK][4];  // fanin differing / active lanes
-  __shared__ uint32_t s_fb[CHUNK][4];                            // fanin pool bases
+  __shared__ int s_fin[CHUNK][NIN];
+  __shared__ unsigned long long s_m[CHUNK][NIN], s_fa[CHUNK][NIN];  // fanin differing / active lanes
+  __shared__ uint32_t s_fb[CHUNK][NIN];                              // fanin pool bases
   __shared__ unsigned long long s_act[CHUNK], s_nm[CHUNK];
   __shared__ int s_pref[CHUNK];
   __shared__ int s_fo0[CHUNK], s_deg[CHUNK];  // live fanout range
@@ -926,6 +952,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
     const Tile T = a.tiles[t];
     const SweepView J = sweep_view(a.jobs, __builtin_amdgcn_readfirstlane(T.job));
     const uint32_t epoch = epoch_base + uint32_t(q) + 1u;
+    // Pending marks are per input pin: byte k of pend[v] holds the tag of the
+    // last tile whose walk wrote v's fanin on pin k (tags 1..255, so cleared
+    // memory never matches). An older tile's tag can match by chance; the
+    // fanin record's full epoch decides, so that costs one record read.
+    const uint32_t tag = epoch % 255u + 1u, tag4 = tag * 0x01010101u;
     const int64_t RW = J.RW;
     const auto good = J.good;
     uint32_t bytes = 0;  // per thread and tile
@@ -940,7 +971,9 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
       s_lmask[lane] = L.mask;
       s_acc[lane] = 0;
       if (valid) {
-        if (!nopend) pend[site] = epoch;
+        // An injection site is pending on every pin: all its fanins' records
+        // are read.
+        if (!nopend) pend[site] = tag4;
         s_lvl[L.level] = 1;
       }
     }
@@ -966,10 +999,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         ++n_pass;
         if (kStats) cy_t = clock64();
         // A. scan and stage. The node's static data (type, fanins, fanout
-        // range) is loaded together with its pending flag, so a pending node
-        // costs two dependent loads: these, then its fanins' records. With
-        // fo_pre its first live fanout is loaded beside the records, so that a
-        // written node with one live fanout marks it without a further load.
+        // range) is loaded together with its pending marks, so a pending node
+        // costs two dependent loads: these, then the records of the fanins on
+        // its marked pins. With fo_pre its first live fanout is loaded beside
+        // the records, so that a written node with one live fanout marks it
+        // without a further load.
         const bool pre = a.fo_pre && !nopend;
         const int u = c0 + tid;
         const bool in = u < hi;
@@ -980,12 +1014,24 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         const uint32_t inf = J.info[uc];
         const int4 f = to_int4(J.fin[uc]);
         const int fo0 = J.fo_off[uc], fo1 = J.fo_off[uc + 1];
-        // Without pending flags every node of the level is a candidate and is
-        // staged when some fanin differs in some lane or it injects.
-        const bool cand = in && (nopend || pf == epoch);
+        // Pins whose mark carries this tile's tag; without pending flags every
+        // node of the level is a candidate, all its fanins' records are read,
+        // and it is staged when some fanin differs in some lane or it injects.
+        uint32_t rd = 0xfu;
+        if (!nopend) {
+          const uint32_t x = pf ^ tag4;
+          rd = ((x & 0xffu) == 0 ? 1u : 0u) | ((x & 0xff00u) == 0 ? 2u : 0u) | ((x & 0xff0000u) == 0 ? 4u : 0u) |
+               ((x & 0xff000000u) == 0 ? 8u : 0u);
+        }
+        const bool cand = in && rd != 0;
         uint64_t act = 0;
-        uint64_t m[4] = {0, 0, 0, 0}, fa[4] = {0, 0, 0, 0};
-        uint32_t fb[4] = {0, 0, 0, 0};
+        uint64_t m[NIN], fa[NIN];
+        uint32_t fb[NIN];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) {
+          m[k] = fa[k] = 0;
+          fb[k] = 0;
+        }
         int2 fp = make_int2(0, 0);
         if (cand) {
           if (pre && fo1 > fo0) {
@@ -993,51 +1039,59 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             bytes += 8;
           }
           const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
-          // The fanins' whole records are loaded together, unconditionally
-          // for the first two (record 0 stands in for a missing fanin), and
-          // masked afterwards by fanin count and epoch, so a node's records
-          // cost one round trip. (Under a per-fanin or per-epoch branch the
+          // The records of the fanins on the read pins are loaded together,
+          // unconditionally for the first two pins (record 0 stands in for a
+          // missing or unread fanin: one line for the whole wave), and masked
+          // afterwards by fanin count, pin and epoch, so a node's records cost
+          // one round trip. (Under a per-fanin or per-epoch branch the
           // compiler loads them one fanin after another.)
           const int nf = type != G_PI ? int(nin) : 0;
           const int fi[4] = {f.x, f.y, f.z, f.w};
-          gu2 ma[4];
-          gi2 eb[4];
+          gu2 ma[NIN];
+          gi2 eb[NIN];
 #pragma unroll
           for (int k = 0; k < 2; ++k) {
-            const PairRec *rp = rec + (k < nf ? fi[k] : 0);
+            const PairRec *rp = rec + (k < nf && ((rd >> k) & 1u) ? fi[k] : 0);
             ma[k] = *reinterpret_cast<const gu2 *>(rp);
             eb[k] = *reinterpret_cast<const gi2 *>(&rp->epoch);
           }
-          ma[2] = ma[3] = gu2{0, 0};
-          eb[2] = eb[3] = gi2{0, 0};
-          if (nf > 2) {
+          if constexpr (NIN > 2) {
+            ma[2] = ma[3] = gu2{0, 0};
+            eb[2] = eb[3] = gi2{0, 0};
+            if (nf > 2) {
 #pragma unroll
-            for (int k = 2; k < 4; ++k) {
-              const PairRec *rp = rec + (k < nf ? fi[k] : 0);
-              ma[k] = *reinterpret_cast<const gu2 *>(rp);
-              eb[k] = *reinterpret_cast<const gi2 *>(&rp->epoch);
+              for (int k = 2; k < 4; ++k) {
+                const PairRec *rp = rec + (k < nf && ((rd >> k) & 1u) ? fi[k] : 0);
+                ma[k] = *reinterpret_cast<const gu2 *>(rp);
+                eb[k] = *reinterpret_cast<const gi2 *>(&rp->epoch);
+              }
             }
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const uint64_t keep = (k < nf && uint32_t(eb[k].x) == epoch) ? ~uint64_t(0) : uint64_t(0);
+          for (int k = 0; k < NIN; ++k) {
+            const uint64_t keep =
+                (k < nf && ((rd >> k) & 1u) && uint32_t(eb[k].x) == epoch) ? ~uint64_t(0) : uint64_t(0);
             m[k] = ma[k].x & keep;
             fa[k] = ma[k].y & keep;
             fb[k] = uint32_t(eb[k].y) & uint32_t(keep);
+            act |= m[k];
           }
-          act = m[0] | m[1] | m[2] | m[3] | inj_lanes(s_lsite, u);
-          bytes += 20 + (type != G_PI ? 24 * nin : 0);
+          act |= inj_lanes(s_lsite, u);
+          bytes += 20 + (type != G_PI ? 24 * uint32_t(__popc(rd & ((1u << nin) - 1u))) : 0);
         }
         if (!nopend) bytes += 4;
-        const bool p = cand && (!nopend || act != 0);
+        // A pending node always has a differing fanin or injects (act != 0);
+        // a mark matched only by an older tile's tag leaves act == 0.
+        const bool p = cand && act != 0;
         const uint64_t slot = wave_claim(p ? ((uint64_t(1) << 32) | uint64_t(__popcll(act))) : 0, lane, &s_cnt2);
         if (p) {
           const int k = int(slot >> 32);
           s_u[k] = u;
           s_info[k] = inf;
-          s_fin[k] = f;
+          const int fj[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < NIN; ++j) {
+            s_fin[k][j] = fj[j];
             s_m[k][j] = m[j];
             s_fa[k][j] = fa[j];
             s_fb[k][j] = fb[j];
@@ -1067,7 +1121,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
         }
         dense_pass = a.dense_lanes > 0 && int64_t(P) >= int64_t(a.dense_lanes) * n;
         if (dense_pass) {
-          // B'. Dense pass: one node per wave at a time, kDenseNodes in flight, the
+          // B'. Dense pass: one node per wave at a time, DN in flight, the
           // wave's lanes being the tile's lanes. A lane's pool word within a
           // node is the count of the node's active lanes below it (mbcnt), so
           // no pair has to be located by search.
@@ -1099,13 +1153,13 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             if (lane == 0) s_nm[k] = nm;
           };
 #pragma unroll 1
-          for (int k0 = wave; k0 < n; k0 += kDenseNodes * (CHUNK / 64)) {
-            int kk[kDenseNodes], v[kDenseNodes];
-            bool has[kDenseNodes];
-            uint32_t type[kDenseNodes], nin[kDenseNodes];
+          for (int k0 = wave; k0 < n; k0 += DN * (CHUNK / 64)) {
+            int kk[DN], v[DN];
+            bool has[DN];
+            uint32_t type[DN], nin[DN];
             bool wide = false;
 #pragma unroll
-            for (int i = 0; i < kDenseNodes; ++i) {
+            for (int i = 0; i < DN; ++i) {
               const int ki = k0 + i * (CHUNK / 64);
               has[i] = ki < n;
               kk[i] = has[i] ? ki : k0;
@@ -1113,20 +1167,19 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               const uint32_t vi = __builtin_amdgcn_readfirstlane(s_info[kk[i]]);
               type[i] = vi & 15u;
               nin[i] = (vi >> 4) & 7u;
-              if (has[i] && nin[i] > 2) wide = true;
+              if (NIN > 2 && has[i] && nin[i] > 2) wide = true;
             }
             if (!wide) {
               // Gates of at most two inputs (the group's choice is uniform):
               // every load of the group's nodes first, then the evaluations.
-              bool on[kDenseNodes];
-              uint64_t gu[kDenseNodes], vals[kDenseNodes][2];
+              bool on[DN];
+              uint64_t gu[DN], vals[DN][2];
 #pragma unroll
-              for (int i = 0; i < kDenseNodes; ++i) {
+              for (int i = 0; i < DN; ++i) {
                 on[i] = has[i] && ((s_act[kk[i]] >> lane) & 1);
                 gu[i] = 0;
                 if (on[i]) gu[i] = good[int64_t(v[i]) * RW + w_l];
-                const int2 fv = *reinterpret_cast<const int2 *>(&s_fin[kk[i]]);
-                const int fi[2] = {fv.x, fv.y};
+                const int fi[2] = {s_fin[kk[i]][0], s_fin[kk[i]][1]};
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                   // Both addresses first, then one load from the selected one
@@ -1139,17 +1192,17 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
                 }
               }
 #pragma unroll
-              for (int i = 0; i < kDenseNodes; ++i) {
+              for (int i = 0; i < DN; ++i) {
                 if (!has[i]) continue;
                 const uint64_t r = (on[i] && type[i] != G_PI)
                                        ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], 0, 0)
                                        : gu[i];
                 dense_out(kk[i], v[i], type[i], nin[i], on[i], gu[i], r);
               }
-            } else {
+            } else if constexpr (NIN > 2) {
               // The group holds a gate of more than two inputs: its nodes go
               // one at a time, with four fanin slots.
-              const int k_end = min(n, k0 + kDenseNodes * (CHUNK / 64));
+              const int k_end = min(n, k0 + DN * (CHUNK / 64));
 #pragma unroll 1
               for (int k1 = k0; k1 < k_end; k1 += CHUNK / 64) {
                 const int v1 = __builtin_amdgcn_readfirstlane(s_u[k1]);
@@ -1158,8 +1211,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
                 const bool on1 = (s_act[k1] >> lane) & 1;
                 uint64_t gu1 = 0;
                 if (on1) gu1 = good[int64_t(v1) * RW + w_l];
-                const int4 fv = s_fin[k1];
-                const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
+                const int fi[4] = {s_fin[k1][0], s_fin[k1][1], s_fin[k1][2], s_fin[k1][3]};
                 uint64_t v4[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -1175,19 +1227,19 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         } else {
-          // B. (node, active lane) pairs, kPairsPerThread per thread at a time
+          // B. (node, active lane) pairs, PPT per thread at a time
           // with their loads interleaved: the pairs' good words and fanin
           // words are all loaded before any is used. Every pair writes its
           // pool word (readers use only the lanes that differ): the pass's
           // words then fill one contiguous block, written in whole lines.
   #pragma unroll 1
-          for (int q0 = tid; q0 < P; q0 += kPairsPerThread * CHUNK) {
-            int k[kPairsPerThread], ln[kPairsPerThread], rk[kPairsPerThread];
-            uint32_t vi[kPairsPerThread];
-            uint64_t gu[kPairsPerThread], vals[kPairsPerThread][4];
-            bool ok[kPairsPerThread];
+          for (int q0 = tid; q0 < P; q0 += PPT * CHUNK) {
+            int k[PPT], ln[PPT], rk[PPT];
+            uint32_t vi[PPT];
+            uint64_t gu[PPT], vals[PPT][NIN];
+            bool ok[PPT];
   #pragma unroll
-            for (int i = 0; i < kPairsPerThread; ++i) {
+            for (int i = 0; i < PPT; ++i) {
               const int qq = q0 + i * CHUNK;
               ok[i] = qq < P;
               const int qe = ok[i] ? qq : q0;
@@ -1196,31 +1248,35 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
               ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
               vi[i] = s_info[k[i]];
               const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
-              const int4 fv = s_fin[k[i]];
               const int w = s_lword[ln[i]];
               const int b = ln[i];
               const uint64_t below = (uint64_t(1) << b) - 1;
-              const int fi[4] = {fv.x, fv.y, fv.z, fv.w};
               gu[i] = 0;
               if (ok[i]) gu[i] = good[int64_t(s_u[k[i]]) * RW + w];
   #pragma unroll
-              for (int j = 0; j < 4; ++j) {
+              for (int j = 0; j < NIN; ++j) {
                 // Both addresses first, then one load from the selected one
                 // (a load in each arm of a branch makes the compiler wait).
                 const auto pa = &pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))];
-                const auto ga = &good[int64_t(fi[j]) * RW + w];
+                const auto ga = &good[int64_t(s_fin[k[i]][j]) * RW + w];
                 const auto src = ((s_m[k[i]][j] >> b) & 1) ? pa : ga;
                 vals[i][j] = 0;
                 if (ok[i] && type != G_PI && j < int(nin)) vals[i][j] = *src;
               }
             }
   #pragma unroll
-            for (int i = 0; i < kPairsPerThread; ++i) {
+            for (int i = 0; i < PPT; ++i) {
               if (!ok[i]) continue;
               const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
               const int b = ln[i];
               const int v = s_u[k[i]];
-              uint64_t r = type != G_PI ? eval_gate(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]) : gu[i];
+              uint64_t r = gu[i];
+              if (type != G_PI) {
+                if constexpr (NIN > 2)
+                  r = eval_gate(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]);
+                else
+                  r = eval_gate(type, nin, vals[i][0], vals[i][1], 0, 0);
+              }
               if (s_lsite[b] == v) {
                 const uint32_t kind = uint32_t(s_lkind[b]);
                 r = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
@@ -1268,8 +1324,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
             if (kStats) ++n_written;
             if (pre && deg) {
               const int2 fv = s_fop[tid];
-              pend[fv.x] = epoch;
-              s_lvl[fv.y] = 1;
+              mark_pins(pend, fv, tag);
+              s_lvl[fo_level(fv)] = 1;
               bytes += 4;
               ++e0;
               --deg;
@@ -1307,8 +1363,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
           for (int i = 0; i < kFanoutsPerLane; ++i)
             if (ok[i]) {
-              pend[fo[i].x] = epoch;
-              s_lvl[fo[i].y] = 1;
+              mark_pins(pend, fo[i], tag);
+              s_lvl[fo_level(fo[i])] = 1;
               bytes += 12;
             }
         }
@@ -1374,9 +1430,19 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(4)))
 using PairKernel = void (*)(PairArgs);
 
 // The pair sweep instance for a workgroup of `threads` (512 or 256).
-PairKernel pairs_kernel(int threads, bool stats) {
-  if (threads == 256) return stats ? k_sweep_pairs<true, 256> : k_sweep_pairs<false, 256>;
-  return stats ? k_sweep_pairs<true, 512> : k_sweep_pairs<false, 512>;
+// The pair sweep for a workgroup size, with statistics or not, for circuits
+// whose gates all have at most two fanins (narrow) or any circuit.
+// The 256-thread form of narrow circuits runs five waves per SIMD (five
+// workgroups per CU where four fit otherwise). A 384-thread form at five waves
+// per SIMD (three workgroups per CU) was slower than the 512-thread one on the
+// 2M-gate workload: 576 against 503 ms of sweep per run.
+PairKernel pairs_kernel(int form, bool stats, bool narrow) {
+  if (narrow) {
+    if (form == 256) return stats ? k_sweep_pairs<true, 256, 2, 5> : k_sweep_pairs<false, 256, 2, 5>;
+    return stats ? k_sweep_pairs<true, 512, 2, 4> : k_sweep_pairs<false, 512, 2, 4>;
+  }
+  if (form == 256) return stats ? k_sweep_pairs<true, 256, 4, 4> : k_sweep_pairs<false, 256, 4, 4>;
+  return stats ? k_sweep_pairs<true, 512, 4, 4> : k_sweep_pairs<false, 512, 4, 4>;
 }
 
 // ---------------------------------------------------------------- detection
@@ -2467,6 +2533,7 @@ struct CircuitRes {
   std::unique_ptr<WindowPlan> win;
   std::vector<std::unique_ptr<GoodGraph>> graphs;  // most recent last, at most kMaxGraphs
   bool wide = false;  // mean live nodes per level above 256 (the pair sweep then uses 512-thread groups)
+  bool narrow = false;  // no gate has more than two fanins (the pair sweep then stages two per node)
   bool has_mux = false;
   std::vector<int32_t> pos2dp, dp2pos, level_dp, live_lvl, dead_lvl;
   std::vector<int32_t> head_dp;     // per position: region head (device position), -1 if dead
@@ -2889,9 +2956,9 @@ struct Engine::Impl {
   void step_batch(const std::vector<JobState *> &act, const EngineOptions &eo);
   void sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles, const std::vector<Tile> *tiles,
              int lvl_entries, int64_t nl_max, const CircuitRes *plain_circuit, bool dense, bool nopend,
-             lsim_run_stats &st);
+             bool narrow, lsim_run_stats &st);
   void sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense, bool nopend,
-                   unsigned long long *tstats, lsim_run_stats &st);
+                   bool narrow, unsigned long long *tstats, lsim_run_stats &st);
   void ensure_slots(const EngineOptions &eo, int64_t want, int64_t nl_max, int per_cu);
   void ensure_pair_scratch(const EngineOptions &eo, int64_t nl_max, int64_t slots_needed, int64_t full_slots);
   void release_slots();
@@ -3062,6 +3129,8 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   const int64_t nl1 = std::max<int64_t>(c.NL, 1);
   std::vector<uint32_t> info(N);
   std::vector<int4> fin(N);
+  c.narrow = true;
+  for (int64_t p = 0; p < N; ++p) c.narrow = c.narrow && nl.fin_off[p + 1] - nl.fin_off[p] <= 2;
   std::vector<int32_t> parent(nl1, -1);
   for (int64_t p = 0; p < N; ++p) {
     const int32_t u = c.pos2dp[p];
@@ -3073,7 +3142,10 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
     fin[u] = make_int4(v[0], v[1], v[2], v[3]);
     if (fs.reach_po[p] && fs.parent[p] >= 0) parent[u] = c.pos2dp[fs.parent[p]];
   }
-  // Live fanouts of live nodes, ascending position, with the fanout's level.
+  // Live fanouts of live nodes, ascending position, each with the fanout's
+  // level in bits 0..23 and, in bits 24..27, the input pins of the fanout
+  // that read the node (a node feeding two pins of one gate is listed once).
+  if (c.L >= (1 << 24)) throw DeviceError(LSIM_E_INPUT, "circuit too deep (levels must fit in 24 bits)");
   std::vector<int32_t> fo_off(nl1 + 1, 0);
   std::vector<int2> fo;
   std::vector<int32_t> outs;
@@ -3083,14 +3155,21 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
     for (int32_t e = fs.fo_off[p]; e < fs.fo_off[p + 1]; ++e)
       if (fs.reach_po[fs.fo[e]]) outs.push_back(c.pos2dp[fs.fo[e]]);
     std::sort(outs.begin(), outs.end());
-    for (int32_t v : outs) fo.push_back(make_int2(v, c.level_dp[v]));
+    for (int32_t v : outs) {
+      const int4 f = fin[v];
+      const int fv[4] = {f.x, f.y, f.z, f.w};
+      const int nin = int((info[v] >> 4) & 7u);
+      uint32_t pins = 0;
+      for (int k = 0; k < nin; ++k) pins |= fv[k] == int(u) ? (1u << k) : 0u;
+      if (!pins) throw DeviceError(LSIM_E_DEVICE, "internal: a live fanout does not read the node");
+      fo.push_back(make_int2(v, int32_t(uint32_t(c.level_dp[v]) | (pins << 24))));
+    }
     fo_off[u + 1] = int32_t(fo.size());
   }
   // info bits 8..31: level of the node's deepest live fanout (the last one,
   // as positions ascend with level), which bounds how far a sweep must walk.
-  if (c.L >= (1 << 24)) throw DeviceError(LSIM_E_INPUT, "circuit too deep (levels must fit in 24 bits)");
   for (int64_t u = 0; u < c.NL; ++u)
-    if (fo_off[u + 1] > fo_off[u]) info[u] |= uint32_t(fo[fo_off[u + 1] - 1].y) << 8;
+    if (fo_off[u + 1] > fo_off[u]) info[u] |= uint32_t(c.level_dp[fo[fo_off[u + 1] - 1].x]) << 8;
   if (fo.empty()) fo.push_back(make_int2(0, 0));
   c.head_is_po.assign(nl1, 0);
   for (int64_t p = 0; p < N; ++p)
@@ -3136,6 +3215,7 @@ CircuitRes *Engine::share_circuit(const CircuitRes *src) {
   c.h_info = src->h_info;
   c.h_fin = src->h_fin;
   c.wide = src->wide;
+  c.narrow = src->narrow;
   c.has_mux = src->has_mux;
   c.pos2dp = src->pos2dp;
   c.dp2pos = src->dp2pos;
@@ -4070,7 +4150,8 @@ void Engine::Impl::ensure_slots(const EngineOptions &eo, int64_t want, int64_t n
 // (`lanes` and `tiles` not null) and copied there first.
 void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nlanes, int64_t ntiles,
                          const std::vector<Tile> *tiles, int lvl_entries, int64_t nl_max,
-                         const CircuitRes *plain_circuit, bool dense, bool nopend, lsim_run_stats &st) {
+                         const CircuitRes *plain_circuit, bool dense, bool nopend, bool narrow,
+                         lsim_run_stats &st) {
   if (!ntiles) return;
   if (lanes && tiles) {
     d_lanes.alloc(size_t(nlanes));
@@ -4108,7 +4189,7 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
   if (!plain_circuit && eo.sweep_kernel == 0) {
     st.sweep_tiles += ntiles;
     time_begin();
-    sweep_pairs(eo, ntiles, lvl_entries, nl_max, dense, nopend, trace ? d_tstats.p : nullptr, st);
+    sweep_pairs(eo, ntiles, lvl_entries, nl_max, dense, nopend, narrow, trace ? d_tstats.p : nullptr, st);
     st.t_sweep_ms += time_end();
     if (trace) print_stats();
     return;
@@ -4184,11 +4265,11 @@ void Engine::Impl::sweep(const EngineOptions &eo, const Lane *lanes, int64_t nla
 }
 
 void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_entries, int64_t nl_max, bool dense,
-                               bool nopend, unsigned long long *tstats, lsim_run_stats &st) {
+                               bool nopend, bool narrow, unsigned long long *tstats, lsim_run_stats &st) {
   const size_t lds = size_t(lvl_entries);  // one flag byte per level
   int pc256 = 0, pc512 = 0;
-  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false), 256, lds));
-  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false), 512, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false, narrow), 256, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false, narrow), 512, lds));
   if (pc512 < 1)
     throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
   // Sized for the largest resident circuit, so later jobs on it reuse the
@@ -4253,7 +4334,8 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   epoch_base += uint32_t(ntiles);
   LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
   LSIM_HIP(hipMemsetAsync(d_ovf_count.p, 0, sizeof(int), stream));
-  hipLaunchKernelGGL(pairs_kernel(threads, tstats != nullptr), dim3(unsigned(grid)), dim3(threads), lds, stream, a);
+  hipLaunchKernelGGL(pairs_kernel(threads, tstats != nullptr, narrow), dim3(unsigned(grid)), dim3(threads), lds,
+                     stream, a);
   LSIM_HIP(hipGetLastError());
   st.sweep_launches++;
   finalize_end();  // finished jobs' results unpacked while the sweep runs
@@ -4279,7 +4361,7 @@ void Engine::Impl::sweep_pairs(const EngineOptions &eo, int64_t ntiles, int lvl_
   epoch_base += uint32_t(count);
   LSIM_HIP(hipMemsetAsync(d_queue.p, 0, sizeof(int), stream));
   LSIM_HIP(hipMemsetAsync(d_ovf_count.p, 0, sizeof(int), stream));
-  hipLaunchKernelGGL(pairs_kernel(512, tstats != nullptr), dim3(unsigned(g2)), dim3(512), lds, stream, a);
+  hipLaunchKernelGGL(pairs_kernel(512, tstats != nullptr, narrow), dim3(unsigned(g2)), dim3(512), lds, stream, a);
   LSIM_HIP(hipGetLastError());
   st.sweep_launches++;
   if (trace_on())
@@ -4314,10 +4396,11 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   sl.h_epoch.alloc(1);
   int64_t F = 0, W = 0, T = 0, I = 0, nl_max = 1;
   int L = 0;
-  bool dense = false;
+  bool dense = false, narrow = true;
   for (int i = 0; i < nj; ++i) {
     JobState &j = *act[i];
     const CircuitRes &c = *j.c;
+    narrow = narrow && c.narrow;
     const int64_t L1 = std::max<int64_t>(int64_t(j.live.size()), 1);
     const int64_t U1 = j.direct ? L1 : std::max<int64_t>(1, std::min<int64_t>(L1, j.stems_cap));
     JobDev d = job_dev(j);
@@ -4372,8 +4455,8 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   const int lvl_entries = L + 1;
   const size_t lds = size_t(lvl_entries);
   int pc256 = 0, pc512 = 0;
-  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false), 256, lds));
-  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false), 512, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc256, pairs_kernel(256, false, narrow), 256, lds));
+  LSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc512, pairs_kernel(512, false, narrow), 512, lds));
   if (pc512 < 1)
     throw DeviceError(LSIM_E_DEVICE, "sweep kernel cannot be resident (" + std::to_string(lds) + " B of level counters)");
   int64_t nl_all = nl_max;
@@ -4542,7 +4625,8 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   };
   auto seg_sweep = [&] {
     for (const SweepLaunch &l : launches)
-      hipLaunchKernelGGL(pairs_kernel(l.form, false), dim3(unsigned(l.grid)), dim3(l.form), lds, stream, l.args);
+      hipLaunchKernelGGL(pairs_kernel(l.form, false, narrow), dim3(unsigned(l.grid)), dim3(l.form), lds, stream,
+                         l.args);
   };
   auto seg_detect = [&] {
     k_detect_gs<<<gI, 256, 0, stream>>>(ia);
@@ -4551,7 +4635,8 @@ void Engine::Impl::step_device(const std::vector<JobState *> &act, const EngineO
   };
   // The launch signature: every argument the host gives the step's kernels
   // (the rest is read from device memory, uploaded above).
-  std::vector<uint64_t> sig = {uint64_t(nj), uint64_t(F), uint64_t(gF), uint64_t(gI), uint64_t(lvl_entries)};
+  std::vector<uint64_t> sig = {uint64_t(nj), uint64_t(F), uint64_t(gF), uint64_t(gI), uint64_t(lvl_entries),
+                               uint64_t(narrow)};
   auto add_bytes = [&](const void *p, size_t n) {
     const uint8_t *b = static_cast<const uint8_t *>(p);
     for (size_t o = 0; o < n; o += 8) {
@@ -4670,6 +4755,8 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
   using clk = std::chrono::steady_clock;
   const bool trace = trace_on();
   const auto h0 = clk::now();
+  bool narrow = true;  // every circuit of the step has gates of at most two fanins
+  for (const JobState *j : act) narrow = narrow && j->c->narrow;
   std::vector<JobDev> devs(nj);
   std::vector<int64_t> item_off(nj + 1, 0);
   std::vector<int64_t> word0(nj + 1, 0);
@@ -4877,7 +4964,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     // more than pair_nopend_pct percent of the nodes it walked.
     const bool nopend =
         eo.pair_nopend == 2 || (eo.pair_nopend == 1 && (dense || 100.0 * staged_share > double(eo.pair_nopend_pct)));
-    sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, nopend, st);
+    sweep(eo, nullptr, nlanes, ntiles, nullptr, lvl_entries, nl_max, nullptr, big_groups, nopend, narrow, st);
   } else {
     // The needed bits decide which (unit, word) pairs become sweep lanes.
     h_need.alloc(size_t(words));
@@ -4976,7 +5063,7 @@ void Engine::Impl::step_batch(const std::vector<JobState *> &act, const EngineOp
     for (const Tile &t : tiles) wide_tiles += act[t.job]->c->wide ? 1 : 0;
     ntiles = int64_t(tiles.size());
     const bool big_groups = dense || 2 * wide_tiles > ntiles;
-    sweep(eo, lanes, nlanes, ntiles, &tiles, lvl_entries, nl_max, act[0]->c, big_groups, false, st);
+    sweep(eo, lanes, nlanes, ntiles, &tiles, lvl_entries, nl_max, act[0]->c, big_groups, false, narrow, st);
   }
   const auto h4 = clk::now();
   time_begin();

@@ -20,16 +20,18 @@ pytestmark = pytest.mark.gpu
 # (pairs_dense). Dense steps walk every node of a level instead of pending
 # flags: by default, and forced never (pairs_pend) or in every step
 # (pairs_nopend). With pending flags a staged node's first fanout is loaded
-# while it is staged: by default, and not (pairs_pend_nopre).
+# while it is staged: by default, and not (pairs_pend_nopre). Word steps are
+# planned on the device by default, and on the host (pairs_host).
 MODES = [(lsim_amd.Simulator.Mode.PLAIN, {}), (lsim_amd.Simulator.Mode.PARTITIONED, {}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"sweep_kernel": 1}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 0}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_dense": 1}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 0}),
          (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 2}),
-         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 0, "pair_fo_pre": 0})]
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"pair_nopend": 0, "pair_fo_pre": 0}),
+         (lsim_amd.Simulator.Mode.PARTITIONED, {"device_steps": 0})]
 MODE_IDS = ["plain", "pairs", "nodewave", "pairs_flat", "pairs_dense", "pairs_pend", "pairs_nopend",
-            "pairs_pend_nopre"]
+            "pairs_pend_nopre", "pairs_host"]
 
 
 def _sim(ckt, flst, ptn, mode=None, **opts):
