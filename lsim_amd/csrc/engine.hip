@@ -901,8 +901,11 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
   static_assert(NIN == 2 || NIN == 4, "fanin slots per node: 2 or 4");
   // Nodes per wave in flight in dense passes and pairs per thread in
   // flattened passes: fewer where more waves share a SIMD's registers.
-  constexpr int DN = WAVES > 4 ? kDenseNodesOcc : kDenseNodes;
+  constexpr int DN = WAVES > 4 ? kDenseNodesOcc : (NIN == 2 ? 4 : kDenseNodes);
   constexpr int PPT = WAVES > 4 ? kPairsPerThreadOcc : kPairsPerThread;
+  // Loads of pair passes without a branch around them (measured faster for
+  // the 512-thread narrow form, slower for the five-wave 256-thread one).
+  constexpr bool kUncond = NIN == 2 && WAVES == 4;
   __shared__ unsigned long long s_acc[64], s_lmask[64];
   __shared__ int s_lsite[64], s_lword[64], s_lkind[64];
   __shared__ int s_u[CHUNK];
@@ -958,6 +961,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
     // fanin record's full epoch decides, so that costs one record read.
     const uint32_t tag = epoch % 255u + 1u, tag4 = tag * 0x01010101u;
     const int64_t RW = J.RW;
+    const uint32_t RW32 = uint32_t(RW);  // words per node (< 2^32): 32 x 32-bit products
     const auto good = J.good;
     uint32_t bytes = 0;  // per thread and tile
     uint32_t n_lv = 0, n_pass = 0, n_staged = 0, n_written = 0, n_pairs = 0;
@@ -1172,23 +1176,27 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
             if (!wide) {
               // Gates of at most two inputs (the group's choice is uniform):
               // every load of the group's nodes first, then the evaluations.
+              // Every load is issued unconditionally, with no branch around it:
+              // an inactive lane, a PI (its fanin slot holds the PI index) or
+              // a missing fanin (slot 0) reads a valid good word and ignores
+              // it, so the group's loads share one wait.
               bool on[DN];
               uint64_t gu[DN], vals[DN][2];
 #pragma unroll
               for (int i = 0; i < DN; ++i) {
                 on[i] = has[i] && ((s_act[kk[i]] >> lane) & 1);
                 gu[i] = 0;
-                if (on[i]) gu[i] = good[int64_t(v[i]) * RW + w_l];
-                const int fi[2] = {s_fin[kk[i]][0], s_fin[kk[i]][1]};
+                if (kUncond || on[i]) gu[i] = good[uint64_t(uint32_t(v[i])) * RW32 + w_l];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                   // Both addresses first, then one load from the selected one
                   // (a load in each arm of a branch makes the compiler wait).
+                  const uint32_t fj = uint32_t(__builtin_amdgcn_readfirstlane(s_fin[kk[i]][j]));
                   const auto pa = &pool[s_fb[kk[i]][j] + lanes_below(s_fa[kk[i]][j])];
-                  const auto ga = &good[int64_t(__builtin_amdgcn_readfirstlane(fi[j])) * RW + w_l];
+                  const auto ga = &good[uint64_t(fj) * RW32 + w_l];
                   const auto src = ((s_m[kk[i]][j] >> lane) & 1) ? pa : ga;
                   vals[i][j] = 0;
-                  if (on[i] && type[i] != G_PI && j < int(nin[i])) vals[i][j] = *src;
+                  if (kUncond || (on[i] && type[i] != G_PI && j < int(nin[i]))) vals[i][j] = *src;
                 }
               }
 #pragma unroll
@@ -1247,21 +1255,25 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
               rk[i] = qe - s_pref[k[i]];
               ln[i] = nth_set_bit(s_act[k[i]], rk[i]);
               vi[i] = s_info[k[i]];
-              const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
               const int w = s_lword[ln[i]];
               const int b = ln[i];
               const uint64_t below = (uint64_t(1) << b) - 1;
+              // With kUncond the loads have no branch around them: a pair past
+              // the end repeats pair q0, and PIs and missing fanins read a
+              // valid word and ignore it.
+              const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
+              const bool use = kUncond || ok[i];
               gu[i] = 0;
-              if (ok[i]) gu[i] = good[int64_t(s_u[k[i]]) * RW + w];
+              if (use) gu[i] = good[uint64_t(uint32_t(s_u[k[i]])) * RW32 + uint32_t(w)];
   #pragma unroll
               for (int j = 0; j < NIN; ++j) {
                 // Both addresses first, then one load from the selected one
                 // (a load in each arm of a branch makes the compiler wait).
                 const auto pa = &pool[s_fb[k[i]][j] + uint32_t(__popcll(s_fa[k[i]][j] & below))];
-                const auto ga = &good[int64_t(s_fin[k[i]][j]) * RW + w];
+                const auto ga = &good[uint64_t(uint32_t(s_fin[k[i]][j])) * RW32 + uint32_t(w)];
                 const auto src = ((s_m[k[i]][j] >> b) & 1) ? pa : ga;
                 vals[i][j] = 0;
-                if (ok[i] && type != G_PI && j < int(nin)) vals[i][j] = *src;
+                if (kUncond || (ok[i] && type != G_PI && j < int(nin))) vals[i][j] = *src;
               }
             }
   #pragma unroll
