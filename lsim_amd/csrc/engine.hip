@@ -234,6 +234,32 @@ __device__ __forceinline__ uint64_t eval_gate(uint32_t type, uint32_t nin, uint6
   }
 }
 
+// eval_gate without branches: the type only selects among results computed
+// for every family, so lanes holding different gate types (the pair sweep's
+// flattened passes) or a type under a lane-dependent condition cost selects,
+// not a divergent switch. Missing fanins take the fold's identity. Same
+// semantics as eval_gate; the 2-input form is for gates of at most two fanins.
+__device__ __forceinline__ uint64_t eval_gate_sel2(uint32_t type, uint32_t nin, uint64_t v0, uint64_t v1) {
+  const bool f_and = type == G_AND || type == G_NAND, f_or = type == G_OR || type == G_NOR;
+  const bool f_xor = type == G_XOR || type == G_XNOR;
+  const bool inv = type == G_INV || type == G_NAND || type == G_NOR || type == G_XNOR;
+  const uint64_t b = nin > 1 ? v1 : (f_and ? ~uint64_t(0) : uint64_t(0));
+  const uint64_t r = f_and ? (v0 & b) : f_or ? (v0 | b) : f_xor ? (v0 ^ b) : v0;
+  return inv ? ~r : r;
+}
+
+__device__ __forceinline__ uint64_t eval_gate_sel(uint32_t type, uint32_t nin, uint64_t v0, uint64_t v1, uint64_t v2,
+                                                  uint64_t v3) {
+  const bool f_and = type == G_AND || type == G_NAND, f_or = type == G_OR || type == G_NOR;
+  const bool f_xor = type == G_XOR || type == G_XNOR;
+  const bool inv = type == G_INV || type == G_NAND || type == G_NOR || type == G_XNOR;
+  const uint64_t id = f_and ? ~uint64_t(0) : uint64_t(0);
+  const uint64_t b1 = nin > 1 ? v1 : id, b2 = nin > 2 ? v2 : id, b3 = nin > 3 ? v3 : id;
+  uint64_t r = f_and ? (v0 & b1 & b2 & b3) : f_or ? (v0 | b1 | b2 | b3) : f_xor ? (v0 ^ b1 ^ b2 ^ b3) : v0;
+  if (type == G_MUX) r = (v2 & v1) | (v0 & (uint64_t)(v2 == 0));
+  return inv ? ~r : r;
+}
+
 __device__ __forceinline__ uint64_t round_mask(int64_t num_patterns, int64_t w) {
   const int64_t bits = (num_patterns / (64 * (w + 1))) ? 64 : (num_patterns % 64);
   return bits < 64 ? ((uint64_t(1) << bits) - 1) : ~uint64_t(0);
@@ -278,7 +304,8 @@ __global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const i
     const uint64_t v1 = nin > 1 ? good[int64_t(f.y) * RW + w] : 0;
     const uint64_t v2 = nin > 2 ? good[int64_t(f.z) * RW + w] : 0;
     const uint64_t v3 = nin > 3 ? good[int64_t(f.w) * RW + w] : 0;
-    r = eval_gate(type, nin, v0, v1, v2, v3);
+    // A wave spans several nodes when a job has fewer than 64 words per node.
+    r = eval_gate_sel(type, nin, v0, v1, v2, v3);
   }
   J.good[int64_t(u) * RW + w] = r;
 }
@@ -1202,9 +1229,10 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
 #pragma unroll
               for (int i = 0; i < DN; ++i) {
                 if (!has[i]) continue;
-                const uint64_t r = (on[i] && type[i] != G_PI)
-                                       ? eval_gate(type[i], nin[i], vals[i][0], vals[i][1], 0, 0)
-                                       : gu[i];
+                // The type is uniform: a switch on it outside the lane-dependent
+                // condition is a scalar branch.
+                const uint64_t e = eval_gate(type[i], nin[i], vals[i][0], vals[i][1], 0, 0);
+                const uint64_t r = (on[i] && type[i] != G_PI) ? e : gu[i];
                 dense_out(kk[i], v[i], type[i], nin[i], on[i], gu[i], r);
               }
             } else if constexpr (NIN > 2) {
@@ -1229,7 +1257,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
                   v4[j] = 0;
                   if (on1 && ty != G_PI && j < int(ni)) v4[j] = *src;
                 }
-                const uint64_t r = (on1 && ty != G_PI) ? eval_gate(ty, ni, v4[0], v4[1], v4[2], v4[3]) : gu1;
+                const uint64_t e = eval_gate(ty, ni, v4[0], v4[1], v4[2], v4[3]);  // uniform type
+                const uint64_t r = (on1 && ty != G_PI) ? e : gu1;
                 dense_out(k1, v1, ty, ni, on1, gu1, r);
               }
             }
@@ -1282,13 +1311,12 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
               const uint32_t type = vi[i] & 15u, nin = (vi[i] >> 4) & 7u;
               const int b = ln[i];
               const int v = s_u[k[i]];
-              uint64_t r = gu[i];
-              if (type != G_PI) {
-                if constexpr (NIN > 2)
-                  r = eval_gate(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]);
-                else
-                  r = eval_gate(type, nin, vals[i][0], vals[i][1], 0, 0);
-              }
+              uint64_t e;
+              if constexpr (NIN > 2)
+                e = eval_gate_sel(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]);
+              else
+                e = eval_gate_sel2(type, nin, vals[i][0], vals[i][1]);
+              uint64_t r = type != G_PI ? e : gu[i];
               if (s_lsite[b] == v) {
                 const uint32_t kind = uint32_t(s_lkind[b]);
                 r = kind == K_FLIP ? (gu[i] ^ s_lmask[b]) : (kind == K_SA1 ? ~uint64_t(0) : uint64_t(0));
