@@ -275,20 +275,40 @@ __device__ __forceinline__ int find_job(const int64_t *off, int njobs, int64_t t
   return lo;
 }
 
+// Runs fn(j) on every active lane of the wave with j the job of the lane's
+// item t, j uniform over the lanes of each call: one binary search (scalar
+// loads) per distinct job among the wave's items. Items ascend with the lane,
+// so a wave usually holds one job.
+template <class Fn>
+__device__ __forceinline__ void for_wave_jobs(const int64_t *off, int njobs, int64_t t, Fn &&fn) {
+  bool done = false;
+  for (;;) {
+    const uint64_t left = __ballot(!done);
+    if (!left) break;
+    const int src = int(__builtin_ctzll(left));
+    const int64_t ts = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(t >> 32), src))) << 32) |
+                               uint64_t(uint32_t(__builtin_amdgcn_readlane(int(t), src))));
+    const int j = __builtin_amdgcn_readfirstlane(find_job(off, njobs, ts));
+    if (!done && t < off[j + 1]) {
+      fn(j);
+      done = true;
+    }
+  }
+}
+
+// lt / d for a job whose items number `span`: 32-bit division when they fit.
+__device__ __forceinline__ int64_t div_items(int64_t lt, int64_t d, int64_t span) {
+  return span <= int64_t(UINT32_MAX) ? int64_t(uint32_t(lt) / uint32_t(d)) : lt / d;
+}
+
 // ---------------------------------------------------------------- good machine
 
 // Level `level` of every job: threads cover (job, node, word); word is the
 // fastest index so a wave reads a node's consecutive words in one coalesced
 // access. off = per-job item prefix for this level.
-__global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const int64_t *__restrict__ off,
-                             int level) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= off[njobs]) return;
-  const int j = find_job(off, njobs, t);
-  const JobDev &J = jobs[j];
+__device__ __forceinline__ void good_item(const JobDev &J, int level, int64_t lt, int64_t span) {
   const int64_t RW = J.RW;
-  const int64_t lt = t - off[j];
-  const int i = int(lt / RW);
+  const int i = int(div_items(lt, RW, span));
   const int64_t w = lt - int64_t(i) * RW;
   const int a0 = J.live_lvl[level], n0 = J.live_lvl[level + 1] - a0;
   const int u = i < n0 ? a0 + i : J.dead_lvl[level] + (i - n0);
@@ -308,6 +328,13 @@ __global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const i
     r = eval_gate_sel(type, nin, v0, v1, v2, v3);
   }
   J.good[int64_t(u) * RW + w] = r;
+}
+
+__global__ void k_good_level(const JobDev *__restrict__ jobs, int njobs, const int64_t *__restrict__ off,
+                             int level) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= off[njobs]) return;
+  for_wave_jobs(off, njobs, t, [&](int j) { good_item(jobs[j], level, t - off[j], off[j + 1] - off[j]); });
 }
 
 // Masked good values of one round, gathered per device position.
@@ -1532,11 +1559,8 @@ struct ItemArgs {
 // ends at a PO is decided here, otherwise its bits join its stem's need.
 // Direct injection: a word needs a lane iff the stuck value differs from the
 // site's good value somewhere in the full word.
-__device__ __forceinline__ void local_item(const ItemArgs &a, int64_t t) {
-  const int jb = find_job(a.item_off, a.njobs, t);
-  const JobDev &J = a.jobs[jb];
-  const int64_t lt = t - a.item_off[jb];
-  const int fi = int(lt / J.Wn);
+__device__ __forceinline__ void local_item(const ItemArgs &a, const JobDev &J, int64_t lt, int64_t span) {
+  const int fi = int(div_items(lt, J.Wn, span));
   const int j = int(lt - int64_t(fi) * J.Wn);
   const int f = J.flist[fi];
   const int64_t w = J.w0 + j;
@@ -1554,17 +1578,21 @@ __device__ __forceinline__ void local_item(const ItemArgs &a, int64_t t) {
   }
 }
 
+// Item t of the step (job lookup once per distinct job of the wave).
+__device__ __forceinline__ void local_at(const ItemArgs &a, int64_t t) {
+  for_wave_jobs(a.item_off, a.njobs, t, [&](int jb) {
+    local_item(a, a.jobs[jb], t - a.item_off[jb], a.item_off[jb + 1] - a.item_off[jb]);
+  });
+}
+
 __global__ void k_local(ItemArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t < a.item_off[a.njobs]) local_item(a, t);
+  if (t < a.item_off[a.njobs]) local_at(a, t);
 }
 
 // Phase 2: stem tracing d = L & D_stem; direct injection d = PO difference.
-__device__ __forceinline__ void detect_item(const ItemArgs &a, int64_t t) {
-  const int jb = find_job(a.item_off, a.njobs, t);
-  const JobDev &J = a.jobs[jb];
-  const int64_t lt = t - a.item_off[jb];
-  const int fi = int(lt / J.Wn);
+__device__ __forceinline__ void detect_item(const ItemArgs &a, const JobDev &J, int64_t lt, int64_t span) {
+  const int fi = int(div_items(lt, J.Wn, span));
   const int j = int(lt - int64_t(fi) * J.Wn);
   const int f = J.flist[fi];
   const int64_t w = J.w0 + j;
@@ -1579,20 +1607,26 @@ __device__ __forceinline__ void detect_item(const ItemArgs &a, int64_t t) {
   record(J, f, w, local_term(J, f, w) & D);
 }
 
+__device__ __forceinline__ void detect_at(const ItemArgs &a, int64_t t) {
+  for_wave_jobs(a.item_off, a.njobs, t, [&](int jb) {
+    detect_item(a, a.jobs[jb], t - a.item_off[jb], a.item_off[jb + 1] - a.item_off[jb]);
+  });
+}
+
 __global__ void k_detect(ItemArgs a) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t < a.item_off[a.njobs]) detect_item(a, t);
+  if (t < a.item_off[a.njobs]) detect_at(a, t);
 }
 
 // The same over a fixed grid: the item count is read from the device.
 __global__ void k_local_gs(ItemArgs a) {
   const int64_t n = a.item_off[a.njobs], stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) local_item(a, t);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) local_at(a, t);
 }
 
 __global__ void k_detect_gs(ItemArgs a) {
   const int64_t n = a.item_off[a.njobs], stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) detect_item(a, t);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride) detect_at(a, t);
 }
 
 // First detecting rounds of the step's faults, packed per dropping job at
