@@ -1528,13 +1528,16 @@ __device__ __forceinline__ uint64_t sens(const JobDev &J, int g, int x, int64_t 
   const uint32_t type = inf & 15u, nin = (inf >> 4) & 7u;
   const int4 f = J.fin[g];
   const auto good = J.good;
+  // The second fanin slot holds node 0 on a one-input gate: loaded anyway,
+  // so the two loads have no branch between them. Lanes hold different
+  // faults, hence different gate types: the branch-free evaluation.
   const uint64_t v0 = good[int64_t(f.x) * J.RW + w];
-  const uint64_t v1 = nin > 1 ? good[int64_t(f.y) * J.RW + w] : 0;
+  const uint64_t v1 = good[int64_t(f.y) * J.RW + w];
   const uint64_t v2 = nin > 2 ? good[int64_t(f.z) * J.RW + w] : 0;
   const uint64_t v3 = nin > 3 ? good[int64_t(f.w) * J.RW + w] : 0;
-  const uint64_t r0 = eval_gate(type, nin, v0, v1, v2, v3);
-  const uint64_t r1 = eval_gate(type, nin, f.x == x ? ~v0 : v0, f.y == x ? ~v1 : v1, f.z == x ? ~v2 : v2,
-                                f.w == x ? ~v3 : v3);
+  const uint64_t r0 = eval_gate_sel(type, nin, v0, v1, v2, v3);
+  const uint64_t r1 = eval_gate_sel(type, nin, f.x == x ? ~v0 : v0, f.y == x ? ~v1 : v1, f.z == x ? ~v2 : v2,
+                                    f.w == x ? ~v3 : v3);
   return r0 ^ r1;
 }
 
