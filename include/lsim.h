@@ -310,7 +310,9 @@ LSIM_API int lsim_server_counters(lsim_server *s, int64_t *counters);
  * "big_evals" and "big_node_words": jobs with at least that many fault x
  * pattern evaluations (default 4e9), or circuit nodes x 64-pattern rounds
  * (default 5e7), run in the large-job lane, one at a time in arrival order
- * ("big_evals" 0 puts every job in the batching lane). */
+ * ("big_evals" 0 puts every job in the batching lane); "small_share_pct"
+ * (1..99, default 25): the batching lane's share of the sweep's workgroup
+ * slots while both lanes have work (the large-job lane takes the rest). */
 LSIM_API int lsim_server_set_option(lsim_server *s, const char *key, int64_t value);
 
 /* Seeded synthetic workload in the reference formats (no reference

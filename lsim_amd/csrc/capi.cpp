@@ -860,7 +860,9 @@ void lsim_server::loop(int l) {
         L.queue.pop_front();
       }
       eo = L.eopt;
-      eo.slot_share = lane_has_work(1 - l) ? L.busy_share : 1.0;
+      // The lane's share of the slots while the other lane has work, within
+      // the engine option's cap (slot_pct).
+      eo.slot_share = L.eopt.slot_share * (lane_has_work(1 - l) ? L.busy_share : 1.0);
     }
     std::vector<ServerJob *> done_now;
     std::vector<ServerJob *> failed_now;
@@ -1184,6 +1186,12 @@ int lsim_server_set_option(lsim_server *s, const char *key, int64_t value) {
   }
   if (k == "big_node_words") {
     s->big_node_words = double(value);
+    return LSIM_OK;
+  }
+  if (k == "small_share_pct") {
+    if (value < 1 || value > 99) return fail(&s->err, LSIM_E_ARG, "small_share_pct must be 1..99");
+    s->lane[0].busy_share = double(value) / 100.0;
+    s->lane[1].busy_share = 1.0 - s->lane[0].busy_share;
     return LSIM_OK;
   }
   bool known = false;
