@@ -109,6 +109,7 @@ struct JobDev {
   // circuit (device positions: live nodes first, then dead nodes)
   const uint32_t *info;      // type | fanin count << 4 | deepest live fanout level << 8
   const int4 *fin;           // fanin positions (PI: PI index in .x)
+  const int4 *nrec;          // narrow circuits: per live node {info, fanin 0, fanin 1, first live fanout}, NL + 1 entries
   const int32_t *live_lvl;   // first live position of each level, L+2 entries
   const int32_t *dead_lvl;   // first dead position of each level, L+2 entries
   const int32_t *fo_off;     // live fanouts of live nodes
@@ -185,6 +186,7 @@ struct SweepView {
   const LSIM_GLOBAL int32_t *__restrict__ live_lvl;
   const LSIM_GLOBAL int32_t *__restrict__ fo_off;
   const LSIM_GLOBAL gi2 *__restrict__ fo;
+  const LSIM_GLOBAL gi4 *__restrict__ nrec;
   int64_t RW;
 };
 
@@ -196,6 +198,7 @@ __device__ __forceinline__ SweepView sweep_view(const JobDev *jobs, int j) {
   v.live_lvl = (const LSIM_GLOBAL int32_t *)jobs[j].live_lvl;
   v.fo_off = (const LSIM_GLOBAL int32_t *)jobs[j].fo_off;
   v.fo = (const LSIM_GLOBAL gi2 *)jobs[j].fo;
+  v.nrec = (const LSIM_GLOBAL gi4 *)jobs[j].nrec;
   v.RW = jobs[j].RW;
   return v;
 }
@@ -1069,9 +1072,23 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
         // the loads are unconditional.
         const int uc = in ? u : lo;
         const uint32_t pf = nopend ? 0u : pend[uc];
-        const uint32_t inf = J.info[uc];
-        const int4 f = to_int4(J.fin[uc]);
-        const int fo0 = J.fo_off[uc], fo1 = J.fo_off[uc + 1];
+        uint32_t inf;
+        int4 f;
+        int fo0, fo1;
+        if constexpr (NIN == 2) {
+          // Narrow circuits: the node's record in one 16-byte load (the next
+          // record's fanout offset ends its range, mostly in the same lines).
+          const gi4 r = J.nrec[uc];
+          inf = uint32_t(r.x);
+          f = make_int4(r.y, r.z, 0, 0);
+          fo0 = r.w;
+          fo1 = J.nrec[uc + 1].w;
+        } else {
+          inf = J.info[uc];
+          f = to_int4(J.fin[uc]);
+          fo0 = J.fo_off[uc];
+          fo1 = J.fo_off[uc + 1];
+        }
         // Pins whose mark carries this tile's tag; without pending flags every
         // node of the level is a candidate, all its fanins' records are read,
         // and it is staged when some fanin differs in some lane or it injects.
@@ -2617,12 +2634,13 @@ struct CircuitRes {
   std::vector<uint8_t> head_is_po;  // per live device position
   DevBuf<uint32_t> info;
   DevBuf<int4> fin;
+  DevBuf<int4> nrec;  // narrow circuits: the pair sweep's per-node record (see JobDev::nrec)
   DevBuf<int32_t> parent, d_live_lvl, d_dead_lvl, fo_off;
   DevBuf<int2> fo;
   DevBuf<uint8_t> is_po;
   int64_t bytes() const {
-    return int64_t(info.bytes() + fin.bytes() + parent.bytes() + d_live_lvl.bytes() + d_dead_lvl.bytes() +
-                   fo_off.bytes() + fo.bytes() + is_po.bytes());
+    return int64_t(info.bytes() + fin.bytes() + nrec.bytes() + parent.bytes() + d_live_lvl.bytes() +
+                   d_dead_lvl.bytes() + fo_off.bytes() + fo.bytes() + is_po.bytes());
   }
 };
 
@@ -3257,6 +3275,15 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
 
   c.info.alloc(N);
   c.fin.alloc(N);
+  std::vector<int4> nrec;
+  if (c.narrow) {
+    // One 16-byte load per scanned node in the pair sweep instead of four.
+    nrec.resize(size_t(c.NL) + 1);
+    for (int64_t u = 0; u < c.NL; ++u)
+      nrec[size_t(u)] = make_int4(int(info[size_t(u)]), fin[size_t(u)].x, fin[size_t(u)].y, fo_off[size_t(u)]);
+    nrec[size_t(c.NL)] = make_int4(0, 0, 0, fo_off[size_t(c.NL)]);
+    c.nrec.alloc(nrec.size());
+  }
   c.parent.alloc(nl1);
   c.d_live_lvl.alloc(c.live_lvl.size());
   c.d_dead_lvl.alloc(c.dead_lvl.size());
@@ -3265,6 +3292,7 @@ CircuitRes *Engine::add_circuit(const Netlist &nl, const FaultStructure &fs) {
   c.is_po.alloc(nl1);
   m.h2d(c.info.p, info.data(), N * sizeof(uint32_t));
   m.h2d(c.fin.p, fin.data(), N * sizeof(int4));
+  if (!nrec.empty()) m.h2d(c.nrec.p, nrec.data(), nrec.size() * sizeof(int4));
   m.h2d(c.parent.p, parent.data(), parent.size() * sizeof(int32_t));
   m.h2d(c.d_live_lvl.p, c.live_lvl.data(), c.live_lvl.size() * sizeof(int32_t));
   m.h2d(c.d_dead_lvl.p, c.dead_lvl.data(), c.dead_lvl.size() * sizeof(int32_t));
@@ -3305,6 +3333,7 @@ CircuitRes *Engine::share_circuit(const CircuitRes *src) {
   auto share = [](auto &dst, const auto &from) { dst.view(from.p, from.n); };
   share(c.info, src->info);
   share(c.fin, src->fin);
+  share(c.nrec, src->nrec);
   share(c.parent, src->parent);
   share(c.d_live_lvl, src->d_live_lvl);
   share(c.d_dead_lvl, src->d_dead_lvl);
@@ -3533,6 +3562,7 @@ JobDev Engine::Impl::job_dev(JobState &j) const {
   std::memset(&d, 0, sizeof(d));
   d.info = c.info.p;
   d.fin = c.fin.p;
+  d.nrec = c.nrec.p;
   d.live_lvl = c.d_live_lvl.p;
   d.dead_lvl = c.d_dead_lvl.p;
   d.fo_off = c.fo_off.p;
