@@ -816,7 +816,7 @@ constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
 constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
 constexpr int kDenseNodesOcc = 2;   // the same at more than four waves per SIMD
-constexpr int kPairsPerThreadOcc = 1;
+constexpr int kPairsPerThreadOcc = 2;
 
 // Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
