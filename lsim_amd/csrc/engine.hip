@@ -263,6 +263,32 @@ __device__ __forceinline__ uint64_t eval_gate_sel(uint32_t type, uint32_t nin, u
   return inv ? ~r : r;
 }
 
+// Gates of one or two inputs as 4-bit truth tables: bit 2a + b is the output
+// for bit a on pin 0 and bit b on pin 1 (a one-input gate's table does not
+// depend on b). Four bits per type code, for two inputs and for one; the same
+// functions as eval_gate.
+__host__ __device__ constexpr uint64_t tt_entry(int type, uint32_t table) { return uint64_t(table) << (4 * type); }
+constexpr uint64_t kGateTT2 = tt_entry(G_INV, 0x3) | tt_entry(G_AND, 0x8) | tt_entry(G_OR, 0xe) |
+                              tt_entry(G_XOR, 0x6) | tt_entry(G_NAND, 0x7) | tt_entry(G_NOR, 0x1) |
+                              tt_entry(G_XNOR, 0x9) | tt_entry(G_CLKBUF, 0xc) | tt_entry(G_PO, 0xc);
+constexpr uint64_t kGateTT1 = tt_entry(G_INV, 0x3) | tt_entry(G_AND, 0xc) | tt_entry(G_OR, 0xc) |
+                              tt_entry(G_XOR, 0xc) | tt_entry(G_NAND, 0x3) | tt_entry(G_NOR, 0x3) |
+                              tt_entry(G_XNOR, 0x3) | tt_entry(G_CLKBUF, 0xc) | tt_entry(G_PO, 0xc);
+
+// Truth table of a gate of at most two inputs (not MUX, not PI).
+__device__ __forceinline__ uint32_t gate_tt(uint32_t type, uint32_t nin) {
+  return uint32_t(((nin > 1 ? kGateTT2 : kGateTT1) >> (4 * type)) & 15u);
+}
+
+// The table applied bitwise: three bit selects, no branch, whether the table
+// is uniform over the wave (scalar masks) or not.
+__device__ __forceinline__ uint64_t eval_tt(uint32_t tt, uint64_t a, uint64_t b) {
+  const uint64_t t0 = 0 - uint64_t(tt & 1u), t1 = 0 - uint64_t((tt >> 1) & 1u);
+  const uint64_t t2 = 0 - uint64_t((tt >> 2) & 1u), t3 = 0 - uint64_t(tt >> 3);
+  const uint64_t hi = (b & t3) | (~b & t2), lo = (b & t1) | (~b & t0);
+  return (a & hi) | (~a & lo);
+}
+
 __device__ __forceinline__ uint64_t round_mask(int64_t num_patterns, int64_t w) {
   const int64_t bits = (num_patterns / (64 * (w + 1))) ? 64 : (num_patterns % 64);
   return bits < 64 ? ((uint64_t(1) << bits) - 1) : ~uint64_t(0);
@@ -1273,9 +1299,9 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
 #pragma unroll
               for (int i = 0; i < DN; ++i) {
                 if (!has[i]) continue;
-                // The type is uniform: a switch on it outside the lane-dependent
-                // condition is a scalar branch.
-                const uint64_t e = eval_gate(type[i], nin[i], vals[i][0], vals[i][1], 0, 0);
+                // The type is uniform: its truth table is computed on the
+                // scalar unit and applied with bit selects, no branch.
+                const uint64_t e = eval_tt(gate_tt(type[i], nin[i]), vals[i][0], vals[i][1]);
                 const uint64_t r = (on[i] && type[i] != G_PI) ? e : gu[i];
                 dense_out(kk[i], v[i], type[i], nin[i], on[i], gu[i], r);
               }
@@ -1359,7 +1385,7 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
               if constexpr (NIN > 2)
                 e = eval_gate_sel(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]);
               else
-                e = eval_gate_sel2(type, nin, vals[i][0], vals[i][1]);
+                e = eval_tt(gate_tt(type, nin), vals[i][0], vals[i][1]);
               uint64_t r = type != G_PI ? e : gu[i];
               if (s_lsite[b] == v) {
                 const uint32_t kind = uint32_t(s_lkind[b]);
