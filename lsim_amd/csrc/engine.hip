@@ -283,10 +283,13 @@ __device__ __forceinline__ uint32_t gate_tt(uint32_t type, uint32_t nin) {
 // The table applied bitwise: three bit selects, no branch, whether the table
 // is uniform over the wave (scalar masks) or not.
 __device__ __forceinline__ uint64_t eval_tt(uint32_t tt, uint64_t a, uint64_t b) {
-  const uint64_t t0 = 0 - uint64_t(tt & 1u), t1 = 0 - uint64_t((tt >> 1) & 1u);
-  const uint64_t t2 = 0 - uint64_t((tt >> 2) & 1u), t3 = 0 - uint64_t(tt >> 3);
-  const uint64_t hi = (b & t3) | (~b & t2), lo = (b & t1) | (~b & t0);
-  return (a & hi) | (~a & lo);
+  // 32-bit masks, applied to each half.
+  const uint32_t t0 = 0u - (tt & 1u), t1 = 0u - ((tt >> 1) & 1u), t2 = 0u - ((tt >> 2) & 1u), t3 = 0u - (tt >> 3);
+  auto half = [&](uint32_t x, uint32_t y) {
+    const uint32_t hi = (y & t3) | (~y & t2), lo = (y & t1) | (~y & t0);
+    return (x & hi) | (~x & lo);
+  };
+  return uint64_t(half(uint32_t(a), uint32_t(b))) | (uint64_t(half(uint32_t(a >> 32), uint32_t(b >> 32))) << 32);
 }
 
 __device__ __forceinline__ uint64_t round_mask(int64_t num_patterns, int64_t w) {
@@ -849,6 +852,15 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
+// An empty use that makes every path wait here for the load that produced x.
+// The vector memory counter is in order over loads and stores: a load left
+// pending on some path (its value unused there) makes the compiler wait for
+// it before the next write of its register, at the head of the next loop
+// iteration, and that wait also waits for every store issued in between.
+// Settled after their batch of loads, the registers are free again and the
+// next batch of loads goes out while the stores are still in flight.
+__device__ __forceinline__ void settle(uint64_t x) { asm volatile("" ::"v"(x)); }
+
 // Marks the fanout of live-fanout entry f pending on the pins that read the
 // written node: a byte store of the tile's tag per pin (pins != 0).
 __device__ __forceinline__ void mark_pins(uint32_t *pend, int2 f, uint32_t tag) {
@@ -1296,6 +1308,14 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
                   if (kUncond || (on[i] && type[i] != G_PI && j < int(nin[i]))) vals[i][j] = *src;
                 }
               }
+              // All of the group's words, on every path (a group short of
+              // DN nodes skips some evaluations), before its stores.
+#pragma unroll
+              for (int i = 0; i < DN; ++i) {
+                settle(gu[i]);
+                settle(vals[i][0]);
+                settle(vals[i][1]);
+              }
 #pragma unroll
               for (int i = 0; i < DN; ++i) {
                 if (!has[i]) continue;
@@ -1384,6 +1404,8 @@ __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVE
               uint64_t e;
               if constexpr (NIN > 2)
                 e = eval_gate_sel(type, nin, vals[i][0], vals[i][1], vals[i][2], vals[i][3]);
+              else if constexpr (WAVES > 4)
+                e = eval_gate_sel2(type, nin, vals[i][0], vals[i][1]);  // fewer registers at five waves
               else
                 e = eval_tt(gate_tt(type, nin), vals[i][0], vals[i][1]);
               uint64_t r = type != G_PI ? e : gu[i];
