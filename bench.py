@@ -267,13 +267,14 @@ def batching_leg(args, steps: int, warmup: int) -> dict:
     F_tot = sum(2 * sp["num_gates"] for sp in specs)
     evals = sum(2 * sp["num_gates"] * sp["num_patterns"] for sp in specs)
     times, agg, detected, st = [], None, 0, None
-    it, timed_runs = 0, 0
+    it, timed_runs, kept_runs = 0, 0, 0
     while timed_runs < steps:
-        # Timed once the warmup is done and, with the autotuner, once the
-        # previous run reported every candidate measured (it then keeps the
-        # fastest).
+        # Timed once the warmup is done and, with the autotuner, once a run
+        # has reported every candidate measured (it then keeps the fastest)
+        # and one run with the kept configuration has captured its step graphs.
         explored = st is not None and st["tune_trials"] >= st["tune_candidates"]
-        timed = it >= warmup and (not args.c4_autotune or explored or it > 60)
+        timed = it >= warmup and (not args.c4_autotune or kept_runs >= 1 or it > 60)
+        kept_runs += 1 if explored else 0
         ids = [batch.add_job(c, f, p, drop=True) for c, (f, p) in zip(cids, jobs_text)]
         t0 = time.perf_counter()
         batch.run()

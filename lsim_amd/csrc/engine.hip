@@ -842,7 +842,7 @@ __global__ void __launch_bounds__(kChunk) __attribute__((amdgpu_waves_per_eu(6))
 // the tile's epoch, so nothing is cleared between tiles.
 constexpr int kTileStats = 13;  // sweep statistics counters (LSIM_TRACE)
 constexpr int kPairsPerThread = 2;
-constexpr int kFanoutsPerLane = 4;  // pair sweep: fanouts per lane marked at a time
+constexpr int kFanoutsPerLane = 2;  // pair sweep: fanouts per lane marked at a time
 constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
 constexpr int kDenseNodesOcc = 2;   // the same at more than four waves per SIMD
 constexpr int kPairsPerThreadOcc = 2;
