@@ -93,6 +93,59 @@ def test_c1_full_against_oracle(c1, mode, direct):
     assert np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
 
 
+def _gate_forms_texts():
+    """Every gate type of at most two inputs with one and with two wires:
+    AND/OR/XOR and their inversions with a single input (the fold of one
+    word), INV/CLKBUF with a second, unread pin. Two layers over four PIs,
+    stems that reach several POs, SA0/SA1 on every node, three rounds with a
+    partial last one."""
+    P = 4
+    types1 = [0, 1, 2, 3, 4, 5, 6, 8]
+    layer1 = [(t, k) for t in types1 for k in (1, 2)]
+    layer2 = [(t, 2) for t in (1, 2, 3, 4, 5, 6)] + [(t, 1) for t in types1]
+    taps = [0, 5, 10, 15]  # layer-1 gates that also drive a PO
+    O = len(layer2) + len(taps)
+    G = len(layer1) + len(layer2)
+    g1 = [P + O + j for j in range(len(layer1))]
+    g2 = [P + O + len(layer1) + m for m in range(len(layer2))]
+    wires = []  # (src, dst, pin)
+    for j, (t, k) in enumerate(layer1):
+        for pin, src in enumerate([j % P, (j + 1) % P][:k], 1):
+            wires.append((src, g1[j], pin))
+    for m, (t, k) in enumerate(layer2):
+        for pin, src in enumerate([g1[m % len(g1)], g1[(5 * m + 3) % len(g1)]][:k], 1):
+            wires.append((src, g2[m], pin))
+    for i in range(len(layer2)):
+        wires.append((g2[i], P + i, 1))
+    for i, j in enumerate(taps):
+        wires.append((g1[j], P + len(layer2) + i, 1))
+    recs = " ".join(f"{a} 0 1 {b} {pin}" for a, b, pin in wires)
+    ckt = f"{P} {O} {G} {len(wires)} {recs} " + " ".join(str(t) for t, _ in layer1 + layer2)
+    N = P + O + G
+    flst = f"{2 * N} " + " ".join(f"{v} {sa}" for v in range(N) for sa in (0, 1))
+    rng = np.random.default_rng(7)
+    words = rng.integers(0, 2**63, size=(3, P), dtype=np.int64)
+    ptn = f"{3 * 64 - 5}\n" + "\n".join(" ".join(str(int(w)) for w in row) for row in words) + "\n"
+    return ckt.encode(), flst.encode(), ptn.encode()
+
+
+@pytest.mark.parametrize("direct", [0, 1], ids=["stem_tracing", "direct"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_gate_forms_against_oracle(mode, direct):
+    """One- and two-wire forms of every gate type (the sweep evaluates gates of
+    at most two inputs from 4-bit truth tables): every d(f, r), the literal
+    report and the last round's good values against the restatement."""
+    ckt, flst, ptn = _gate_forms_texts()
+    ref = oracle.Oracle(ckt, flst, ptn).run_literal(threads=1)
+    s = _sim(ckt, flst, ptn, mode, direct=direct)
+    s.run(mode[0])
+    assert np.array_equal(s.detect_matrix(), ref["det"])
+    assert np.array_equal(s.found_fault_to_pattern(), ref["table"])
+    pi, gate, po = s.good_results()
+    assert np.array_equal(gate, ref["g_gate"]) and np.array_equal(po, ref["g_po"])
+    assert ref["det"].any(), "no fault detected: the circuit does not exercise the sweep"
+
+
 def test_reference_modes_run_persistent_engine(c1):
     """The reference's three modes (fsim/simulation.hpp:45) all run the
     persistent pair sweep and give identical, oracle-exact results; the
