@@ -94,7 +94,7 @@ LSIM_API int lsim_read_buffers(lsim_sim *s, const char *ckt, size_t ckt_len, con
  *                            default), 2 every step.
  *  "pair_nopend_pct"    0..100: with pair_nopend 1, a step after one that staged
  *                            more than this percentage of the nodes its tiles
- *                            walked also runs without pending flags (default 45).
+ *                            walked also runs without pending flags (default 65).
  *  "pair_fo_pre"        0/1  pair sweep with pending flags: a staged node's first
  *                            live fanout is loaded beside its fanin records, so a
  *                            node with one live fanout marks it pending without a

@@ -81,7 +81,7 @@ struct EngineOptions {
   int64_t pool_cap = 0;              // pair sweep: at most this many pool words per slot (0 = as memory allows)
   int pair_dense = 24;               // pair sweep: passes averaging this many active lanes per node go one node per wave (0: never)
   int pair_nopend = 1;               // pair sweep without pending flags: 0 never, 1 dense steps, 2 every step
-  int pair_nopend_pct = 45;          // with pair_nopend 1: also after a step that staged more than this % of walked nodes
+  int pair_nopend_pct = 65;          // with pair_nopend 1: also after a step that staged more than this % of walked nodes
   bool pair_fo_pre = true;           // pair sweep with pending flags: load a staged node's first fanout in the scan
   int max_step_words = 128;          // with dropping: most words of one step (the growth stops there)
   double slot_share = 1.0;           // pair sweep: use at most this share of the resident workgroup slots
