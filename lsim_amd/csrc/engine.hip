@@ -844,7 +844,7 @@ constexpr int kTileStats = 13;  // sweep statistics counters (LSIM_TRACE)
 constexpr int kPairsPerThread = 2;
 constexpr int kFanoutsPerLane = 2;  // pair sweep: fanouts per lane marked at a time
 constexpr int kDenseNodes = 3;      // pair sweep dense passes: nodes per wave in flight
-constexpr int kDenseNodesOcc = 3;   // the same at more than four waves per SIMD
+constexpr int kDenseNodesOcc = 4;   // the same at more than four waves per SIMD
 constexpr int kPairsPerThreadOcc = 2;
 
 // Set bits of m below the calling lane (popcount of m & (2^lane - 1)).
