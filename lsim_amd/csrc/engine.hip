@@ -995,7 +995,8 @@ template <bool kStats, int CHUNK, int NIN, int WAVES>
 __global__ void __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(WAVES))) k_sweep_pairs(PairArgs a) {
   static_assert(NIN == 2 || NIN == 4, "fanin slots per node: 2 or 4");
   // Nodes per wave in flight in dense passes and pairs per thread in
-  // flattened passes: fewer where more waves share a SIMD's registers.
+  // flattened passes, with constants of their own for more than four waves
+  // per SIMD (where the registers are scarcer).
   constexpr int DN = WAVES > 4 ? kDenseNodesOcc : (NIN == 2 ? 4 : kDenseNodes);
   constexpr int PPT = WAVES > 4 ? kPairsPerThreadOcc : kPairsPerThread;
   // Loads of pair passes without a branch around them (measured faster for
